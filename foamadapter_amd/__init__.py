@@ -1,0 +1,13 @@
+"""foamadapter_amd -- MI355X-native finite-volume hot path of FoamAdapter/NeoN.
+
+The numerical work runs in libfvhip.so (hand-written HIP kernels for gfx950
+behind the C ABI in include/fvhip.h).  This package is the thin Python host
+layer over that ABI, named after NeoN's fvcc / la / dsl interfaces.
+"""
+from ._lib import FvError, LIB_PATH, lib  # noqa: F401  (fails loudly if the library is missing)
+from .runtime import Context, DeviceArray  # noqa: F401
+from .mesh import BoundaryConditions, Mesh, PolyMesh, read_polymesh  # noqa: F401
+from . import fvcc  # noqa: F401
+
+__all__ = ["Context", "DeviceArray", "Mesh", "PolyMesh", "BoundaryConditions", "read_polymesh",
+           "fvcc", "FvError", "LIB_PATH"]

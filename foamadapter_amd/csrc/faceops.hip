@@ -1,0 +1,399 @@
+// faceops.hip -- explicit finite-volume face operators (NeoN fvcc:
+// SurfaceInterpolation, GaussGreenGrad, GaussGreenDiv, GaussGreenLaplacian,
+// FaceNormalGradient; FoamAdapter::flux).
+//
+// Cell-result operators (grad, div, laplacian) are computed as a deterministic
+// cell gather: each cell walks its faces in ascending face order (the cell->face
+// stencil), recomputes the face value from the face stream and the cell on the
+// other side, and accumulates it with the owner (+) / neighbour (-) sign, then
+// the boundary faces, then divides by the volume.  That is the serial
+// accumulation order of OpenFOAM's surfaceIntegrate / gaussGrad::gradf, so no
+// float atomics are used and the result does not depend on scheduling.
+// Face-result operators (interpolate, snGrad, flux) are one coalesced pass
+// over the faces.
+#include "mesh.hpp"
+
+using namespace fvhip;
+
+namespace {
+
+constexpr int kBlock = 256;
+
+template <typename S>
+struct Geo {
+    const S *V, *Sf, *magSf, *w, *nodc, *Sfb, *magSfb, *dcb;
+};
+
+Geo<double> geo64(const fv_mesh* m) {
+    return {m->V.p, m->Sf.p, m->magSf.p, m->w.p, m->nodc.p, m->Sfb.p, m->magSfb.p, m->dcb.p};
+}
+Geo<float> geo32(fv_mesh* m) {
+    ensure_f32(m);
+    auto& g = m->f32;
+    return {g.V.p, g.Sf.p, g.magSf.p, g.w.p, g.nodc.p, g.Sfb.p, g.magSfb.p, g.dcb.p};
+}
+
+// ---- cell-result operators -------------------------------------------------
+
+template <typename S>
+__global__ __launch_bounds__(kBlock) void k_grad(int N, int F, const int32_t* __restrict__ off,
+                                                 const int32_t* __restrict__ idx,
+                                                 const int32_t* __restrict__ oth, Geo<S> g,
+                                                 const S* __restrict__ T, const S* __restrict__ Tb,
+                                                 S coeff, S* __restrict__ out) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= N) return;
+    const S Tc = T[c];
+    S ax = 0, ay = 0, az = 0;
+    const int e1 = off[c + 1];
+    for (int k = off[c]; k < e1; ++k) {
+        const int e = idx[k];
+        if (e >= F) {
+            const int b = e - F;
+            const S tb = Tb[b];
+            ax += g.Sfb[3 * b] * tb;
+            ay += g.Sfb[3 * b + 1] * tb;
+            az += g.Sfb[3 * b + 2] * tb;
+        } else {
+            const S To = T[oth[k]];
+            if (e >= 0) {
+                const S tf = g.w[e] * (Tc - To) + To;
+                ax += g.Sf[3 * (int64_t)e] * tf;
+                ay += g.Sf[3 * (int64_t)e + 1] * tf;
+                az += g.Sf[3 * (int64_t)e + 2] * tf;
+            } else {
+                const int f = ~e;
+                const S tf = g.w[f] * (To - Tc) + Tc;
+                ax -= g.Sf[3 * (int64_t)f] * tf;
+                ay -= g.Sf[3 * (int64_t)f + 1] * tf;
+                az -= g.Sf[3 * (int64_t)f + 2] * tf;
+            }
+        }
+    }
+    const S v = g.V[c];
+    out[3 * (int64_t)c] = coeff * (ax / v);
+    out[3 * (int64_t)c + 1] = coeff * (ay / v);
+    out[3 * (int64_t)c + 2] = coeff * (az / v);
+}
+
+// scheme: 0 linear, 1 upwind; T == nullptr: surfaceIntegrate(phi)
+template <typename S>
+__global__ __launch_bounds__(kBlock) void k_div(int N, int F, int scheme,
+                                                const int32_t* __restrict__ off,
+                                                const int32_t* __restrict__ idx,
+                                                const int32_t* __restrict__ oth, Geo<S> g,
+                                                const S* __restrict__ phi, const S* __restrict__ T,
+                                                const S* __restrict__ Tb, S coeff,
+                                                S* __restrict__ out) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= N) return;
+    S acc = 0;
+    const int e1 = off[c + 1];
+    if (T) {
+        const S Tc = T[c];
+        for (int k = off[c]; k < e1; ++k) {
+            const int e = idx[k];
+            if (e >= F) {
+                const int b = e - F;
+                acc += phi[e] * Tb[b];
+            } else {
+                const S To = T[oth[k]];
+                const int f = e >= 0 ? e : ~e;
+                const S ph = phi[f];
+                const S lam = scheme == 0 ? g.w[f] : (ph >= S(0) ? S(1) : S(0));
+                // owner side: P = this cell; neighbour side: P = other cell
+                const S TP = e >= 0 ? Tc : To, TN = e >= 0 ? To : Tc;
+                const S v = ph * (lam * (TP - TN) + TN);
+                if (e >= 0) acc += v; else acc -= v;
+            }
+        }
+    } else {
+        for (int k = off[c]; k < e1; ++k) {
+            const int e = idx[k];
+            if (e >= F) acc += phi[e];
+            else if (e >= 0) acc += phi[e];
+            else acc -= phi[~e];
+        }
+    }
+    out[c] = coeff * (acc / g.V[c]);
+}
+
+template <typename S>
+__device__ inline S bnd_sngrad(int8_t kind, S dcb, S Tb, S TP, const S* Tg, int b) {
+    if (kind == FV_BC_FIXED_GRADIENT) return Tg ? Tg[b] : S(0);
+    return dcb * (Tb - TP);
+}
+
+template <typename S>
+__global__ __launch_bounds__(kBlock) void k_laplacian(
+    int N, int F, const int32_t* __restrict__ off, const int32_t* __restrict__ idx,
+    const int32_t* __restrict__ oth, Geo<S> g, const int8_t* __restrict__ kind,
+    const S* __restrict__ gamma, const S* __restrict__ T, const S* __restrict__ Tb,
+    const S* __restrict__ Tg, S coeff, S* __restrict__ out) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= N) return;
+    const S Tc = T[c];
+    S acc = 0;
+    const int e1 = off[c + 1];
+    for (int k = off[c]; k < e1; ++k) {
+        const int e = idx[k];
+        if (e >= F) {
+            const int b = e - F;
+            const S sn = bnd_sngrad<S>(kind[b], g.dcb[b], Tb[b], Tc, Tg, b);
+            acc += (gamma[e] * g.magSfb[b]) * sn;
+        } else {
+            const S To = T[oth[k]];
+            const int f = e >= 0 ? e : ~e;
+            const S TP = e >= 0 ? Tc : To, TN = e >= 0 ? To : Tc;
+            const S v = (gamma[f] * g.magSf[f]) * (g.nodc[f] * (TN - TP));
+            if (e >= 0) acc += v; else acc -= v;
+        }
+    }
+    out[c] = coeff * (acc / g.V[c]);
+}
+
+// ---- face-result operators -------------------------------------------------
+
+template <typename S>
+__global__ __launch_bounds__(kBlock) void k_interp(int F, int B, int scheme,
+                                                   const int32_t* __restrict__ own,
+                                                   const int32_t* __restrict__ nbr,
+                                                   const S* __restrict__ w,
+                                                   const S* __restrict__ phi,
+                                                   const S* __restrict__ T,
+                                                   const S* __restrict__ Tb, S* __restrict__ Tf) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < F) {
+        const S lam = scheme == 0 ? w[i] : (phi[i] >= S(0) ? S(1) : S(0));
+        const S TN = T[nbr[i]];
+        Tf[i] = lam * (T[own[i]] - TN) + TN;
+    } else if (i < F + B) {
+        Tf[i] = Tb[i - F];
+    }
+}
+
+template <typename S>
+__global__ __launch_bounds__(kBlock) void k_sngrad(int F, int B, const int32_t* __restrict__ own,
+                                                   const int32_t* __restrict__ nbr,
+                                                   const int32_t* __restrict__ fc,
+                                                   const S* __restrict__ nodc,
+                                                   const S* __restrict__ dcb,
+                                                   const int8_t* __restrict__ kind,
+                                                   const S* __restrict__ T,
+                                                   const S* __restrict__ Tb,
+                                                   const S* __restrict__ Tg, S* __restrict__ out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < F) {
+        out[i] = nodc[i] * (T[nbr[i]] - T[own[i]]);
+    } else if (i < F + B) {
+        const int b = i - F;
+        out[i] = bnd_sngrad<S>(kind[b], dcb[b], Tb[b], T[fc[b]], Tg, b);
+    }
+}
+
+template <typename S>
+__global__ __launch_bounds__(kBlock) void k_flux(int F, int B, const int32_t* __restrict__ own,
+                                                 const int32_t* __restrict__ nbr,
+                                                 const S* __restrict__ w, const S* __restrict__ Sf,
+                                                 const S* __restrict__ Sfb,
+                                                 const S* __restrict__ U, const S* __restrict__ Ub,
+                                                 S* __restrict__ phi) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < F) {
+        const int64_t o = own[i], n = nbr[i];
+        const S wf = w[i];
+        const S ux = wf * (U[3 * o] - U[3 * n]) + U[3 * n];
+        const S uy = wf * (U[3 * o + 1] - U[3 * n + 1]) + U[3 * n + 1];
+        const S uz = wf * (U[3 * o + 2] - U[3 * n + 2]) + U[3 * n + 2];
+        phi[i] = Sf[3 * (int64_t)i] * ux + Sf[3 * (int64_t)i + 1] * uy + Sf[3 * (int64_t)i + 2] * uz;
+    } else if (i < F + B) {
+        const int64_t b = i - F;
+        phi[i] = Sfb[3 * b] * Ub[3 * b] + Sfb[3 * b + 1] * Ub[3 * b + 1] + Sfb[3 * b + 2] * Ub[3 * b + 2];
+    }
+}
+
+// fixedGradient patches: Tb = T_P + g/deltaCoeffs (fixedGradientFvPatchField::evaluate);
+// fixedValue and calculated faces keep their stored value
+template <typename S>
+__global__ __launch_bounds__(kBlock) void k_correct_bc(int B, const int32_t* __restrict__ fc,
+                                                       const double* __restrict__ dcb,
+                                                       const int8_t* __restrict__ kind,
+                                                       const S* __restrict__ T,
+                                                       const S* __restrict__ Tg,
+                                                       S* __restrict__ Tb) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= B || kind[b] != FV_BC_FIXED_GRADIENT) return;
+    const S g = Tg ? Tg[b] : S(0);
+    Tb[b] = T[fc[b]] + g / (S)dcb[b];
+}
+
+void check_common(fv_ctx* ctx, const fv_mesh* m, fv_prec prec) {
+    FV_REQUIRE(ctx && m, "NULL ctx/mesh");
+    FV_REQUIRE(m->ctx == ctx, "mesh belongs to another context");
+    FV_REQUIRE(prec == FV_F64 || prec == FV_F32, "unknown precision");
+}
+
+void check_bc(const fv_mesh* m, const fv_bc* bc) {
+    FV_REQUIRE(bc, "boundary conditions are NULL");
+    FV_REQUIRE(bc->mesh == m, "boundary conditions belong to another mesh");
+}
+
+}  // namespace
+
+extern "C" {
+
+int fv_interpolate(fv_ctx* ctx, const fv_mesh* mc, fv_scheme scheme, fv_prec prec,
+                   const void* phi, const void* T, const void* Tb, void* Tf) {
+    return guarded([&] {
+        fv_mesh* m = const_cast<fv_mesh*>(mc);
+        check_common(ctx, m, prec);
+        FV_REQUIRE(scheme == FV_LINEAR || scheme == FV_UPWIND, "unknown scheme");
+        FV_REQUIRE(T && Tf && (Tb || m->nB == 0), "NULL field");
+        FV_REQUIRE(scheme == FV_LINEAR || phi, "upwind interpolation needs phi");
+        const int n = m->nInt + m->nB;
+        if (!n) return;
+        if (prec == FV_F64) {
+            k_interp<double><<<grid_for(n, kBlock), kBlock, 0, ctx->stream>>>(
+                m->nInt, m->nB, scheme, m->own.p, m->nbr.p, m->w.p, (const double*)phi,
+                (const double*)T, (const double*)Tb, (double*)Tf);
+        } else {
+            auto g = geo32(m);
+            k_interp<float><<<grid_for(n, kBlock), kBlock, 0, ctx->stream>>>(
+                m->nInt, m->nB, scheme, m->own.p, m->nbr.p, g.w, (const float*)phi,
+                (const float*)T, (const float*)Tb, (float*)Tf);
+        }
+        FV_LAUNCH_CHECK();
+    });
+}
+
+int fv_grad_gauss(fv_ctx* ctx, const fv_mesh* mc, fv_prec prec, double coeff, const void* T,
+                  const void* Tb, void* out) {
+    return guarded([&] {
+        fv_mesh* m = const_cast<fv_mesh*>(mc);
+        check_common(ctx, m, prec);
+        FV_REQUIRE(T && out && (Tb || m->nB == 0), "NULL field");
+        const int N = m->nCells;
+        if (prec == FV_F64)
+            k_grad<double><<<grid_for(N, kBlock), kBlock, 0, ctx->stream>>>(
+                N, m->nInt, m->cfOff.p, m->cfIdx.p, m->cfOther.p, geo64(m), (const double*)T,
+                (const double*)Tb, coeff, (double*)out);
+        else
+            k_grad<float><<<grid_for(N, kBlock), kBlock, 0, ctx->stream>>>(
+                N, m->nInt, m->cfOff.p, m->cfIdx.p, m->cfOther.p, geo32(m), (const float*)T,
+                (const float*)Tb, (float)coeff, (float*)out);
+        FV_LAUNCH_CHECK();
+    });
+}
+
+int fv_div_gauss(fv_ctx* ctx, const fv_mesh* mc, fv_scheme scheme, fv_prec prec, double coeff,
+                 const void* phi, const void* T, const void* Tb, void* out) {
+    return guarded([&] {
+        fv_mesh* m = const_cast<fv_mesh*>(mc);
+        check_common(ctx, m, prec);
+        FV_REQUIRE(scheme == FV_LINEAR || scheme == FV_UPWIND, "unknown scheme");
+        FV_REQUIRE(phi && out, "NULL field");
+        FV_REQUIRE(!T || Tb || m->nB == 0, "boundary values are NULL");
+        const int N = m->nCells;
+        if (prec == FV_F64)
+            k_div<double><<<grid_for(N, kBlock), kBlock, 0, ctx->stream>>>(
+                N, m->nInt, scheme, m->cfOff.p, m->cfIdx.p, m->cfOther.p, geo64(m),
+                (const double*)phi, (const double*)T, (const double*)Tb, coeff, (double*)out);
+        else
+            k_div<float><<<grid_for(N, kBlock), kBlock, 0, ctx->stream>>>(
+                N, m->nInt, scheme, m->cfOff.p, m->cfIdx.p, m->cfOther.p, geo32(m),
+                (const float*)phi, (const float*)T, (const float*)Tb, (float)coeff, (float*)out);
+        FV_LAUNCH_CHECK();
+    });
+}
+
+int fv_laplacian_uncorrected(fv_ctx* ctx, const fv_mesh* mc, fv_prec prec, double coeff,
+                             const void* gamma, const void* T, const fv_bc* bc, const void* Tb,
+                             const void* Tg, void* out) {
+    return guarded([&] {
+        fv_mesh* m = const_cast<fv_mesh*>(mc);
+        check_common(ctx, m, prec);
+        check_bc(m, bc);
+        FV_REQUIRE(gamma && T && out && (Tb || m->nB == 0), "NULL field");
+        const int N = m->nCells;
+        if (prec == FV_F64)
+            k_laplacian<double><<<grid_for(N, kBlock), kBlock, 0, ctx->stream>>>(
+                N, m->nInt, m->cfOff.p, m->cfIdx.p, m->cfOther.p, geo64(m), bc->faceKind.p,
+                (const double*)gamma, (const double*)T, (const double*)Tb, (const double*)Tg,
+                coeff, (double*)out);
+        else
+            k_laplacian<float><<<grid_for(N, kBlock), kBlock, 0, ctx->stream>>>(
+                N, m->nInt, m->cfOff.p, m->cfIdx.p, m->cfOther.p, geo32(m), bc->faceKind.p,
+                (const float*)gamma, (const float*)T, (const float*)Tb, (const float*)Tg,
+                (float)coeff, (float*)out);
+        FV_LAUNCH_CHECK();
+    });
+}
+
+int fv_sngrad_uncorrected(fv_ctx* ctx, const fv_mesh* mc, fv_prec prec, const void* T,
+                          const fv_bc* bc, const void* Tb, const void* Tg, void* out) {
+    return guarded([&] {
+        fv_mesh* m = const_cast<fv_mesh*>(mc);
+        check_common(ctx, m, prec);
+        check_bc(m, bc);
+        FV_REQUIRE(T && out && (Tb || m->nB == 0), "NULL field");
+        const int n = m->nInt + m->nB;
+        if (!n) return;
+        if (prec == FV_F64)
+            k_sngrad<double><<<grid_for(n, kBlock), kBlock, 0, ctx->stream>>>(
+                m->nInt, m->nB, m->own.p, m->nbr.p, m->faceCells.p, m->nodc.p, m->dcb.p,
+                bc->faceKind.p, (const double*)T, (const double*)Tb, (const double*)Tg,
+                (double*)out);
+        else {
+            auto g = geo32(m);
+            k_sngrad<float><<<grid_for(n, kBlock), kBlock, 0, ctx->stream>>>(
+                m->nInt, m->nB, m->own.p, m->nbr.p, m->faceCells.p, g.nodc, g.dcb,
+                bc->faceKind.p, (const float*)T, (const float*)Tb, (const float*)Tg,
+                (float*)out);
+        }
+        FV_LAUNCH_CHECK();
+    });
+}
+
+int fv_flux(fv_ctx* ctx, const fv_mesh* mc, fv_prec prec, const void* U, const void* Ub,
+            void* phi) {
+    return guarded([&] {
+        fv_mesh* m = const_cast<fv_mesh*>(mc);
+        check_common(ctx, m, prec);
+        FV_REQUIRE(U && phi && (Ub || m->nB == 0), "NULL field");
+        const int n = m->nInt + m->nB;
+        if (!n) return;
+        if (prec == FV_F64)
+            k_flux<double><<<grid_for(n, kBlock), kBlock, 0, ctx->stream>>>(
+                m->nInt, m->nB, m->own.p, m->nbr.p, m->w.p, m->Sf.p, m->Sfb.p, (const double*)U,
+                (const double*)Ub, (double*)phi);
+        else {
+            auto g = geo32(m);
+            k_flux<float><<<grid_for(n, kBlock), kBlock, 0, ctx->stream>>>(
+                m->nInt, m->nB, m->own.p, m->nbr.p, g.w, g.Sf, g.Sfb, (const float*)U,
+                (const float*)Ub, (float*)phi);
+        }
+        FV_LAUNCH_CHECK();
+    });
+}
+
+int fv_correct_boundary(fv_ctx* ctx, const fv_mesh* mc, fv_prec prec, const void* T,
+                        const fv_bc* bc, const void* Tg, void* Tb) {
+    return guarded([&] {
+        fv_mesh* m = const_cast<fv_mesh*>(mc);
+        check_common(ctx, m, prec);
+        check_bc(m, bc);
+        FV_REQUIRE(T && (Tb || m->nB == 0), "NULL field");
+        if (!m->nB) return;
+        if (prec == FV_F64)
+            k_correct_bc<double><<<grid_for(m->nB, kBlock), kBlock, 0, ctx->stream>>>(
+                m->nB, m->faceCells.p, m->dcb.p, bc->faceKind.p, (const double*)T,
+                (const double*)Tg, (double*)Tb);
+        else
+            k_correct_bc<float><<<grid_for(m->nB, kBlock), kBlock, 0, ctx->stream>>>(
+                m->nB, m->faceCells.p, m->dcb.p, bc->faceKind.p, (const float*)T,
+                (const float*)Tg, (float*)Tb);
+        FV_LAUNCH_CHECK();
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,52 @@
+// mesh.hpp -- device-resident finite-volume mesh (NeoN UnstructuredMesh +
+// BoundaryMesh arrays, meshAdapter.cpp:59-136) plus the gather structures the
+// deterministic kernels use.
+#pragma once
+
+#include "common.hpp"
+
+// Layout in HBM (all device arrays, int32 indices):
+//   cells:   V[N], C[3N]
+//   internal faces (OpenFOAM order): own[F], nbr[F], Sf[3F], Cf[3F], magSf[F],
+//            w[F], dc[F], nodc[F]
+//   boundary faces (flattened patches): faceCells[B], Sfb[3B], Cfb[3B], magSfb[B],
+//            nfb[3B], deltab[3B], wb[B], dcb[B], nodcb[B]
+//   cell->face stencil in ascending face order (CellToFaceStencil):
+//            cfOff[N+1], cfIdx[2F+B]: entry e >= 0 && e < F: internal face e with
+//            the cell as owner; e < 0: internal face ~e with the cell as
+//            neighbour; e >= F: boundary face e-F.  cfOther[2F+B] holds the cell
+//            on the other side of an internal face (the cell itself for a
+//            boundary face) so the kernels read it as a stream.
+//   FP32 mirror of every floating array, built on first FP32 use.
+struct MeshF32 {
+    fvhip::DBuf<float> V, Sf, magSf, w, nodc, Sfb, magSfb, dcb;
+    bool ready = false;
+};
+
+struct fv_mesh {
+    fv_ctx* ctx = nullptr;
+    int32_t nCells = 0, nInt = 0, nB = 0, nPatches = 0;
+    std::vector<int32_t> patchOff;  // host copy [nPatches+1]
+    fvhip::DBuf<double> V, C;
+    fvhip::DBuf<int32_t> own, nbr;
+    fvhip::DBuf<double> Sf, Cf, magSf, w, dc, nodc;
+    fvhip::DBuf<int32_t> faceCells;
+    fvhip::DBuf<double> Sfb, Cfb, magSfb, nfb, deltab, wb, dcb, nodcb;
+    fvhip::DBuf<int32_t> patchOffD;
+    fvhip::DBuf<int32_t> cfOff, cfIdx, cfOther;
+    MeshF32 f32;
+};
+
+struct fv_bc {
+    fv_mesh* mesh = nullptr;
+    std::vector<int32_t> kinds;        // per patch
+    fvhip::DBuf<int8_t> faceKind;      // per boundary face
+};
+
+namespace fvhip {
+// Build cfOff/cfIdx/cfOther on the host from owner/neighbour/faceCells and upload.
+void build_stencil(fv_mesh* m, const int32_t* h_own, const int32_t* h_nbr,
+                   const int32_t* h_faceCells);
+// FP32 copies of the geometry (lazy).
+void ensure_f32(fv_mesh* m);
+}  // namespace fvhip

@@ -1,0 +1,225 @@
+// runtime.hip -- context, device memory accounting, copies, fills and the
+// counter-based uniform generator.
+#include "common.hpp"
+
+namespace fvhip {
+
+static thread_local std::string g_last_error;
+
+void set_error(const std::string& msg) { g_last_error = msg; }
+void clear_error() { g_last_error.clear(); }
+
+void* ctx_alloc(fv_ctx* ctx, size_t bytes) {
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    if (ctx->budget && ctx->in_use + bytes > ctx->budget)
+        throw Error{FV_ENOMEM_BUDGET, "HBM budget exceeded: in use " + std::to_string(ctx->in_use) +
+                                          " + request " + std::to_string(bytes) + " > budget " +
+                                          std::to_string(ctx->budget)};
+    void* p = nullptr;
+    FV_HIP(hipSetDevice(ctx->device));
+    hipError_t e = hipMalloc(&p, bytes ? bytes : 1);
+    if (e != hipSuccess)
+        throw Error{FV_EDEVICE, std::string("hipMalloc(") + std::to_string(bytes) + "): " +
+                                    hipGetErrorString(e)};
+    ctx->sizes[p] = bytes;
+    ctx->in_use += bytes;
+    return p;
+}
+
+void ctx_free(fv_ctx* ctx, void* p) {
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    auto it = ctx->sizes.find(p);
+    if (it == ctx->sizes.end()) return;
+    ctx->in_use -= it->second;
+    ctx->sizes.erase(it);
+    // device-wide sync keeps a free from racing work still queued on the stream
+    (void)hipStreamSynchronize(ctx->stream);
+    (void)hipFree(p);
+}
+
+template <typename S>
+__global__ void k_fill(S* __restrict__ p, int64_t n, S v) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t s = (int64_t)gridDim.x * blockDim.x;
+    for (; i < n; i += s) p[i] = v;
+}
+
+template <typename D, typename S>
+__global__ void k_convert(D* __restrict__ d, const S* __restrict__ s, int64_t n) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t st = (int64_t)gridDim.x * blockDim.x;
+    for (; i < n; i += st) d[i] = (D)s[i];
+}
+
+template <typename S>
+__global__ void k_uniform(S* __restrict__ p, int64_t n, double lo, double hi, uint64_t seed,
+                          uint64_t stream) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t s = (int64_t)gridDim.x * blockDim.x;
+    for (; i < n; i += s) p[i] = (S)(lo + (hi - lo) * rng_u01(seed, stream, (uint64_t)i));
+}
+
+}  // namespace fvhip
+
+using namespace fvhip;
+
+extern "C" {
+
+const char* fv_last_error(void) { return g_last_error.c_str(); }
+const char* fv_version(void) { return "fvhip 0.1 (gfx950)"; }
+
+int fv_ctx_create(int device, size_t hbm_budget_bytes, fv_ctx** out) {
+    return guarded([&] {
+        FV_REQUIRE(out, "out is NULL");
+        int n = 0;
+        FV_HIP(hipGetDeviceCount(&n));
+        FV_REQUIRE(device >= 0 && device < n, "device index out of range");
+        FV_HIP(hipSetDevice(device));
+        auto* c = new fv_ctx();
+        c->device = device;
+        c->budget = hbm_budget_bytes;
+        hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+        if (e != hipSuccess) {
+            delete c;
+            throw Error{FV_EDEVICE, std::string("hipStreamCreate: ") + hipGetErrorString(e)};
+        }
+        *out = c;
+    });
+}
+
+int fv_ctx_destroy(fv_ctx* ctx) {
+    return guarded([&] {
+        if (!ctx) return;
+        (void)hipSetDevice(ctx->device);
+        (void)hipStreamSynchronize(ctx->stream);
+        for (auto& kv : ctx->sizes) (void)hipFree(kv.first);
+        ctx->sizes.clear();
+        for (auto& e : ctx->events)
+            if (e) (void)hipEventDestroy(e);
+        (void)hipStreamDestroy(ctx->stream);
+        delete ctx;
+    });
+}
+
+int fv_sync(fv_ctx* ctx) {
+    return guarded([&] {
+        FV_REQUIRE(ctx, "ctx is NULL");
+        FV_HIP(hipStreamSynchronize(ctx->stream));
+    });
+}
+
+int fv_event_record(fv_ctx* ctx, int slot) {
+    return guarded([&] {
+        FV_REQUIRE(ctx && slot >= 0 && slot < fv_ctx::kEvents, "bad event slot");
+        if (!ctx->events[slot]) FV_HIP(hipEventCreate(&ctx->events[slot]));
+        FV_HIP(hipEventRecord(ctx->events[slot], ctx->stream));
+    });
+}
+
+int fv_event_elapsed(fv_ctx* ctx, int a, int b, float* ms) {
+    return guarded([&] {
+        FV_REQUIRE(ctx && ms && a >= 0 && b >= 0 && a < fv_ctx::kEvents && b < fv_ctx::kEvents,
+                   "bad event slot");
+        FV_REQUIRE(ctx->events[a] && ctx->events[b], "event slot not recorded");
+        FV_HIP(hipEventSynchronize(ctx->events[b]));
+        FV_HIP(hipEventElapsedTime(ms, ctx->events[a], ctx->events[b]));
+    });
+}
+
+void* fv_ctx_stream(fv_ctx* ctx) { return ctx ? (void*)ctx->stream : nullptr; }
+size_t fv_ctx_bytes_in_use(fv_ctx* ctx) { return ctx ? ctx->in_use : 0; }
+
+int fv_device_alloc(fv_ctx* ctx, size_t bytes, void** d_out) {
+    return guarded([&] {
+        FV_REQUIRE(ctx && d_out, "NULL argument");
+        *d_out = ctx_alloc(ctx, bytes);
+    });
+}
+
+int fv_device_free(fv_ctx* ctx, void* d_ptr) {
+    return guarded([&] {
+        FV_REQUIRE(ctx, "ctx is NULL");
+        if (d_ptr) {
+            FV_REQUIRE(ctx->sizes.count(d_ptr), "pointer was not allocated by this context");
+            ctx_free(ctx, d_ptr);
+        }
+    });
+}
+
+int fv_memcpy_h2d(fv_ctx* ctx, void* d_dst, const void* h_src, size_t bytes) {
+    return guarded([&] {
+        FV_REQUIRE(ctx, "ctx is NULL");
+        if (!bytes) return;
+        FV_REQUIRE(d_dst && h_src, "NULL pointer");
+        FV_HIP(hipMemcpyAsync(d_dst, h_src, bytes, hipMemcpyHostToDevice, ctx->stream));
+        FV_HIP(hipStreamSynchronize(ctx->stream));
+    });
+}
+
+int fv_memcpy_d2h(fv_ctx* ctx, void* h_dst, const void* d_src, size_t bytes) {
+    return guarded([&] {
+        FV_REQUIRE(ctx, "ctx is NULL");
+        if (!bytes) return;
+        FV_REQUIRE(h_dst && d_src, "NULL pointer");
+        FV_HIP(hipMemcpyAsync(h_dst, d_src, bytes, hipMemcpyDeviceToHost, ctx->stream));
+        FV_HIP(hipStreamSynchronize(ctx->stream));
+    });
+}
+
+int fv_memcpy_d2d(fv_ctx* ctx, void* d_dst, const void* d_src, size_t bytes) {
+    return guarded([&] {
+        FV_REQUIRE(ctx, "ctx is NULL");
+        if (!bytes) return;
+        FV_REQUIRE(d_dst && d_src, "NULL pointer");
+        FV_HIP(hipMemcpyAsync(d_dst, d_src, bytes, hipMemcpyDeviceToDevice, ctx->stream));
+    });
+}
+
+int fv_fill(fv_ctx* ctx, fv_prec prec, void* d_ptr, int64_t n, double value) {
+    return guarded([&] {
+        FV_REQUIRE(ctx && n >= 0, "bad argument");
+        if (!n) return;
+        FV_REQUIRE(d_ptr, "NULL pointer");
+        int g = grid_for(n, 256, 4096);
+        if (prec == FV_F64)
+            k_fill<double><<<g, 256, 0, ctx->stream>>>((double*)d_ptr, n, value);
+        else
+            k_fill<float><<<g, 256, 0, ctx->stream>>>((float*)d_ptr, n, (float)value);
+        FV_LAUNCH_CHECK();
+    });
+}
+
+int fv_convert(fv_ctx* ctx, fv_prec to, void* d_dst, fv_prec from, const void* d_src, int64_t n) {
+    return guarded([&] {
+        FV_REQUIRE(ctx && n >= 0, "bad argument");
+        if (!n) return;
+        FV_REQUIRE(d_dst && d_src, "NULL pointer");
+        int g = grid_for(n, 256, 4096);
+        if (to == FV_F32 && from == FV_F64)
+            k_convert<float, double><<<g, 256, 0, ctx->stream>>>((float*)d_dst, (const double*)d_src, n);
+        else if (to == FV_F64 && from == FV_F32)
+            k_convert<double, float><<<g, 256, 0, ctx->stream>>>((double*)d_dst, (const float*)d_src, n);
+        else if (to == FV_F64)
+            k_convert<double, double><<<g, 256, 0, ctx->stream>>>((double*)d_dst, (const double*)d_src, n);
+        else
+            k_convert<float, float><<<g, 256, 0, ctx->stream>>>((float*)d_dst, (const float*)d_src, n);
+        FV_LAUNCH_CHECK();
+    });
+}
+
+int fv_fill_uniform(fv_ctx* ctx, fv_prec prec, void* d_ptr, int64_t n, double lo, double hi,
+                    uint64_t seed, uint64_t stream) {
+    return guarded([&] {
+        FV_REQUIRE(ctx && n >= 0, "bad argument");
+        if (!n) return;
+        FV_REQUIRE(d_ptr, "NULL pointer");
+        int g = grid_for(n, 256, 8192);
+        if (prec == FV_F64)
+            k_uniform<double><<<g, 256, 0, ctx->stream>>>((double*)d_ptr, n, lo, hi, seed, stream);
+        else
+            k_uniform<float><<<g, 256, 0, ctx->stream>>>((float*)d_ptr, n, lo, hi, seed, stream);
+        FV_LAUNCH_CHECK();
+    });
+}
+
+}  // extern "C"
