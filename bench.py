@@ -1,16 +1,28 @@
 #!/usr/bin/env python3
-"""Explicit face-operator sweep on one MI355X (the 256^3 jittered, randomly
-renumbered hex mesh; FP64 and FP32): achieved GB/s against the HBM roofline.
+"""Face-operator sweep and implicit solves on one MI355X.
 
-A step is one pass of every operator of the sweep over the device-resident
+Headline (value): the explicit operator sweep on the 256^3 hex mesh with
+jittered interior vertices and a randomly permuted cell numbering, FP64 and
+FP32.  A step is one pass of every operator of the sweep over device-resident
 synthetic fields: interpolate(linear), gaussGreenGrad, div(Gauss linear),
 div(Gauss upwind), laplacian(Gauss linear uncorrected), snGrad(uncorrected),
-each in FP64 and in FP32 (benchmarks/bench_explicitOperators.cpp: the
-"No allocation" sections).  Throughput counts algorithmic bytes: every
-API-level input array once at its stored precision plus every output once
-(per internal face / cell / boundary face coefficients in BYTES below).
+in FP64 and in FP32 (benchmarks/bench_explicitOperators.cpp, "No allocation"
+sections).  Throughput counts algorithmic bytes: every API-level input array
+once at its stored precision plus every output once (per internal face / cell
+/ boundary face coefficients in BYTES).
+
+The device mesh is renumbered at import (reverse Cuthill-McKee, as OpenFOAM's
+renumberMesh); the caller's fields are mapped into that numbering once when
+they are uploaded (cost reported as boundary_mapping).  The same sweep on the
+numbering as given is reported beside it (as_given), and the results of the
+two are compared.
+
+Also reported: the implicit heat-transfer case (32^3, 10 Euler steps,
+Jacobi-CG to 1e-10) and the pressure-Poisson case (128^3 jittered, permuted,
+FP64 CG vs FP32-inner/FP64-refined CG to 1e-8).
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 256] [--no-cpu]
+                       [--no-implicit] [--no-as-given]
 """
 from __future__ import annotations
 
@@ -49,78 +61,118 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def build_case(n, seed):
-    import foamadapter_amd as fa
-    from foamadapter_amd import fvcc
+class Case:
+    """Device mesh + the sweep's fields in the mesh's numbering."""
 
-    t0 = time.time()
-    poly = fa.PolyMesh.hex(n, jitter=0.2, jitter_seed=seed, perm_seed=seed)
-    log(f"hex mesh {n}^3 generated in {time.time() - t0:.1f}s")
-    ctx = fa.Context(0)
-    t0 = time.time()
-    mesh = fa.Mesh.from_poly(ctx, poly)
-    log(f"device mesh (GPU geometry) built in {time.time() - t0:.1f}s")
-    del poly
-    N, F, B = mesh.n_cells, mesh.n_internal_faces, mesh.n_boundary_faces
-    bc = fa.BoundaryConditions(mesh, ["zeroGradient"] * mesh.n_patches)
-    # T ~ U[-1,1] per cell; zeroGradient walls give T_b = T_P
-    T = ctx.uniform(N, -1.0, 1.0, seed, 0)
-    Tb = ctx.zeros((B,))
-    Tf = fvcc.VolumeField(mesh, T, Tb, bc, "T")
-    Tf.correct_boundary_conditions()
-    # phi = Sf . interpolate(U), U_i ~ U[-1,1], U_b = U_P
-    U = ctx.uniform(3 * N, -1.0, 1.0, seed, 1)
-    fc = mesh.get("faceCells")
-    Uh = U.numpy().reshape(N, 3)
-    Ub = ctx.upload(np.ascontiguousarray(Uh[fc]))
-    del Uh
-    Uf = fvcc.VolumeField(mesh, U, Ub)
-    phi = fvcc.flux(ctx, mesh, Uf)
-    U.close()
-    Ub.close()
-    gamma = fvcc.SurfaceField(mesh, ctx.uniform(F + B, 0.5, 2.0, seed, 2))
-    fields = {"f64": (Tf, phi, gamma)}
-    T32 = fvcc.VolumeField(mesh, Tf.internal.astype(np.float32), Tf.boundary.astype(np.float32), bc)
-    fields["f32"] = (T32, fvcc.SurfaceField(mesh, phi.values.astype(np.float32)),
-                     fvcc.SurfaceField(mesh, gamma.values.astype(np.float32)))
+    def __init__(self, ctx, poly, seed, renumber):
+        import foamadapter_amd as fa
+        from foamadapter_amd import fvcc
+
+        t0 = time.time()
+        self.mesh = mesh = fa.Mesh.from_poly(ctx, poly, renumber=renumber)
+        self.setup_s = time.time() - t0
+        N, F, B = mesh.n_cells, mesh.n_internal_faces, mesh.n_boundary_faces
+        self.bc = fa.BoundaryConditions(mesh, ["zeroGradient"] * mesh.n_patches)
+        # caller-numbered inputs: T ~ U[-1,1] per cell, U_i ~ U[-1,1] per cell,
+        # gamma ~ U[0.5,2] per face (the generator indexes the caller's numbering)
+        Tc = ctx.uniform(N, -1.0, 1.0, seed, 0)
+        Uc = ctx.uniform(3 * N, -1.0, 1.0, seed, 1)
+        Uc.shape = (N, 3)
+        gc = ctx.uniform(F + B, 0.5, 2.0, seed, 2)
+        ctx.sync()
+        ctx.record(60)
+        T = mesh.to_device(Tc, "cell")
+        U = mesh.to_device(Uc, "cell")
+        gamma = mesh.to_device(gc, "face")
+        ctx.record(61)
+        self.mapping_ms = ctx.elapsed_ms(60, 61) if renumber else 0.0
+        Tf = fvcc.VolumeField(mesh, T, ctx.zeros((B,)), self.bc, "T")
+        Tf.correct_boundary_conditions()          # zeroGradient: T_b = T_P
+        fc = mesh.get("faceCells")
+        Ub = ctx.upload(np.ascontiguousarray(U.numpy()[fc]))   # U_b = U_P
+        phi = fvcc.flux(ctx, mesh, fvcc.VolumeField(mesh, U, Ub))
+        self.fields = {"f64": (Tf, phi, fvcc.SurfaceField(mesh, gamma))}
+        T32 = fvcc.VolumeField(mesh, T.astype(np.float32), Tf.boundary.astype(np.float32), self.bc)
+        self.fields["f32"] = (T32, fvcc.SurfaceField(mesh, phi.values.astype(np.float32)),
+                              fvcc.SurfaceField(mesh, gamma.astype(np.float32)))
+        # free the caller-numbered copies (on a mesh that is not renumbered they
+        # are the device fields themselves and stay)
+        for dev, caller in ((T, Tc), (gamma, gc)):
+            if dev is not caller:
+                caller.close()
+        for a in (U, Uc, Ub):
+            a.close()
+        ctx.sync()
+
+    def ops(self, ctx):
+        from foamadapter_amd import fvcc
+        mesh = self.mesh
+        nFB = mesh.n_internal_faces + mesh.n_boundary_faces
+        ops = {}
+        for prec, (T, phi, gamma) in self.fields.items():
+            dt = np.float64 if prec == "f64" else np.float32
+            out_f = fvcc.SurfaceField(mesh, ctx.empty((nFB,), dt))
+            out_c = ctx.empty((mesh.n_cells,), dt)
+            out_g = ctx.empty((mesh.n_cells, 3), dt)
+            lin = fvcc.SurfaceInterpolation(ctx, mesh, ["linear"])
+            grd = fvcc.GaussGreenGrad(ctx, mesh)
+            dl = fvcc.GaussGreenDiv(ctx, mesh, ["Gauss", "linear"])
+            du = fvcc.GaussGreenDiv(ctx, mesh, ["Gauss", "upwind"])
+            lap = fvcc.GaussGreenLaplacian(ctx, mesh, ["Gauss", "linear", "uncorrected"])
+            sng = fvcc.FaceNormalGradient(ctx, mesh, ["uncorrected"])
+            ops[("interp_linear", prec)] = (lambda T=T, o=out_f, f=lin: f.interpolate(T, o), out_f.values)
+            ops[("grad", prec)] = (lambda T=T, o=out_g, f=grd: f.grad(T, 1.0, o), out_g)
+            ops[("div_linear", prec)] = (lambda T=T, p=phi, o=out_c, f=dl: f.div(o, p, T, 1.0), out_c)
+            ops[("div_upwind", prec)] = (lambda T=T, p=phi, o=out_c, f=du: f.div(o, p, T, 1.0), out_c)
+            ops[("laplacian", prec)] = (lambda T=T, g=gamma, o=out_c, f=lap: f.laplacian(o, g, T, 1.0), out_c)
+            ops[("sngrad", prec)] = (lambda T=T, o=out_f, f=sng: f.faceNormalGrad(T, o), out_f.values)
+        return ops
+
+
+def time_sweep(ctx, case, steps, warmup):
+    N, F, B = case.mesh.n_cells, case.mesh.n_internal_faces, case.mesh.n_boundary_faces
+    ops = case.ops(ctx)
+    keys = [(op, p) for p in ("f64", "f32") for op in OPS]
+    for _ in range(warmup):
+        for k in keys:
+            ops[k][0]()
     ctx.sync()
-    return ctx, mesh, bc, fields
+    per = {k: 0.0 for k in keys}
+    ctx.record(62)
+    for _ in range(steps):
+        for i, k in enumerate(keys):
+            ctx.record(2 * i)
+            ops[k][0]()
+            ctx.record(2 * i + 1)
+        for i, k in enumerate(keys):
+            per[k] += ctx.elapsed_ms(2 * i, 2 * i + 1)
+    ctx.record(63)
+    ctx.sync()
+    total_ms = ctx.elapsed_ms(62, 63)
+    ms_step = total_ms / steps
+    bytes_step = sum(alg_bytes(op, p, N, F, B) for op, p in keys)
+    per_op = {}
+    for (op, p) in keys:
+        ms = per[(op, p)] / steps
+        gbs = alg_bytes(op, p, N, F, B) / (ms * 1e-3) / 1e9
+        per_op[f"{op}_{p}"] = {"ms": round(ms, 4), "GB/s": round(gbs, 1),
+                               "frac_peak": round(gbs / HBM_PEAK_GBS, 4)}
+    dom = max(keys, key=lambda k: per[k])
+    dom_ms = per[dom] / steps
+    dom_bytes = alg_bytes(dom[0], dom[1], N, F, B)
+    return {"ms_per_step": ms_step, "GB/s": bytes_step / (ms_step * 1e-3) / 1e9,
+            "per_op": per_op, "dominant": (dom, dom_ms, dom_bytes), "ops": ops}
 
 
-def make_ops(ctx, mesh, fields):
-    """Returns {(op, prec): launch()} over preallocated outputs."""
-    from foamadapter_amd import fvcc
-
-    nFB = mesh.n_internal_faces + mesh.n_boundary_faces
-    ops = {}
-    for prec, (T, phi, gamma) in fields.items():
-        dt = np.float64 if prec == "f64" else np.float32
-        out_f = fvcc.SurfaceField(mesh, ctx.empty((nFB,), dt))
-        out_c = ctx.empty((mesh.n_cells,), dt)
-        out_g = ctx.empty((mesh.n_cells, 3), dt)
-        lin = fvcc.SurfaceInterpolation(ctx, mesh, ["linear"])
-        grd = fvcc.GaussGreenGrad(ctx, mesh)
-        dl = fvcc.GaussGreenDiv(ctx, mesh, ["Gauss", "linear"])
-        du = fvcc.GaussGreenDiv(ctx, mesh, ["Gauss", "upwind"])
-        lap = fvcc.GaussGreenLaplacian(ctx, mesh, ["Gauss", "linear", "uncorrected"])
-        sng = fvcc.FaceNormalGradient(ctx, mesh, ["uncorrected"])
-        ops[("interp_linear", prec)] = (lambda T=T, o=out_f, f=lin: f.interpolate(T, o))
-        ops[("grad", prec)] = (lambda T=T, o=out_g, f=grd: f.grad(T, 1.0, o))
-        ops[("div_linear", prec)] = (lambda T=T, p=phi, o=out_c, f=dl: f.div(o, p, T, 1.0))
-        ops[("div_upwind", prec)] = (lambda T=T, p=phi, o=out_c, f=du: f.div(o, p, T, 1.0))
-        ops[("laplacian", prec)] = (lambda T=T, g=gamma, o=out_c, f=lap: f.laplacian(o, g, T, 1.0))
-        ops[("sngrad", prec)] = (lambda T=T, o=out_f, f=sng: f.faceNormalGrad(T, o))
-    return ops
-
-
-def cpu_baseline(mesh, fields, budget_s=20.0):
+def cpu_baseline(case, budget_s=20.0):
     """The oracle (serial C++ restatement, 1 thread) on the same mesh and fields,
-    FP64 sweep, each op run until about budget_s/6 seconds have passed."""
+    FP64 sweep, each op repeated until about budget_s/6 seconds have passed."""
     sys.path.insert(0, str(ROOT / "tests"))
     import oracle_cpu as orc
 
+    mesh = case.mesh
     g = mesh.geometry()
-    T, phi, gamma = fields["f64"]
+    T, phi, gamma = case.fields["f64"]
     Th, Tbh, ph, gh = T.internal.numpy(), T.boundary.numpy(), phi.numpy(), gamma.numpy()
     N, F, B = mesh.n_cells, mesh.n_internal_faces, mesh.n_boundary_faces
     kinds = [1] * mesh.n_patches
@@ -144,8 +196,86 @@ def cpu_baseline(mesh, fields, budget_s=20.0):
         tot_t += t
         runs[op] = n
     return {"value": round(tot_b / tot_t / 1e9, 3), "unit": "GB/s", "cores": 1, "kind": "port",
-            "sample": f"oracle (serial C++ restatement, 1 thread) FP64 sweep of the 6 operators on "
-                      f"the same {N}-cell mesh and fields, runs per op {runs}, {tot_t:.1f}s total"}
+            "sample": f"oracle (serial C++ restatement of the same formulas, 1 thread; NeoN is "
+                      f"not buildable offline) FP64 sweep of the 6 operators on the same "
+                      f"{N}-cell renumbered mesh and fields, runs per op {runs}, {tot_t:.1f}s"}
+
+
+def bench_heat(ctx, n=32, steps=10):
+    """Implicit heat transfer: ddt(T) - laplacian(1e-2, T), Euler dt=1e-3, 10 steps,
+    Jacobi-CG to 1e-10; walls T=300 (x0), 400 (x1), 350 elsewhere."""
+    import foamadapter_amd as fa
+    from foamadapter_amd import dsl, fvcc
+    poly = fa.PolyMesh.hex(n)
+    m = fa.Mesh.from_poly(ctx, poly)
+    N, F, B = m.n_cells, m.n_internal_faces, m.n_boundary_faces
+    po = m.get("patchOffsets")
+    Tb = np.full(B, 350.0)
+    Tb[po[0]:po[1]] = 300.0
+    Tb[po[1]:po[2]] = 400.0
+    bc = fa.BoundaryConditions(m, ["fixedValue"] * 6)
+    res = {}
+    for mode in ("double", "mixed"):
+        T = fvcc.VolumeField(m, ctx.uniform(N, 300.0, 400.0, 1001, 0), ctx.upload(Tb), bc, "T")
+        kap = fvcc.SurfaceField(m, ctx.upload(np.full(F + B, 1e-2)))
+        eqn = dsl.imp.ddt(T) - dsl.imp.laplacian(kap, T)
+        iters = []
+        ctx.sync()
+        t0 = time.perf_counter()
+        for s in range(steps):
+            st = dsl.solve(eqn, T, s * 1e-3, 1e-3, solver={"relTol": 1e-10, "maxIter": 1000,
+                                                             "precision": mode})
+            iters.append(st.numIter)
+        ctx.sync()
+        wall = time.perf_counter() - t0
+        res[mode] = {"wall_ms": round(wall * 1e3, 2), "ms_per_step": round(wall * 1e3 / steps, 3),
+                     "cg_iters": iters, "Tmin": float(T.numpy().min()), "Tmax": float(T.numpy().max())}
+    m.close()
+    return {"mesh": f"{n}^3", "cells": N, **res}
+
+
+def bench_poisson(ctx, n=128, seed=1003, repeats=3):
+    """laplacian(1,p) = div(phi(U)), p=0 on x0, zeroGradient elsewhere, 128^3
+    jittered + permuted (renumbered at import); FP64 CG vs FP32 CG in FP64
+    refinement, both to 1e-8 relative residual."""
+    import foamadapter_amd as fa
+    from foamadapter_amd import dsl, fvcc, la
+    t0 = time.time()
+    poly = fa.PolyMesh.hex(n, jitter=0.2, jitter_seed=seed, perm_seed=seed)
+    m = fa.Mesh.from_poly(ctx, poly, renumber=True)
+    del poly
+    log(f"poisson mesh {n}^3 ready in {time.time() - t0:.1f}s")
+    N, F, B = m.n_cells, m.n_internal_faces, m.n_boundary_faces
+    bc = fa.BoundaryConditions(m, ["fixedValue"] + ["zeroGradient"] * (m.n_patches - 1))
+    p = fvcc.VolumeField(m, ctx.zeros((N,)), ctx.zeros((B,)), bc, "p")
+    U = fvcc.VolumeField(m, ctx.uniform(3 * N, -1, 1, seed, 1), ctx.uniform(3 * B, -1, 1, seed, 2))
+    phi = fvcc.flux(ctx, m, U)
+    one = fvcc.SurfaceField(m, ctx.upload(np.ones(F + B)))
+    ls = dsl.assemble(dsl.imp.laplacian(one, p) - dsl.exp.div(phi), p, 0.0, 1.0)
+    out = {}
+    xs = {}
+    for mode in ("double", "mixed"):
+        best = None
+        for _ in range(repeats):
+            x = ctx.zeros((N,))
+            ctx.sync()
+            ctx.record(50)
+            st = la.solve_cg(ls, x, {"relTol": 1e-8, "maxIter": 20000, "precision": mode,
+                                     "innerRelTol": 1e-4, "checkEvery": 32})
+            ctx.record(51)
+            ms = ctx.elapsed_ms(50, 51)
+            if best is None or ms < best[0]:
+                best = (ms, st)
+            xs[mode] = x
+        ms, st = best
+        out[mode] = {"ms": round(ms, 3), "solves_per_s": round(1e3 / ms, 2), "iters": st.numIter,
+                     "outer": st.outerIter, "final_rel_res": st.finalResNorm,
+                     "true_rel_res": st.trueResNorm}
+    a, b = xs["mixed"].numpy(), xs["double"].numpy()
+    out["mixed_vs_fp64_rel_err"] = float(np.linalg.norm(a - b) / np.linalg.norm(b))
+    out["speedup_mixed_over_fp64"] = round(out["double"]["ms"] / out["mixed"]["ms"], 3)
+    m.close()
+    return {"mesh": f"{n}^3 jittered, permuted, renumbered", "cells": N, **out}
 
 
 def main():
@@ -156,78 +286,88 @@ def main():
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--seed", type=int, default=1002)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-implicit", action="store_true")
+    ap.add_argument("--no-as-given", action="store_true")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="first copy 2 GiB device-to-device 3 times (a known-size reference "
+                         "for hardware traffic counters)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py measures one MI355X: the face-operator sweep has no multi-GPU path in "
-              "this build (sharding is described in DESIGN.md); run with --gpus 1",
+        print("bench.py measures one MI355X: the finite-volume path runs on one GPU in this "
+              "build (how it would shard is described in DESIGN.md); run with --gpus 1",
               file=sys.stderr)
         return 2
+    import foamadapter_amd as fa
 
-    ctx, mesh, bc, fields = build_case(args.n, args.seed)
-    N, F, B = mesh.n_cells, mesh.n_internal_faces, mesh.n_boundary_faces
-    ops = make_ops(ctx, mesh, fields)
-    keys = [(op, p) for p in ("f64", "f32") for op in OPS]
-    for _ in range(args.warmup):
-        for k in keys:
-            ops[k]()
-    ctx.sync()
-    per = {k: 0.0 for k in keys}
-    ctx.record(62)
-    t_wall = time.perf_counter()
-    for _ in range(args.steps):
-        for i, k in enumerate(keys):
-            ctx.record(2 * i)
-            ops[k]()
-            ctx.record(2 * i + 1)
-        for i, k in enumerate(keys):
-            per[k] += ctx.elapsed_ms(2 * i, 2 * i + 1)
-    ctx.record(63)
-    ctx.sync()
-    t_wall = time.perf_counter() - t_wall
-    total_ms = ctx.elapsed_ms(62, 63)
-    ms_step = total_ms / args.steps
-    bytes_step = sum(alg_bytes(op, p, N, F, B) for op, p in keys)
-    # whole-job throughput: all algorithmic bytes of the K steps over the
-    # event-bracketed time of the K steps
-    value = bytes_step / (ms_step * 1e-3) / 1e9
-    per_op = {}
-    for (op, p) in keys:
-        ms = per[(op, p)] / args.steps
-        gbs = alg_bytes(op, p, N, F, B) / (ms * 1e-3) / 1e9
-        per_op[f"{op}_{p}"] = {"ms": round(ms, 4), "GB/s": round(gbs, 1),
-                               "frac_peak": round(gbs / HBM_PEAK_GBS, 4)}
-    dom = max(keys, key=lambda k: per[k])
-    dom_ms = per[dom] / args.steps
-    dom_bytes = alg_bytes(dom[0], dom[1], N, F, B)
+    ctx = fa.Context(0)
+    if args.copy_probe:
+        from foamadapter_amd import _lib
+        nb = 2 << 30
+        a, b = ctx.empty((nb // 8,)), ctx.empty((nb // 8,))
+        a.fill(1.0)
+        for _ in range(3):
+            _lib.call("fv_memcpy_d2d", ctx.handle, b.ptr, a.ptr, nb)
+        ctx.sync()
+        a.close()
+        b.close()
+    t0 = time.time()
+    poly = fa.PolyMesh.hex(args.n, jitter=0.2, jitter_seed=args.seed, perm_seed=args.seed)
+    log(f"hex mesh {args.n}^3 generated in {time.time() - t0:.1f}s")
+    case = Case(ctx, poly, args.seed, renumber=True)
+    log(f"renumbered device mesh + fields ready (mesh setup {case.setup_s:.1f}s)")
+    sweep = time_sweep(ctx, case, args.steps, args.warmup)
+    N, F, B = case.mesh.n_cells, case.mesh.n_internal_faces, case.mesh.n_boundary_faces
+    (dop, dprec), dom_ms, dom_bytes = sweep["dominant"]
     achieved = dom_bytes / (dom_ms * 1e-3) / 1e9
     rec = {
         "metric": "face-op GB/s vs HBM peak (explicit operator sweep, FP64+FP32)",
-        "value": round(value, 1),
+        "value": round(sweep["GB/s"], 1),
         "unit": "GB/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(ms_step, 4),
+        "ms_per_step": round(sweep["ms_per_step"], 4),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64+f32",
         "data": "synthetic",
-        "config": {"workload": f"explicit operator sweep, {args.n}^3 hex cells, interior "
-                               f"vertices jittered +-0.2h, cell numbering randomly permuted "
-                               f"(seed {args.seed}), numbering as given",
-                   "cells": N, "internal_faces": F, "boundary_faces": B,
-                   "ops": OPS, "precisions": ["f64", "f32"]},
-        "per_op": per_op,
-        "roofline": {"bound": "hbm", "kernel": f"{dom[0]}_{dom[1]}", "achieved": round(achieved, 1),
+        "config": {"workload": f"explicit operator sweep: {args.n}^3 hex cells, interior vertices "
+                               f"jittered +-0.2h, cell numbering randomly permuted (seed "
+                               f"{args.seed}); mesh renumbered at import (reverse Cuthill-McKee), "
+                               f"fields mapped once at upload",
+                   "cells": N, "internal_faces": F, "boundary_faces": B, "ops": OPS,
+                   "precisions": ["f64", "f32"]},
+        "per_op": sweep["per_op"],
+        "roofline": {"bound": "hbm", "kernel": f"{dop}_{dprec}", "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": None, "bytes_per_launch": dom_bytes,
-                     "avg_launch_ms": round(dom_ms, 4)},
-        "wall_s_timed": round(t_wall, 3),
+                     "traffic": None, "bytes_per_launch": dom_bytes, "avg_launch_ms": round(dom_ms, 4)},
+        "boundary_mapping": {"ms": round(case.mapping_ms, 3),
+                             "what": "permuting the caller's T, U (cells) and gamma (faces) into "
+                                     "the renumbered order once at upload"},
     }
+    if not args.no_as_given:
+        g = Case(ctx, poly, args.seed, renumber=False)
+        ag = time_sweep(ctx, g, max(3, args.steps // 4), 1)
+        # same operator, both numberings: gradient mapped back to the caller's numbering
+        ren_grad = case.mesh.to_caller(sweep["ops"][("grad", "f64")][1], "cell").numpy()
+        ag_grad = ag["ops"][("grad", "f64")][1].numpy()
+        diff = float(np.max(np.abs(ren_grad - ag_grad)) / np.max(np.abs(ag_grad)))
+        rec["as_given"] = {"GB/s": round(ag["GB/s"], 1), "ms_per_step": round(ag["ms_per_step"], 3),
+                           "per_op": ag["per_op"], "grad_f64_rel_diff_vs_renumbered": diff}
+        for k in list(g.fields):
+            del g.fields[k]
+        g.mesh.close()
+    del poly
     if not args.no_cpu:
         log("timing the CPU baseline (oracle, 1 thread)")
-        rec["cpu_baseline"] = cpu_baseline(mesh, fields)
+        rec["cpu_baseline"] = cpu_baseline(case)
+    case.mesh.close()
+    if not args.no_implicit:
+        log("implicit heat transfer 32^3")
+        rec["implicit_heat"] = bench_heat(ctx)
+        log("pressure Poisson 128^3, FP64 vs mixed")
+        rec["implicit_poisson"] = bench_poisson(ctx)
     print(json.dumps(rec), flush=True)
     ctx.close()
     return 0

@@ -7,7 +7,7 @@ layer over that ABI, named after NeoN's fvcc / la / dsl interfaces.
 from ._lib import FvError, LIB_PATH, lib  # noqa: F401  (fails loudly if the library is missing)
 from .runtime import Context, DeviceArray  # noqa: F401
 from .mesh import BoundaryConditions, Mesh, PolyMesh, read_polymesh  # noqa: F401
-from . import fvcc  # noqa: F401
+from . import fvcc, la, dsl  # noqa: F401
 
 __all__ = ["Context", "DeviceArray", "Mesh", "PolyMesh", "BoundaryConditions", "read_polymesh",
-           "fvcc", "FvError", "LIB_PATH"]
+           "fvcc", "la", "dsl", "FvError", "LIB_PATH"]

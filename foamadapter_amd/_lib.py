@@ -64,10 +64,13 @@ class FvMeshDesc(C.Structure):
         "h_nonorth_delta_coeffs_b", "h_patch_offsets")]
 
 
+FV_SOLVE_F64, FV_SOLVE_F32_IR = 0, 1
+
+
 class FvSolveOpts(C.Structure):
     _fields_ = [("rel_tol", C.c_double), ("max_iter", C.c_int32), ("mode", C.c_int32),
                 ("inner_rel_tol", C.c_double), ("max_outer", C.c_int32),
-                ("block", C.c_int32)]
+                ("check_every", C.c_int32)]
 
 
 class FvSolverStats(C.Structure):
@@ -100,6 +103,8 @@ _SIGS = {
     "fv_hexmesh_sizes": [_i32, _i32, _i32, _vp],
     "fv_hexmesh_generate": [_i32, _i32, _i32, _d, _d, _d, _d, _u64, _u64,
                             _vp, _vp, _vp, _vp, _vp, _vp, _vp],
+    "fv_polymesh_renumber": [C.POINTER(FvPolyMeshDesc), C.c_int, _vp, _vp, _vp, _vp, _vp, _vp],
+    "fv_permute": [_vp, C.c_int, _i64, C.c_int, C.c_int, C.c_int, _vp, _vp, _vp],
     "fv_mesh_create": [_vp, C.POINTER(FvMeshDesc), _pp],
     "fv_mesh_create_poly": [_vp, C.POINTER(FvPolyMeshDesc), _pp],
     "fv_mesh_destroy": [_vp],
@@ -114,12 +119,31 @@ _SIGS = {
     "fv_sngrad_uncorrected": [_vp, _vp, C.c_int, _vp, _vp, _vp, _vp, _vp],
     "fv_flux": [_vp, _vp, C.c_int, _vp, _vp, _vp],
     "fv_correct_boundary": [_vp, _vp, C.c_int, _vp, _vp, _vp, _vp],
+    "fv_sparsity_create": [_vp, _pp],
+    "fv_sparsity_destroy": [_vp],
+    "fv_sparsity_sizes": [_vp, _vp],
+    "fv_sparsity_get": [_vp, C.c_int, _vp],
+    "fv_linsys_create": [_vp, _pp],
+    "fv_linsys_destroy": [_vp],
+    "fv_linsys_values": [_vp],
+    "fv_linsys_rhs": [_vp],
+    "fv_linsys_zero": [_vp],
+    "fv_assemble_ddt_euler": [_vp, _vp, _d, _d],
+    "fv_assemble_source": [_vp, _vp, _d],
+    "fv_assemble_explicit": [_vp, _vp, _d],
+    "fv_assemble_laplacian": [_vp, _vp, _vp, _vp, _vp, _d],
+    "fv_assemble_div": [_vp, C.c_int, _vp, _vp, _vp, _vp, _d],
+    "fv_set_reference": [_vp, C.c_int, _d],
+    "fv_residual": [_vp, _vp, _vp],
+    "fv_solve_cg": [_vp, _vp, C.POINTER(FvSolveOpts), C.POINTER(FvSolverStats)],
 }
 _RESTYPES = {
     "fv_last_error": C.c_char_p,
     "fv_version": C.c_char_p,
     "fv_ctx_stream": C.c_void_p,
     "fv_ctx_bytes_in_use": C.c_size_t,
+    "fv_linsys_values": C.c_void_p,
+    "fv_linsys_rhs": C.c_void_p,
 }
 
 

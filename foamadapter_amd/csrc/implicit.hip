@@ -1,0 +1,396 @@
+// implicit.hip -- la::SparsityPattern, la::LinearSystem and the implicit
+// operator assembly (ddt Euler, laplacian Gauss linear uncorrected, div Gauss
+// linear/upwind, source, explicit source, SetReference, A.x - b).
+//
+// Every assembly kernel is a cell gather over the mesh's cell->face stencil in
+// ascending face order: a cell writes the off-diagonal coefficients of its own
+// row (each CSR slot has exactly one writer) and accumulates its diagonal and
+// right-hand side in the serial face order, so no atomics are used and the
+// results repeat bit for bit.
+#include "implicit.hpp"
+
+#include <algorithm>
+
+using namespace fvhip;
+
+namespace {
+
+constexpr int kBlock = 256;
+
+__global__ __launch_bounds__(kBlock) void k_ddt(int N, const int32_t* __restrict__ rowOffs,
+                                                const int32_t* __restrict__ diag,
+                                                const double* __restrict__ V,
+                                                const double* __restrict__ Told, double rdt,
+                                                double coeff, double* __restrict__ values,
+                                                double* __restrict__ rhs) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= N) return;
+    // EulerDdtScheme::fvmDdt: diag = rDeltaT*V, source = rDeltaT*T_old*V
+    const double v = V[c];
+    values[rowOffs[c] + diag[c]] += coeff * (rdt * v);
+    rhs[c] += coeff * ((rdt * Told[c]) * v);
+}
+
+__global__ __launch_bounds__(kBlock) void k_source(int N, const int32_t* __restrict__ rowOffs,
+                                                   const int32_t* __restrict__ diag,
+                                                   const double* __restrict__ V,
+                                                   const double* __restrict__ cf, double coeff,
+                                                   double* __restrict__ values) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= N) return;
+    values[rowOffs[c] + diag[c]] += coeff * (cf[c] * V[c]);
+}
+
+__global__ __launch_bounds__(kBlock) void k_explicit(int N, const double* __restrict__ V,
+                                                     const double* __restrict__ E, double coeff,
+                                                     double* __restrict__ rhs) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= N) return;
+    rhs[c] -= coeff * (E[c] * V[c]);
+}
+
+struct Stencil {
+    const int32_t *off, *idx, *slot;
+};
+
+__global__ __launch_bounds__(kBlock) void k_laplacian(
+    int N, int F, Stencil st, const int32_t* __restrict__ rowOffs,
+    const int32_t* __restrict__ diag, const double* __restrict__ magSf,
+    const double* __restrict__ nodc, const double* __restrict__ magSfb,
+    const double* __restrict__ dcb, const int8_t* __restrict__ kind,
+    const double* __restrict__ gamma, const double* __restrict__ Tb,
+    const double* __restrict__ Tg, double coeff, double* __restrict__ values,
+    double* __restrict__ rhs) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= N) return;
+    const int row = rowOffs[c];
+    double dsum = 0.0, bsum = 0.0;
+    const int e1 = st.off[c + 1];
+    for (int k = st.off[c]; k < e1; ++k) {
+        const int e = st.idx[k];
+        if (e >= F) {
+            const int b = e - F;
+            const double gm = gamma[e] * magSfb[b];
+            if (kind[b] == FV_BC_FIXED_GRADIENT) {
+                bsum -= gm * (Tg ? Tg[b] : 0.0);
+            } else {
+                const double ic = gm * dcb[b];
+                dsum -= ic;
+                bsum -= ic * Tb[b];
+            }
+        } else {
+            const int f = e >= 0 ? e : ~e;
+            const double up = nodc[f] * (gamma[f] * magSf[f]);
+            values[row + st.slot[k]] += coeff * up;
+            dsum -= up;
+        }
+    }
+    values[row + diag[c]] += coeff * dsum;
+    rhs[c] += coeff * bsum;
+}
+
+__global__ __launch_bounds__(kBlock) void k_div(
+    int N, int F, int scheme, Stencil st, const int32_t* __restrict__ rowOffs,
+    const int32_t* __restrict__ diag, const double* __restrict__ w,
+    const double* __restrict__ dcb, const int8_t* __restrict__ kind,
+    const double* __restrict__ phi, const double* __restrict__ Tb,
+    const double* __restrict__ Tg, double coeff, double* __restrict__ values,
+    double* __restrict__ rhs) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= N) return;
+    const int row = rowOffs[c];
+    double dsum = 0.0, bsum = 0.0;
+    const int e1 = st.off[c + 1];
+    for (int k = st.off[c]; k < e1; ++k) {
+        const int e = st.idx[k];
+        if (e >= F) {
+            const int b = e - F;
+            const double pf = phi[e];
+            if (kind[b] == FV_BC_FIXED_GRADIENT) {
+                dsum += pf;
+                bsum -= pf * ((Tg ? Tg[b] : 0.0) / dcb[b]);
+            } else {
+                bsum -= pf * Tb[b];
+            }
+        } else {
+            const int f = e >= 0 ? e : ~e;
+            const double ph = phi[f];
+            const double lam = scheme == 0 ? w[f] : (ph >= 0.0 ? 1.0 : 0.0);
+            const double lower = -lam * ph;
+            const double upper = lower + ph;
+            // gaussConvectionScheme::fvmDiv: owner row gets upper in column N,
+            // neighbour row gets lower in column P; negSumDiag
+            if (e >= 0) {
+                values[row + st.slot[k]] += coeff * upper;
+                dsum -= lower;
+            } else {
+                values[row + st.slot[k]] += coeff * lower;
+                dsum -= upper;
+            }
+        }
+    }
+    values[row + diag[c]] += coeff * dsum;
+    rhs[c] += coeff * bsum;
+}
+
+__global__ void k_set_reference(const int32_t* __restrict__ rowOffs,
+                                const int32_t* __restrict__ diag, int cell, double value,
+                                double* __restrict__ values, double* __restrict__ rhs) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        const int d = rowOffs[cell] + diag[cell];
+        const double dv = values[d];
+        rhs[cell] += dv * value;
+        values[d] += dv;
+    }
+}
+
+// res = A.x - b, one thread per row, row entries summed in column order
+__global__ __launch_bounds__(kBlock) void k_residual(int N, const int32_t* __restrict__ rowOffs,
+                                                     const int32_t* __restrict__ col,
+                                                     const double* __restrict__ values,
+                                                     const double* __restrict__ b,
+                                                     const double* __restrict__ x,
+                                                     double* __restrict__ res) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= N) return;
+    double s = 0.0;
+    const int e = rowOffs[r + 1];
+    for (int k = rowOffs[r]; k < e; ++k) s += values[k] * x[col[k]];
+    res[r] = s - b[r];
+}
+
+fv_mesh* mesh_of(const fv_linsys* ls) { return ls->sp->mesh; }
+
+void check_ls(const fv_linsys* ls) { FV_REQUIRE(ls && ls->sp && ls->sp->mesh, "NULL linear system"); }
+
+void check_bc(const fv_linsys* ls, const fv_bc* bc) {
+    FV_REQUIRE(bc && bc->mesh == mesh_of(ls), "boundary conditions belong to another mesh");
+}
+
+}  // namespace
+
+extern "C" {
+
+int fv_sparsity_create(fv_mesh* m, fv_sparsity** out) {
+    fv_sparsity* sp = nullptr;
+    int rc = guarded([&] {
+        FV_REQUIRE(m && out, "NULL argument");
+        const int64_t N = m->nCells, F = m->nInt;
+        std::vector<int32_t> off(N + 1), idx(m->cfIdx.n), oth(m->cfOther.n);
+        m->cfOff.download(off.data());
+        m->cfIdx.download(idx.data());
+        m->cfOther.download(oth.data());
+        std::vector<int32_t> rowOffs(N + 1, 0), diag(N), oo(F), no(F), slot(idx.size(), -1);
+        for (int64_t c = 0; c < N; ++c) {
+            int32_t nint = 0;
+            for (int32_t k = off[c]; k < off[c + 1]; ++k)
+                if (idx[k] < F) ++nint;
+            rowOffs[c + 1] = rowOffs[c] + nint + 1;
+        }
+        std::vector<int32_t> col(rowOffs[N]);
+        for (int64_t c = 0; c < N; ++c) {
+            int32_t j = 0, row = rowOffs[c];
+            int phase = 0;  // 0: lower entries, 1: upper entries, 2: boundary entries
+            bool diagDone = false;
+            for (int32_t k = off[c]; k < off[c + 1]; ++k) {
+                const int32_t e = idx[k];
+                const int ph = e < 0 ? 0 : (e < F ? 1 : 2);
+                FV_REQUIRE(ph >= phase,
+                           "mesh faces are not in upper-triangular order (owner < neighbour, "
+                           "sorted by owner then neighbour)");
+                phase = ph;
+                if (ph >= 1 && !diagDone) {
+                    diag[c] = j;
+                    col[row + j++] = (int32_t)c;
+                    diagDone = true;
+                }
+                if (ph == 0) {
+                    no[~e] = j;
+                    slot[k] = j;
+                    col[row + j++] = oth[k];
+                } else if (ph == 1) {
+                    oo[e] = j;
+                    slot[k] = j;
+                    col[row + j++] = oth[k];
+                }
+            }
+            if (!diagDone) {
+                diag[c] = j;
+                col[row + j++] = (int32_t)c;
+            }
+            for (int32_t t = 1; t < j; ++t)
+                FV_REQUIRE(col[row + t] > col[row + t - 1], "duplicate or unsorted columns in a row");
+        }
+        sp = new fv_sparsity();
+        sp->mesh = m;
+        sp->nRows = (int32_t)N;
+        sp->nnz = rowOffs[N];
+        fv_ctx* ctx = m->ctx;
+        sp->rowOffs.upload(ctx, rowOffs.data(), rowOffs.size());
+        sp->colIdxs.upload(ctx, col.data(), col.size());
+        sp->diagOffset.upload(ctx, diag.data(), diag.size());
+        sp->ownerOffset.upload(ctx, oo.data(), oo.size());
+        sp->neighbourOffset.upload(ctx, no.data(), no.size());
+        sp->entrySlot.upload(ctx, slot.data(), slot.size());
+        FV_HIP(hipStreamSynchronize(ctx->stream));
+        *out = sp;
+    });
+    if (rc != FV_OK) delete sp;
+    return rc;
+}
+
+int fv_sparsity_destroy(fv_sparsity* sp) {
+    return guarded([&] { delete sp; });
+}
+
+int fv_sparsity_sizes(const fv_sparsity* sp, int64_t* s) {
+    return guarded([&] {
+        FV_REQUIRE(sp && s, "NULL argument");
+        s[0] = sp->nRows;
+        s[1] = sp->nnz;
+    });
+}
+
+int fv_sparsity_get(const fv_sparsity* sp, int which, int32_t* h) {
+    return guarded([&] {
+        FV_REQUIRE(sp && h, "NULL argument");
+        switch (which) {
+            case 0: sp->rowOffs.download(h); break;
+            case 1: sp->colIdxs.download(h); break;
+            case 2: sp->diagOffset.download(h); break;
+            case 3: sp->ownerOffset.download(h); break;
+            case 4: sp->neighbourOffset.download(h); break;
+            default: FV_REQUIRE(false, "unknown sparsity array");
+        }
+    });
+}
+
+int fv_linsys_create(fv_sparsity* sp, fv_linsys** out) {
+    fv_linsys* ls = nullptr;
+    int rc = guarded([&] {
+        FV_REQUIRE(sp && out, "NULL argument");
+        fv_ctx* ctx = sp->mesh->ctx;
+        ls = new fv_linsys();
+        ls->sp = sp;
+        ls->values.alloc(ctx, sp->nnz);
+        ls->rhs.alloc(ctx, sp->nRows);
+        FV_HIP(hipMemsetAsync(ls->values.p, 0, sp->nnz * sizeof(double), ctx->stream));
+        FV_HIP(hipMemsetAsync(ls->rhs.p, 0, sp->nRows * sizeof(double), ctx->stream));
+        *out = ls;
+    });
+    if (rc != FV_OK) delete ls;
+    return rc;
+}
+
+int fv_linsys_destroy(fv_linsys* ls) {
+    return guarded([&] { delete ls; });
+}
+
+void* fv_linsys_values(fv_linsys* ls) { return ls ? (void*)ls->values.p : nullptr; }
+void* fv_linsys_rhs(fv_linsys* ls) { return ls ? (void*)ls->rhs.p : nullptr; }
+
+int fv_linsys_zero(fv_linsys* ls) {
+    return guarded([&] {
+        check_ls(ls);
+        hipStream_t s = mesh_of(ls)->ctx->stream;
+        FV_HIP(hipMemsetAsync(ls->values.p, 0, ls->values.n * sizeof(double), s));
+        FV_HIP(hipMemsetAsync(ls->rhs.p, 0, ls->rhs.n * sizeof(double), s));
+    });
+}
+
+int fv_assemble_ddt_euler(fv_linsys* ls, const void* Told, double dt, double coeff) {
+    return guarded([&] {
+        check_ls(ls);
+        FV_REQUIRE(Told && dt > 0.0, "bad ddt arguments");
+        fv_mesh* m = mesh_of(ls);
+        fv_sparsity* sp = ls->sp;
+        k_ddt<<<grid_for(m->nCells, kBlock), kBlock, 0, m->ctx->stream>>>(
+            m->nCells, sp->rowOffs.p, sp->diagOffset.p, m->V.p, (const double*)Told, 1.0 / dt,
+            coeff, ls->values.p, ls->rhs.p);
+        FV_LAUNCH_CHECK();
+    });
+}
+
+int fv_assemble_source(fv_linsys* ls, const void* cf, double coeff) {
+    return guarded([&] {
+        check_ls(ls);
+        FV_REQUIRE(cf, "NULL coefficient field");
+        fv_mesh* m = mesh_of(ls);
+        fv_sparsity* sp = ls->sp;
+        k_source<<<grid_for(m->nCells, kBlock), kBlock, 0, m->ctx->stream>>>(
+            m->nCells, sp->rowOffs.p, sp->diagOffset.p, m->V.p, (const double*)cf, coeff,
+            ls->values.p);
+        FV_LAUNCH_CHECK();
+    });
+}
+
+int fv_assemble_explicit(fv_linsys* ls, const void* E, double coeff) {
+    return guarded([&] {
+        check_ls(ls);
+        FV_REQUIRE(E, "NULL explicit field");
+        fv_mesh* m = mesh_of(ls);
+        k_explicit<<<grid_for(m->nCells, kBlock), kBlock, 0, m->ctx->stream>>>(
+            m->nCells, m->V.p, (const double*)E, coeff, ls->rhs.p);
+        FV_LAUNCH_CHECK();
+    });
+}
+
+int fv_assemble_laplacian(fv_linsys* ls, const void* gamma, const fv_bc* bc, const void* Tb,
+                          const void* Tg, double coeff) {
+    return guarded([&] {
+        check_ls(ls);
+        check_bc(ls, bc);
+        fv_mesh* m = mesh_of(ls);
+        FV_REQUIRE(gamma && (Tb || m->nB == 0), "NULL field");
+        fv_sparsity* sp = ls->sp;
+        Stencil st{m->cfOff.p, m->cfIdx.p, sp->entrySlot.p};
+        k_laplacian<<<grid_for(m->nCells, kBlock), kBlock, 0, m->ctx->stream>>>(
+            m->nCells, m->nInt, st, sp->rowOffs.p, sp->diagOffset.p, m->magSf.p, m->nodc.p,
+            m->magSfb.p, m->dcb.p, bc->faceKind.p, (const double*)gamma, (const double*)Tb,
+            (const double*)Tg, coeff, ls->values.p, ls->rhs.p);
+        FV_LAUNCH_CHECK();
+    });
+}
+
+int fv_assemble_div(fv_linsys* ls, fv_scheme scheme, const void* phi, const fv_bc* bc,
+                    const void* Tb, const void* Tg, double coeff) {
+    return guarded([&] {
+        check_ls(ls);
+        check_bc(ls, bc);
+        FV_REQUIRE(scheme == FV_LINEAR || scheme == FV_UPWIND, "unknown scheme");
+        fv_mesh* m = mesh_of(ls);
+        FV_REQUIRE(phi && (Tb || m->nB == 0), "NULL field");
+        fv_sparsity* sp = ls->sp;
+        Stencil st{m->cfOff.p, m->cfIdx.p, sp->entrySlot.p};
+        k_div<<<grid_for(m->nCells, kBlock), kBlock, 0, m->ctx->stream>>>(
+            m->nCells, m->nInt, scheme, st, sp->rowOffs.p, sp->diagOffset.p, m->w.p, m->dcb.p,
+            bc->faceKind.p, (const double*)phi, (const double*)Tb, (const double*)Tg, coeff,
+            ls->values.p, ls->rhs.p);
+        FV_LAUNCH_CHECK();
+    });
+}
+
+int fv_set_reference(fv_linsys* ls, int cell, double value) {
+    return guarded([&] {
+        check_ls(ls);
+        fv_mesh* m = mesh_of(ls);
+        FV_REQUIRE(cell >= 0 && cell < m->nCells, "reference cell out of range");
+        k_set_reference<<<1, 64, 0, m->ctx->stream>>>(ls->sp->rowOffs.p, ls->sp->diagOffset.p,
+                                                      cell, value, ls->values.p, ls->rhs.p);
+        FV_LAUNCH_CHECK();
+    });
+}
+
+int fv_residual(fv_linsys* ls, const void* x, void* res) {
+    return guarded([&] {
+        check_ls(ls);
+        FV_REQUIRE(x && res && x != res, "NULL or aliased vector");
+        fv_mesh* m = mesh_of(ls);
+        k_residual<<<grid_for(m->nCells, kBlock), kBlock, 0, m->ctx->stream>>>(
+            m->nCells, ls->sp->rowOffs.p, ls->sp->colIdxs.p, ls->values.p, ls->rhs.p,
+            (const double*)x, (double*)res);
+        FV_LAUNCH_CHECK();
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,24 @@
+// implicit.hpp -- sparsity pattern and linear system handles.
+#pragma once
+
+#include "mesh.hpp"
+
+// la::SparsityPattern (expression.hpp:35; test_sparsityPattern.cpp:36-62):
+// CSR of the face graph, columns ascending per row; row c is
+// [lower neighbours | c | upper neighbours] in the cell's stencil order.
+struct fv_sparsity {
+    fv_mesh* mesh = nullptr;
+    int32_t nRows = 0;
+    int64_t nnz = 0;
+    fvhip::DBuf<int32_t> rowOffs, colIdxs, diagOffset, ownerOffset, neighbourOffset;
+    // per stencil entry k (cell->face stencil of the mesh): CSR slot of the entry's
+    // coefficient in the cell's row (-1 for boundary entries)
+    fvhip::DBuf<int32_t> entrySlot;
+};
+
+// la::LinearSystem<scalar, localIdx>: values[nnz], rhs[N] (FP64), with the
+// convention A.x - b = (operator applied to x) * V.
+struct fv_linsys {
+    fv_sparsity* sp = nullptr;
+    fvhip::DBuf<double> values, rhs;
+};

@@ -59,6 +59,28 @@ __global__ void k_uniform(S* __restrict__ p, int64_t n, double lo, double hi, ui
     for (; i < n; i += s) p[i] = (S)(lo + (hi - lo) * rng_u01(seed, stream, (uint64_t)i));
 }
 
+template <typename S>
+__global__ void k_permute(int64_t n, int nc, int mode, int oriented,
+                          const int32_t* __restrict__ map, const S* __restrict__ src,
+                          S* __restrict__ dst) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t st = (int64_t)gridDim.x * blockDim.x;
+    for (; i < n; i += st) {
+        const int32_t e = map[i];
+        const int64_t j = e >= 0 ? e : ~e;
+        const bool neg = oriented && e < 0;
+        for (int k = 0; k < nc; ++k) {
+            if (mode == 0) {
+                const S v = src[j * nc + k];
+                dst[i * nc + k] = neg ? -v : v;
+            } else {
+                const S v = src[i * nc + k];
+                dst[j * nc + k] = neg ? -v : v;
+            }
+        }
+    }
+}
+
 }  // namespace fvhip
 
 using namespace fvhip;
@@ -203,6 +225,23 @@ int fv_convert(fv_ctx* ctx, fv_prec to, void* d_dst, fv_prec from, const void* d
             k_convert<double, double><<<g, 256, 0, ctx->stream>>>((double*)d_dst, (const double*)d_src, n);
         else
             k_convert<float, float><<<g, 256, 0, ctx->stream>>>((float*)d_dst, (const float*)d_src, n);
+        FV_LAUNCH_CHECK();
+    });
+}
+
+int fv_permute(fv_ctx* ctx, fv_prec prec, int64_t n, int ncomp, int mode, int oriented,
+               const int32_t* map, const void* src, void* dst) {
+    return guarded([&] {
+        FV_REQUIRE(ctx && n >= 0 && ncomp >= 1 && (mode == 0 || mode == 1), "bad argument");
+        if (!n) return;
+        FV_REQUIRE(map && src && dst && src != dst, "NULL or aliased pointer");
+        int g = grid_for(n, 256, 8192);
+        if (prec == FV_F64)
+            k_permute<double><<<g, 256, 0, ctx->stream>>>(n, ncomp, mode, oriented, map,
+                                                          (const double*)src, (double*)dst);
+        else
+            k_permute<float><<<g, 256, 0, ctx->stream>>>(n, ncomp, mode, oriented, map,
+                                                         (const float*)src, (float*)dst);
         FV_LAUNCH_CHECK();
     });
 }

@@ -1,0 +1,167 @@
+"""Linear algebra handles with NeoN's la names: SparsityPattern, LinearSystem,
+computeResidual and the Jacobi-PCG solve (libfvhip fv_sparsity_*, fv_linsys_*,
+fv_solve_cg)."""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _lib
+from ._lib import call, lib
+from .mesh import Mesh
+from .runtime import DeviceArray
+
+
+class SparsityPattern:
+    """la::SparsityPattern (expression.hpp:35; test_sparsityPattern.cpp:36-62)."""
+
+    def __init__(self, mesh: Mesh):
+        h = C.c_void_p()
+        call("fv_sparsity_create", mesh.handle, C.byref(h))
+        self.handle = h
+        self.mesh = mesh
+        s = np.zeros(2, np.int64)
+        call("fv_sparsity_sizes", h, C.c_void_p(s.ctypes.data))
+        self.n_rows, self.nnz = int(s[0]), int(s[1])
+        mesh.ctx.adopt(self)
+
+    @classmethod
+    def readOrCreate(cls, mesh: Mesh) -> "SparsityPattern":
+        sp = getattr(mesh, "_sparsity", None)
+        if sp is None or sp.handle is None:
+            sp = cls(mesh)
+            mesh._sparsity = sp
+        return sp
+
+    def _get(self, which: int, n: int) -> np.ndarray:
+        out = np.empty(n, np.int32)
+        call("fv_sparsity_get", self.handle, which, C.c_void_p(out.ctypes.data))
+        return out
+
+    def rowOffs(self):
+        return self._get(0, self.n_rows + 1)
+
+    def colIdxs(self):
+        return self._get(1, self.nnz)
+
+    def diagOffset(self):
+        return self._get(2, self.n_rows)
+
+    def ownerOffset(self):
+        return self._get(3, self.mesh.n_internal_faces)
+
+    def neighbourOffset(self):
+        return self._get(4, self.mesh.n_internal_faces)
+
+    def close(self):
+        if self.handle:
+            lib.fv_sparsity_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class LinearSystem:
+    """la::LinearSystem<scalar, localIdx> in FP64 (createEmptyLinearSystem)."""
+
+    def __init__(self, sp: SparsityPattern):
+        h = C.c_void_p()
+        call("fv_linsys_create", sp.handle, C.byref(h))
+        self.handle = h
+        self.sp = sp
+        self.mesh = sp.mesh
+        sp.mesh.ctx.adopt(self)
+
+    def zero(self):
+        call("fv_linsys_zero", self.handle)
+
+    def values(self) -> np.ndarray:
+        out = np.empty(self.sp.nnz)
+        call("fv_memcpy_d2h", self.mesh.ctx.handle, C.c_void_p(out.ctypes.data),
+             lib.fv_linsys_values(self.handle), out.nbytes)
+        return out
+
+    def rhs(self) -> np.ndarray:
+        out = np.empty(self.sp.n_rows)
+        call("fv_memcpy_d2h", self.mesh.ctx.handle, C.c_void_p(out.ctypes.data),
+             lib.fv_linsys_rhs(self.handle), out.nbytes)
+        return out
+
+    def diag(self) -> np.ndarray:
+        """FoamAdapter::diag (expression.hpp:181-201)."""
+        return self.values()[self.sp.rowOffs()[:-1] + self.sp.diagOffset()]
+
+    def scipy(self):
+        import scipy.sparse as sps
+        n = self.sp.n_rows
+        return sps.csr_matrix((self.values(), self.sp.colIdxs(), self.sp.rowOffs()), shape=(n, n))
+
+    def residual(self, x: DeviceArray, out: DeviceArray | None = None) -> DeviceArray:
+        """la::computeResidual: out = A.x - b (expression.hpp:204-220)."""
+        if out is None:
+            out = self.mesh.ctx.empty((self.sp.n_rows,))
+        call("fv_residual", self.handle, x.ptr, out.ptr)
+        return out
+
+    def close(self):
+        if self.handle:
+            lib.fv_linsys_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+@dataclass
+class SolverStats:
+    """la::SolverStats (expression.hpp:143-158) plus the refinement count and
+    the explicit FP64 residual at exit."""
+    initResNorm: float
+    finalResNorm: float
+    numIter: int
+    outerIter: int
+    trueResNorm: float
+    status: int
+
+
+def solver_opts(solver: dict | None) -> _lib.FvSolveOpts:
+    """Map an fvSolution-style solver dictionary (src/compatibility/fvSolution.cpp:106-159:
+    relTol -> relative_residual_norm, maxIter -> iteration, default 1000) to solver options.
+    'precision': 'mixed' selects FP32 CG inside FP64 iterative refinement."""
+    d = dict(solver or {})
+    o = _lib.FvSolveOpts()
+    o.rel_tol = float(d.get("relTol", d.get("rel_tol", 1e-6)))
+    o.max_iter = int(d.get("maxIter", d.get("maxIters", d.get("max_iter", 1000))))
+    mode = d.get("precision", "double")
+    if mode not in ("double", "mixed"):
+        raise ValueError(f"unknown precision {mode!r}")
+    o.mode = _lib.FV_SOLVE_F32_IR if mode == "mixed" else _lib.FV_SOLVE_F64
+    o.inner_rel_tol = float(d.get("innerRelTol", 1e-4))
+    o.max_outer = int(d.get("maxOuter", 50))
+    o.check_every = int(d.get("checkEvery", 8))
+    solver_name = d.get("solver", "PCG")
+    if solver_name not in ("PCG", "CG"):
+        raise ValueError(f"solver {solver_name!r} is not supported (PCG only)")
+    pre = d.get("preconditioner", "DIC")
+    if pre not in ("DIC", "Jacobi", "diagonal"):
+        raise ValueError(f"preconditioner {pre!r} is not supported (point Jacobi only)")
+    return o
+
+
+def solve_cg(ls: LinearSystem, x: DeviceArray, solver: dict | None = None,
+             raise_on_noconverge: bool = True) -> SolverStats:
+    o = solver_opts(solver)
+    st = _lib.FvSolverStats()
+    rc = lib.fv_solve_cg(ls.handle, x.ptr, C.byref(o), C.byref(st))
+    if rc not in (_lib.FV_OK, _lib.FV_ENOCONVERGE) or (rc != _lib.FV_OK and raise_on_noconverge):
+        raise _lib.FvError(rc, _lib.last_error())
+    return SolverStats(st.init_res, st.final_res, st.iters, st.outer_iters, st.true_res, rc)

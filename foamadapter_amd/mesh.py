@@ -20,7 +20,7 @@ import numpy as np
 
 from . import _lib
 from ._lib import call, lib
-from .runtime import Context
+from .runtime import Context, prec_of
 
 
 def _ptr(a: np.ndarray | None):
@@ -94,6 +94,24 @@ class PolyMesh:
         d.h_patch_size = arr([p.size for p in self.patches], np.int32)
         d.h_patch_empty = arr([1 if p.empty else 0 for p in self.patches], np.int32)
         return d, keep
+
+    def renumbered(self, method: int = 0) -> tuple["PolyMesh", np.ndarray, np.ndarray]:
+        """Reverse Cuthill-McKee renumbering (fv_polymesh_renumber).
+
+        Returns (mesh, cell_new_to_old, face_new_to_old); face entries are ~old
+        for faces whose orientation flipped."""
+        d, keep = self.desc()
+        nF, N = self.n_faces, self.n_cells
+        foff = np.empty(nF + 1, np.int32)
+        fpts = np.empty_like(self.face_points)
+        own = np.empty(nF, np.int32)
+        nbr = np.empty(self.n_internal_faces, np.int32)
+        cmap = np.empty(N, np.int32)
+        fmap = np.empty(nF, np.int32)
+        call("fv_polymesh_renumber", C.byref(d), method, _ptr(foff), _ptr(fpts), _ptr(own),
+             _ptr(nbr), _ptr(cmap), _ptr(fmap))
+        del keep
+        return PolyMesh(self.points, foff, fpts, own, nbr, list(self.patches)), cmap, fmap
 
     @staticmethod
     def hex(n: int | tuple[int, int, int], lengths=(1.0, 1.0, 1.0), jitter: float = 0.0,
@@ -238,13 +256,50 @@ class Mesh:
         self.patches = patches or []
         ctx.adopt(self)
 
+    cell_map = None   # device int32 [N]: device cell i holds caller cell cell_map[i]
+    face_map = None   # device int32 [F+B]: device face j holds caller face (~ if flipped)
+
     @classmethod
-    def from_poly(cls, ctx: Context, poly: PolyMesh) -> "Mesh":
+    def from_poly(cls, ctx: Context, poly: PolyMesh, renumber: bool = False) -> "Mesh":
+        """Device mesh from polyMesh primitives (geometry computed on the GPU).
+
+        renumber=True stores the mesh in reverse Cuthill-McKee order (what
+        OpenFOAM's renumberMesh does to a case); fields then cross the
+        boundary through to_device()/to_caller(), which apply the permutation."""
+        cmap = fmap = None
+        if renumber:
+            poly, cmap, fmap = poly.renumbered()
         d, keep = poly.desc()
         h = C.c_void_p()
         call("fv_mesh_create_poly", ctx.handle, C.byref(d), C.byref(h))
         del keep
-        return cls(ctx, h, poly.fv_patches() if poly.patches else None)
+        m = cls(ctx, h, poly.fv_patches() if poly.patches else None)
+        if renumber:
+            F = m.n_internal_faces
+            bidx = np.arange(F, F + m.n_boundary_faces, dtype=np.int32)
+            m.cell_map = ctx.upload(cmap)
+            m.face_map = ctx.upload(np.concatenate([fmap[:F], bidx]))
+            m.poly = poly
+        return m
+
+    def _permute(self, src, mode: int, kind: str, oriented: bool):
+        if self.cell_map is None:
+            return src
+        mp = self.cell_map if kind == "cell" else self.face_map
+        ncomp = int(np.prod(src.shape[1:], dtype=np.int64)) if len(src.shape) > 1 else 1
+        dst = self.ctx.empty(src.shape, src.dtype)
+        call("fv_permute", self.ctx.handle, prec_of(src.dtype), src.shape[0], ncomp, mode,
+             1 if oriented else 0, mp.ptr, src.ptr, dst.ptr)
+        return dst
+
+    def to_device(self, src, kind: str = "cell", oriented: bool = False):
+        """Caller-numbered device array -> this mesh's numbering (identity if not renumbered).
+        kind: 'cell' ([N,...]) or 'face' ([F+B], oriented=True for fluxes)."""
+        return self._permute(src, 0, kind, oriented)
+
+    def to_caller(self, src, kind: str = "cell", oriented: bool = False):
+        """This mesh's numbering -> the caller's numbering."""
+        return self._permute(src, 1, kind, oriented)
 
     @classmethod
     def from_geometry(cls, ctx: Context, g: dict) -> "Mesh":

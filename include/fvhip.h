@@ -157,6 +157,27 @@ int fv_hexmesh_generate(int32_t nx, int32_t ny, int32_t nz, double lx, double ly
                         int32_t* h_owner, int32_t* h_neighbour, int32_t* h_patch_start,
                         int32_t* h_patch_size);
 
+/* Locality renumbering of a polyMesh (host), the counterpart of OpenFOAM's
+ * renumberMesh (reverse Cuthill-McKee, method 0) that a case runs before the
+ * solver: cells reordered by a breadth-first sweep of the face graph, internal
+ * faces rewritten in upper-triangular order of the new numbering (a face whose
+ * owner/neighbour swap has its point list reversed), boundary faces unchanged.
+ * Outputs are sized like the input (points and patches are unchanged):
+ * cell_new_to_old[n_cells]; face_new_to_old[n_faces] = old face, or ~old when
+ * the face was flipped (oriented face fields change sign). */
+int fv_polymesh_renumber(const fv_polymesh_desc* in, int method, int32_t* h_face_offsets_out,
+                         int32_t* h_face_points_out, int32_t* h_owner_out,
+                         int32_t* h_neighbour_out, int32_t* h_cell_new_to_old,
+                         int32_t* h_face_new_to_old);
+/* Device permutation of n items of ncomp values (field transfer between a
+ * caller's numbering and a renumbered mesh):
+ *   mode 0 (gather):  dst[i] = s * src[j]
+ *   mode 1 (scatter): dst[j] = s * src[i]
+ * with e = d_map[i], j = e >= 0 ? e : ~e, s = e >= 0 ? +1 : -1 when `oriented`
+ * is non-zero, else +1. */
+int fv_permute(fv_ctx* ctx, fv_prec prec, int64_t n, int ncomp, int mode, int oriented,
+               const int32_t* d_map, const void* d_src, void* d_dst);
+
 /* Upload NeoN-layout geometry (readOpenFOAMMesh, meshAdapter.cpp:59-136). */
 int fv_mesh_create(fv_ctx* ctx, const fv_mesh_desc* desc, fv_mesh** out);
 /* Build a mesh from polyMesh primitives, computing face centres/areas, cell
@@ -223,6 +244,84 @@ int fv_correct_boundary(fv_ctx* ctx, const fv_mesh* mesh, fv_prec prec, const vo
  * src/algorithms/pressureVelocityCoupling.cpp:242-264): U[3N], Ub[3B] -> phi[F+B]. */
 int fv_flux(fv_ctx* ctx, const fv_mesh* mesh, fv_prec prec, const void* d_U, const void* d_Ub,
             void* d_phi);
+
+/* ---- implicit path: sparsity pattern, linear system, assembly, solve ----- */
+
+typedef struct fv_sparsity fv_sparsity;
+typedef struct fv_linsys fv_linsys;
+
+/* la::SparsityPattern::readOrCreate(mesh) / la::createSparsity (expression.hpp:35;
+ * test_sparsityPattern.cpp:36-62): CSR of the face graph with ascending columns
+ * per row; requires OpenFOAM upper-triangular face order. */
+int fv_sparsity_create(fv_mesh* mesh, fv_sparsity** out);
+int fv_sparsity_destroy(fv_sparsity* sp);
+/* sizes2 = {n_rows, nnz} (nnz = N + 2F) */
+int fv_sparsity_sizes(const fv_sparsity* sp, int64_t* h_sizes2);
+/* which: 0 rowOffs[N+1], 1 colIdxs[nnz], 2 diagOffset[N], 3 ownerOffset[F]
+ * (slot of column neighbour in row owner), 4 neighbourOffset[F] (slot of
+ * column owner in row neighbour) (pressureVelocityCoupling.cpp:87-107). */
+int fv_sparsity_get(const fv_sparsity* sp, int which, int32_t* h_out);
+
+/* la::createEmptyLinearSystem<scalar, localIdx>(mesh, sparsity)
+ * (bench_implicitOperators.cpp:109-125): FP64 values[nnz] and rhs[N], zeroed.
+ * Convention: A.x - b equals the assembled operators applied to x, times V
+ * (test_implicitOperators.cpp:92-100,254-262). */
+int fv_linsys_create(fv_sparsity* sp, fv_linsys** out);
+int fv_linsys_destroy(fv_linsys* ls);
+/* Device pointers of matrix().values() and rhs() (LinearSystem::view). */
+void* fv_linsys_values(fv_linsys* ls);
+void* fv_linsys_rhs(fv_linsys* ls);
+/* NeoN::fill(values, 0); fill(rhs, 0) (bench_implicitOperators.cpp:116-117). */
+int fv_linsys_zero(fv_linsys* ls);
+
+/* DdtOperator(Implicit).implicitOperation(ls, t, dt), Euler
+ * (test_implicitOperators.cpp:67-73): diag += coeff*V/dt, rhs += coeff*V/dt*T_old. */
+int fv_assemble_ddt_euler(fv_linsys* ls, const void* d_Told, double dt, double coeff);
+/* SourceTerm(Implicit, coeffField).implicitOperation(ls)
+ * (test_implicitOperators.cpp:115-153): diag += coeff*c_P*V_P. */
+int fv_assemble_source(fv_linsys* ls, const void* d_coeff_field, double coeff);
+/* An explicit operator's per-volume result E[N] moved to the right-hand side of
+ * the equation (dsl::exp terms of an Expression, neoIcoFoam.cpp:145):
+ * rhs -= coeff*E*V. */
+int fv_assemble_explicit(fv_linsys* ls, const void* d_E, double coeff);
+/* LaplacianOperator(Implicit, gamma, T, "Gauss linear uncorrected").implicitOperation(ls)
+ * (test_implicitOperators.cpp:242-249): off-diagonals coeff*gamma|Sf|nonOrthDeltaCoeffs,
+ * diagonal minus their sum, boundary faces folded into diag/rhs. */
+int fv_assemble_laplacian(fv_linsys* ls, const void* d_gamma, const fv_bc* bc, const void* d_Tb,
+                          const void* d_Tgrad, double coeff);
+/* DivOperator(Implicit, phi, T, "Gauss <scheme>").implicitOperation(ls)
+ * (test_implicitOperators.cpp:188-195): lower = -w phi, upper = lower + phi. */
+int fv_assemble_div(fv_linsys* ls, fv_scheme scheme, const void* d_phi, const fv_bc* bc,
+                    const void* d_Tb, const void* d_Tgrad, double coeff);
+/* PDESolver::SetReference (expression.hpp:102-111): rhs[c] += A_cc*value; A_cc *= 2. */
+int fv_set_reference(fv_linsys* ls, int cell, double value);
+/* la::computeResidual via FoamAdapter::applyOperator (expression.hpp:204-220):
+ * res = A.x - b. */
+int fv_residual(fv_linsys* ls, const void* d_x, void* d_res);
+
+typedef enum { FV_SOLVE_F64 = 0, FV_SOLVE_F32_IR = 1 } fv_solve_mode;
+typedef struct {
+    double rel_tol;       /* stop when ||r||/||b|| <= rel_tol (criteria relative_residual_norm) */
+    int32_t max_iter;     /* criteria iteration (default 1000, fvSolution.cpp:114-118) */
+    int32_t mode;         /* fv_solve_mode */
+    double inner_rel_tol; /* FP32 inner CG tolerance per refinement step (F32_IR) */
+    int32_t max_outer;    /* refinement steps (F32_IR) */
+    int32_t check_every;  /* host polls the stop flag every this many iterations */
+} fv_solve_opts;
+typedef struct {
+    double init_res;      /* SolverStats::initResNorm, relative */
+    double final_res;     /* SolverStats::finalResNorm, relative */
+    int32_t iters;        /* SolverStats::numIter (inner iterations for F32_IR) */
+    int32_t outer_iters;  /* refinement steps (1 for FP64) */
+    double true_res;      /* explicit FP64 ||b - A x|| / ||b|| at exit */
+} fv_solver_stats;
+/* Jacobi-preconditioned CG on the assembled system, x as initial guess and
+ * result (dsl::solve -> Ginkgo solver::Cg + preconditioner::Jacobi,
+ * src/compatibility/fvSolution.cpp:22-28,50-55,106-142).  FV_SOLVE_F32_IR runs
+ * the CG in FP32 for the correction and keeps r = b - A x, the update of x and
+ * the convergence test in FP64.  Returns FV_ENOCONVERGE (stats filled) when
+ * the tolerance is not reached. */
+int fv_solve_cg(fv_linsys* ls, void* d_x, const fv_solve_opts* opts, fv_solver_stats* stats);
 
 #ifdef __cplusplus
 }

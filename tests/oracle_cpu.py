@@ -24,11 +24,22 @@ def _load():
 lib = _load()
 
 
+class _Arg:
+    """ctypes argument that keeps its array alive for the duration of the call
+    (a bare c_void_p of a temporary conversion would dangle)."""
+
+    __slots__ = ("_as_parameter_", "arr")
+
+    def __init__(self, a):
+        self.arr = a
+        self._as_parameter_ = C.c_void_p(a.ctypes.data)
+
+
 def P(a):
     if a is None:
         return None
     assert a.flags["C_CONTIGUOUS"], "oracle arrays must be contiguous"
-    return C.c_void_p(a.ctypes.data)
+    return _Arg(a)
 
 
 def _f(x, dt=np.float64):
