@@ -261,7 +261,7 @@ def bench_poisson(ctx, n=128, seed=1003, repeats=3):
             ctx.sync()
             ctx.record(50)
             st = la.solve_cg(ls, x, {"relTol": 1e-8, "maxIter": 20000, "precision": mode,
-                                     "innerRelTol": 1e-4, "checkEvery": 32})
+                                     "checkEvery": 64})
             ctx.record(51)
             ms = ctx.elapsed_ms(50, 51)
             if best is None or ms < best[0]:

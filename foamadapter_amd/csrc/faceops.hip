@@ -152,6 +152,157 @@ __global__ __launch_bounds__(kBlock) void k_laplacian(
     out[c] = coeff * (acc / g.V[c]);
 }
 
+// ---- LDS-staged cell gathers (meshes in upper-triangular face order) --------
+//
+// A workgroup owns kCB consecutive cells [c0, c1).  In OpenFOAM face order the
+// internal faces those cells own are one contiguous range [fs, fe), so:
+//   phase 1: the workgroup streams that face range with coalesced loads,
+//            computes every face contribution once and keeps it in LDS;
+//   phase 2: one thread per cell accumulates, in ascending face order, its
+//            lower faces (owned by lower-numbered cells: read from LDS when the
+//            owner is in this workgroup, recomputed otherwise), its own faces
+//            (from LDS), then its boundary faces, and divides by V.
+// The accumulation order is the serial face order of the other kernels, so the
+// result is the same bit for bit.  On a Z-order renumbered mesh a workgroup is a
+// compact brick and only the faces on its lower sides are recomputed.
+// Workgroups are dispatched round-robin over the 8 XCDs; the block index is
+// remapped so each XCD walks one contiguous range of bricks and keeps the
+// faces it recomputes in its own L2.
+
+constexpr int kCB = 256;      // cells per workgroup
+constexpr int kMaxUp = 1024;  // LDS face slots per workgroup
+constexpr int kXcd = 8;
+
+struct Split {
+    const int32_t *own, *nbr, *ownStart, *lowOff, *lowFace, *lowOwn, *bOff, *bFace;
+};
+
+template <typename S>
+struct GradOp {
+    static constexpr int NC = 3;
+    const S *T, *w, *Sf, *Sfb, *Tb;
+    __device__ void face(int f, int o, int n, S* v) const {
+        const S TP = T[o], TN = T[n];
+        const S tf = w[f] * (TP - TN) + TN;
+        v[0] = Sf[3 * (int64_t)f] * tf;
+        v[1] = Sf[3 * (int64_t)f + 1] * tf;
+        v[2] = Sf[3 * (int64_t)f + 2] * tf;
+    }
+    __device__ void bface(int b, int, S* v) const {
+        const S tb = Tb[b];
+        v[0] = Sfb[3 * b] * tb;
+        v[1] = Sfb[3 * b + 1] * tb;
+        v[2] = Sfb[3 * b + 2] * tb;
+    }
+};
+
+template <typename S, int SCHEME>  // 0 linear, 1 upwind
+struct DivOp {
+    static constexpr int NC = 1;
+    const S *T, *w, *phi, *Tb;
+    int F;
+    __device__ void face(int f, int o, int n, S* v) const {
+        const S ph = phi[f];
+        const S lam = SCHEME == 0 ? w[f] : (ph >= S(0) ? S(1) : S(0));
+        const S TP = T[o], TN = T[n];
+        v[0] = ph * (lam * (TP - TN) + TN);
+    }
+    __device__ void bface(int b, int, S* v) const { v[0] = phi[F + b] * Tb[b]; }
+};
+
+template <typename S>
+struct IntegrateOp {  // surfaceIntegrate(phi): div with a unit field
+    static constexpr int NC = 1;
+    const S* phi;
+    int F;
+    __device__ void face(int f, int, int, S* v) const { v[0] = phi[f]; }
+    __device__ void bface(int b, int, S* v) const { v[0] = phi[F + b]; }
+};
+
+template <typename S>
+struct LapOp {
+    static constexpr int NC = 1;
+    const S *T, *gamma, *magSf, *nodc, *magSfb, *dcb, *Tb, *Tg;
+    const int8_t* kind;
+    int F;
+    __device__ void face(int f, int o, int n, S* v) const {
+        v[0] = (gamma[f] * magSf[f]) * (nodc[f] * (T[n] - T[o]));
+    }
+    __device__ void bface(int b, int c, S* v) const {
+        const S sn = kind[b] == FV_BC_FIXED_GRADIENT ? (Tg ? Tg[b] : S(0)) : dcb[b] * (Tb[b] - T[c]);
+        v[0] = (gamma[F + b] * magSfb[b]) * sn;
+    }
+};
+
+template <typename S, typename Op>
+__global__ __launch_bounds__(kCB) void k_gather(int N, int nBlocks, int perXcd, Split t, Op op,
+                                                const S* __restrict__ V, S coeff,
+                                                S* __restrict__ out) {
+    constexpr int NC = Op::NC;
+    __shared__ S flux[NC][kMaxUp];
+    const int lb = (blockIdx.x % kXcd) * perXcd + blockIdx.x / kXcd;
+    if (lb >= nBlocks) return;
+    const int c0 = lb * kCB, c1 = min(c0 + kCB, N);
+    const int fs = t.ownStart[c0], fe = t.ownStart[c1];
+    const int nUp = min(fe - fs, kMaxUp);
+    for (int i = threadIdx.x; i < nUp; i += kCB) {
+        const int f = fs + i;
+        S v[NC];
+        op.face(f, t.own[f], t.nbr[f], v);
+        for (int j = 0; j < NC; ++j) flux[j][i] = v[j];
+    }
+    __syncthreads();
+    const int c = c0 + threadIdx.x;
+    if (c >= c1) return;
+    S acc[NC];
+    for (int j = 0; j < NC; ++j) acc[j] = S(0);
+    const int l1 = t.lowOff[c + 1];
+    for (int k = t.lowOff[c]; k < l1; ++k) {
+        const int f = t.lowFace[k];
+        S v[NC];
+        const int i = f - fs;
+        if (i >= 0 && i < nUp) {
+            for (int j = 0; j < NC; ++j) v[j] = flux[j][i];
+        } else {
+            op.face(f, t.lowOwn[k], c, v);
+        }
+        for (int j = 0; j < NC; ++j) acc[j] -= v[j];
+    }
+    const int u1 = t.ownStart[c + 1];
+    for (int f = t.ownStart[c]; f < u1; ++f) {
+        S v[NC];
+        const int i = f - fs;
+        if (i < nUp) {
+            for (int j = 0; j < NC; ++j) v[j] = flux[j][i];
+        } else {
+            op.face(f, c, t.nbr[f], v);
+        }
+        for (int j = 0; j < NC; ++j) acc[j] += v[j];
+    }
+    const int b1 = t.bOff[c + 1];
+    for (int k = t.bOff[c]; k < b1; ++k) {
+        S v[NC];
+        op.bface(t.bFace[k], c, v);
+        for (int j = 0; j < NC; ++j) acc[j] += v[j];
+    }
+    const S vol = V[c];
+    for (int j = 0; j < NC; ++j) out[(int64_t)NC * c + j] = coeff * (acc[j] / vol);
+}
+
+Split split_of(const fv_mesh* m) {
+    return {m->own.p,     m->nbr.p,    m->ownStart.p, m->lowOff.p,
+            m->lowFace.p, m->lowOwn.p, m->bOff.p,     m->bFace.p};
+}
+
+template <typename S, typename Op>
+void launch_gather(fv_ctx* ctx, const fv_mesh* m, const Op& op, const S* V, S coeff, S* out) {
+    const int nBlocks = (m->nCells + kCB - 1) / kCB;
+    const int perXcd = (nBlocks + kXcd - 1) / kXcd;
+    k_gather<S, Op><<<perXcd * kXcd, kCB, 0, ctx->stream>>>(m->nCells, nBlocks, perXcd,
+                                                            split_of(m), op, V, coeff, out);
+    FV_LAUNCH_CHECK();
+}
+
 // ---- face-result operators -------------------------------------------------
 
 template <typename S>
@@ -227,6 +378,53 @@ __global__ __launch_bounds__(kBlock) void k_correct_bc(int B, const int32_t* __r
     Tb[b] = T[fc[b]] + g / (S)dcb[b];
 }
 
+// Operator drivers: the LDS-staged gather on upper-triangular meshes, the
+// stencil walk otherwise (same arithmetic, same order).
+template <typename S>
+void run_grad(fv_ctx* ctx, const fv_mesh* m, Geo<S> g, const S* T, const S* Tb, S coeff, S* out) {
+    if (m->upperTri && !m->forceWalk) {
+        launch_gather<S>(ctx, m, GradOp<S>{T, g.w, g.Sf, g.Sfb, Tb}, g.V, coeff, out);
+        return;
+    }
+    const int N = m->nCells;
+    k_grad<S><<<grid_for(N, kBlock), kBlock, 0, ctx->stream>>>(
+        N, m->nInt, m->cfOff.p, m->cfIdx.p, m->cfOther.p, g, T, Tb, coeff, out);
+    FV_LAUNCH_CHECK();
+}
+
+template <typename S>
+void run_div(fv_ctx* ctx, const fv_mesh* m, int scheme, Geo<S> g, const S* phi, const S* T,
+             const S* Tb, S coeff, S* out) {
+    if (m->upperTri && !m->forceWalk) {
+        if (!T)
+            launch_gather<S>(ctx, m, IntegrateOp<S>{phi, m->nInt}, g.V, coeff, out);
+        else if (scheme == FV_LINEAR)
+            launch_gather<S>(ctx, m, DivOp<S, 0>{T, g.w, phi, Tb, m->nInt}, g.V, coeff, out);
+        else
+            launch_gather<S>(ctx, m, DivOp<S, 1>{T, g.w, phi, Tb, m->nInt}, g.V, coeff, out);
+        return;
+    }
+    const int N = m->nCells;
+    k_div<S><<<grid_for(N, kBlock), kBlock, 0, ctx->stream>>>(
+        N, m->nInt, scheme, m->cfOff.p, m->cfIdx.p, m->cfOther.p, g, phi, T, Tb, coeff, out);
+    FV_LAUNCH_CHECK();
+}
+
+template <typename S>
+void run_lap(fv_ctx* ctx, const fv_mesh* m, Geo<S> g, const int8_t* kind, const S* gamma,
+             const S* T, const S* Tb, const S* Tg, S coeff, S* out) {
+    if (m->upperTri && !m->forceWalk) {
+        launch_gather<S>(ctx, m,
+                         LapOp<S>{T, gamma, g.magSf, g.nodc, g.magSfb, g.dcb, Tb, Tg, kind, m->nInt},
+                         g.V, coeff, out);
+        return;
+    }
+    const int N = m->nCells;
+    k_laplacian<S><<<grid_for(N, kBlock), kBlock, 0, ctx->stream>>>(
+        N, m->nInt, m->cfOff.p, m->cfIdx.p, m->cfOther.p, g, kind, gamma, T, Tb, Tg, coeff, out);
+    FV_LAUNCH_CHECK();
+}
+
 void check_common(fv_ctx* ctx, const fv_mesh* m, fv_prec prec) {
     FV_REQUIRE(ctx && m, "NULL ctx/mesh");
     FV_REQUIRE(m->ctx == ctx, "mesh belongs to another context");
@@ -272,16 +470,12 @@ int fv_grad_gauss(fv_ctx* ctx, const fv_mesh* mc, fv_prec prec, double coeff, co
         fv_mesh* m = const_cast<fv_mesh*>(mc);
         check_common(ctx, m, prec);
         FV_REQUIRE(T && out && (Tb || m->nB == 0), "NULL field");
-        const int N = m->nCells;
         if (prec == FV_F64)
-            k_grad<double><<<grid_for(N, kBlock), kBlock, 0, ctx->stream>>>(
-                N, m->nInt, m->cfOff.p, m->cfIdx.p, m->cfOther.p, geo64(m), (const double*)T,
-                (const double*)Tb, coeff, (double*)out);
+            run_grad<double>(ctx, m, geo64(m), (const double*)T, (const double*)Tb, coeff,
+                             (double*)out);
         else
-            k_grad<float><<<grid_for(N, kBlock), kBlock, 0, ctx->stream>>>(
-                N, m->nInt, m->cfOff.p, m->cfIdx.p, m->cfOther.p, geo32(m), (const float*)T,
-                (const float*)Tb, (float)coeff, (float*)out);
-        FV_LAUNCH_CHECK();
+            run_grad<float>(ctx, m, geo32(m), (const float*)T, (const float*)Tb, (float)coeff,
+                            (float*)out);
     });
 }
 
@@ -293,16 +487,12 @@ int fv_div_gauss(fv_ctx* ctx, const fv_mesh* mc, fv_scheme scheme, fv_prec prec,
         FV_REQUIRE(scheme == FV_LINEAR || scheme == FV_UPWIND, "unknown scheme");
         FV_REQUIRE(phi && out, "NULL field");
         FV_REQUIRE(!T || Tb || m->nB == 0, "boundary values are NULL");
-        const int N = m->nCells;
         if (prec == FV_F64)
-            k_div<double><<<grid_for(N, kBlock), kBlock, 0, ctx->stream>>>(
-                N, m->nInt, scheme, m->cfOff.p, m->cfIdx.p, m->cfOther.p, geo64(m),
-                (const double*)phi, (const double*)T, (const double*)Tb, coeff, (double*)out);
+            run_div<double>(ctx, m, scheme, geo64(m), (const double*)phi, (const double*)T,
+                            (const double*)Tb, coeff, (double*)out);
         else
-            k_div<float><<<grid_for(N, kBlock), kBlock, 0, ctx->stream>>>(
-                N, m->nInt, scheme, m->cfOff.p, m->cfIdx.p, m->cfOther.p, geo32(m),
-                (const float*)phi, (const float*)T, (const float*)Tb, (float)coeff, (float*)out);
-        FV_LAUNCH_CHECK();
+            run_div<float>(ctx, m, scheme, geo32(m), (const float*)phi, (const float*)T,
+                           (const float*)Tb, (float)coeff, (float*)out);
     });
 }
 
@@ -314,18 +504,14 @@ int fv_laplacian_uncorrected(fv_ctx* ctx, const fv_mesh* mc, fv_prec prec, doubl
         check_common(ctx, m, prec);
         check_bc(m, bc);
         FV_REQUIRE(gamma && T && out && (Tb || m->nB == 0), "NULL field");
-        const int N = m->nCells;
         if (prec == FV_F64)
-            k_laplacian<double><<<grid_for(N, kBlock), kBlock, 0, ctx->stream>>>(
-                N, m->nInt, m->cfOff.p, m->cfIdx.p, m->cfOther.p, geo64(m), bc->faceKind.p,
-                (const double*)gamma, (const double*)T, (const double*)Tb, (const double*)Tg,
-                coeff, (double*)out);
+            run_lap<double>(ctx, m, geo64(m), bc->faceKind.p, (const double*)gamma,
+                            (const double*)T, (const double*)Tb, (const double*)Tg, coeff,
+                            (double*)out);
         else
-            k_laplacian<float><<<grid_for(N, kBlock), kBlock, 0, ctx->stream>>>(
-                N, m->nInt, m->cfOff.p, m->cfIdx.p, m->cfOther.p, geo32(m), bc->faceKind.p,
-                (const float*)gamma, (const float*)T, (const float*)Tb, (const float*)Tg,
-                (float)coeff, (float*)out);
-        FV_LAUNCH_CHECK();
+            run_lap<float>(ctx, m, geo32(m), bc->faceKind.p, (const float*)gamma,
+                           (const float*)T, (const float*)Tb, (const float*)Tg, (float)coeff,
+                           (float*)out);
     });
 }
 

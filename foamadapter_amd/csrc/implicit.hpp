@@ -14,6 +14,13 @@ struct fv_sparsity {
     // per stencil entry k (cell->face stencil of the mesh): CSR slot of the entry's
     // coefficient in the cell's row (-1 for boundary entries)
     fvhip::DBuf<int32_t> entrySlot;
+    // sliced-ELL copy of the structure for the solver's SpMV (slices of 64 rows,
+    // one wavefront each; entry k of row 64s+l at sellOff[s] + 64k + l):
+    // column and source CSR index (-1 for padding)
+    static constexpr int kSlice = 64;
+    int64_t sellTotal = 0;
+    fvhip::DBuf<int64_t> sellOff;
+    fvhip::DBuf<int32_t> sellCol, sellSrc;
 };
 
 // la::LinearSystem<scalar, localIdx>: values[nnz], rhs[N] (FP64), with the

@@ -211,6 +211,42 @@ void build_stencil(fv_mesh* m, const int32_t* h_own, const int32_t* h_nbr,
     m->cfOff.upload(m->ctx, off.data(), off.size());
     m->cfIdx.upload(m->ctx, idx.data(), idx.size());
     m->cfOther.upload(m->ctx, oth.data(), oth.size());
+    // split stencil when the faces are in upper-triangular order
+    bool upper = true;
+    for (int64_t f = 0; f < F && upper; ++f) {
+        if (h_own[f] >= h_nbr[f]) upper = false;
+        if (f > 0 && (h_own[f] < h_own[f - 1] ||
+                      (h_own[f] == h_own[f - 1] && h_nbr[f] <= h_nbr[f - 1])))
+            upper = false;
+    }
+    m->upperTri = upper;
+    if (upper) {
+        std::vector<int32_t> ownStart(N + 1, 0), lowOff(N + 1, 0), bOff(N + 1, 0);
+        for (int64_t f = 0; f < F; ++f) {
+            ownStart[h_own[f] + 1]++;
+            lowOff[h_nbr[f] + 1]++;
+        }
+        for (int64_t b = 0; b < B; ++b) bOff[h_faceCells[b] + 1]++;
+        for (int64_t c = 0; c < N; ++c) {
+            ownStart[c + 1] += ownStart[c];
+            lowOff[c + 1] += lowOff[c];
+            bOff[c + 1] += bOff[c];
+        }
+        std::vector<int32_t> lowFace(F), lowOwn(F), bFace(B);
+        std::vector<int32_t> lf(lowOff.begin(), lowOff.end() - 1), bf(bOff.begin(), bOff.end() - 1);
+        for (int64_t f = 0; f < F; ++f) {
+            int32_t k = lf[h_nbr[f]]++;
+            lowFace[k] = (int32_t)f;
+            lowOwn[k] = h_own[f];
+        }
+        for (int64_t b = 0; b < B; ++b) bFace[bf[h_faceCells[b]]++] = (int32_t)b;
+        m->ownStart.upload(m->ctx, ownStart.data(), ownStart.size());
+        m->lowOff.upload(m->ctx, lowOff.data(), lowOff.size());
+        m->lowFace.upload(m->ctx, lowFace.data(), lowFace.size());
+        m->lowOwn.upload(m->ctx, lowOwn.data(), lowOwn.size());
+        m->bOff.upload(m->ctx, bOff.data(), bOff.size());
+        m->bFace.upload(m->ctx, bFace.data(), bFace.size());
+    }
     FV_HIP(hipStreamSynchronize(m->ctx->stream));
 }
 
@@ -442,6 +478,20 @@ int fv_mesh_get(const fv_mesh* m, fv_geom which, void* h) {
             }
             default: FV_REQUIRE(false, "unknown geometry id");
         }
+    });
+}
+
+int fv_mesh_set_gather(fv_mesh* m, int mode) {
+    return guarded([&] {
+        FV_REQUIRE(m && (mode == 0 || mode == 1), "bad argument");
+        m->forceWalk = mode == 1;
+    });
+}
+
+int fv_mesh_get_gather(const fv_mesh* m, int* mode) {
+    return guarded([&] {
+        FV_REQUIRE(m && mode, "NULL argument");
+        *mode = (m->upperTri && !m->forceWalk) ? 2 : 1;
     });
 }
 

@@ -34,6 +34,15 @@ struct fv_mesh {
     fvhip::DBuf<double> Sfb, Cfb, magSfb, nfb, deltab, wb, dcb, nodcb;
     fvhip::DBuf<int32_t> patchOffD;
     fvhip::DBuf<int32_t> cfOff, cfIdx, cfOther;
+    // Split stencil for meshes in upper-triangular face order (owner < neighbour,
+    // sorted by owner then neighbour), used by the LDS-staged face kernels:
+    //   ownStart[N+1]: cell c owns internal faces [ownStart[c], ownStart[c+1])
+    //   lowOff[N+1], lowFace[F], lowOwn[F]: faces where c is the neighbour,
+    //     ascending, with their owner
+    //   bOff[N+1], bFace[B]: boundary faces of c, ascending
+    bool upperTri = false;
+    bool forceWalk = false;  // run the stencil-walk kernels even when upperTri
+    fvhip::DBuf<int32_t> ownStart, lowOff, lowFace, lowOwn, bOff, bFace;
     MeshF32 f32;
 };
 

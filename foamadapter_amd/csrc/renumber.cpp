@@ -1,9 +1,13 @@
 // renumber.cpp -- locality renumbering of a polyMesh (host), the counterpart of
-// OpenFOAM's renumberMesh utility with its default Cuthill-McKee method.
+// OpenFOAM's renumberMesh utility.
 //
-// Cells are ordered by a breadth-first sweep of the face graph started from a
-// pseudo-peripheral cell, visiting neighbours by increasing degree (reverse
-// Cuthill-McKee: the sweep order reversed).  Internal faces are then rewritten
+// Method 0 (renumberMesh's default Cuthill-McKee): cells ordered by a
+// breadth-first sweep of the face graph started from a pseudo-peripheral cell,
+// visiting neighbours by increasing degree, the sweep order reversed.
+// Method 1 (Z-order): cells sorted by the Morton code of their centres, so that
+// any aligned run of 2^k cells is a compact brick of space -- the property the
+// LDS-staged face kernels use (a 256-cell workgroup owns most of its faces).
+// Internal faces are then rewritten
 // in upper-triangular order of the new numbering (owner < neighbour, sorted by
 // owner then neighbour); a face whose owner and neighbour change sides has its
 // point list reversed (face::reverseFace keeps the first point), so its area
@@ -68,11 +72,66 @@ int32_t bfs(const Graph& g, int32_t start, std::vector<int32_t>& order, std::vec
     return order.back();
 }
 
+// spread the low 21 bits of v to every third bit
+inline uint64_t spread3(uint64_t v) {
+    v &= 0x1fffff;
+    v = (v | v << 32) & 0x1f00000000ffffull;
+    v = (v | v << 16) & 0x1f0000ff0000ffull;
+    v = (v | v << 8) & 0x100f00f00f00f00full;
+    v = (v | v << 4) & 0x10c30c30c30c30c3ull;
+    v = (v | v << 2) & 0x1249249249249249ull;
+    return v;
+}
+
+// Z-order (Morton) of approximate cell centres (average of the cell's face-point
+// averages), quantised to 2^21 levels per axis over the points' bounding box.
+void morton_order(const fv_polymesh_desc* d, std::vector<int32_t>& order) {
+    const int64_t N = d->n_cells, nF = d->n_faces, F = d->n_internal_faces, nP = d->n_points;
+    double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+    for (int64_t p = 0; p < nP; ++p)
+        for (int k = 0; k < 3; ++k) {
+            lo[k] = std::min(lo[k], d->h_points[3 * p + k]);
+            hi[k] = std::max(hi[k], d->h_points[3 * p + k]);
+        }
+    std::vector<double> cs(3 * N, 0.0);
+    std::vector<int32_t> cnt(N, 0);
+    for (int64_t f = 0; f < nF; ++f) {
+        const int32_t a = d->h_face_offsets[f], n = d->h_face_offsets[f + 1] - a;
+        double fc[3] = {0, 0, 0};
+        for (int32_t i = 0; i < n; ++i)
+            for (int k = 0; k < 3; ++k) fc[k] += d->h_points[3 * (int64_t)d->h_face_points[a + i] + k];
+        for (int k = 0; k < 3; ++k) fc[k] /= n;
+        const int32_t o = d->h_owner[f];
+        for (int k = 0; k < 3; ++k) cs[3 * o + k] += fc[k];
+        cnt[o]++;
+        if (f < F) {
+            const int32_t nb = d->h_neighbour[f];
+            for (int k = 0; k < 3; ++k) cs[3 * nb + k] += fc[k];
+            cnt[nb]++;
+        }
+    }
+    std::vector<std::pair<uint64_t, int32_t>> key(N);
+    const double scale = 2097152.0;  // 2^21
+    for (int64_t c = 0; c < N; ++c) {
+        uint64_t code = 0;
+        for (int k = 0; k < 3; ++k) {
+            const double ext = hi[k] > lo[k] ? hi[k] - lo[k] : 1.0;
+            double u = (cs[3 * c + k] / cnt[c] - lo[k]) / ext * scale;
+            uint64_t q = u <= 0.0 ? 0 : (u >= scale - 1.0 ? (uint64_t)scale - 1 : (uint64_t)u);
+            code |= spread3(q) << (2 - k);
+        }
+        key[c] = {code, (int32_t)c};
+    }
+    std::sort(key.begin(), key.end());
+    order.resize(N);
+    for (int64_t i = 0; i < N; ++i) order[i] = key[i].second;
+}
+
 }  // namespace
 
 extern "C" {
 
-// method 0: reverse Cuthill-McKee
+// method 0: reverse Cuthill-McKee; method 1: Z-order (Morton) of cell centres
 int fv_polymesh_renumber(const fv_polymesh_desc* d, int method, int32_t* h_face_offsets_out,
                          int32_t* h_face_points_out, int32_t* h_owner_out,
                          int32_t* h_neighbour_out, int32_t* h_cell_new_to_old,
@@ -81,31 +140,36 @@ int fv_polymesh_renumber(const fv_polymesh_desc* d, int method, int32_t* h_face_
         FV_REQUIRE(d && h_face_offsets_out && h_face_points_out && h_owner_out &&
                        h_neighbour_out && h_cell_new_to_old && h_face_new_to_old,
                    "NULL argument");
-        FV_REQUIRE(method == 0, "unknown renumbering method");
+        FV_REQUIRE(method == 0 || method == 1, "unknown renumbering method");
         const int64_t N = d->n_cells, nF = d->n_faces, F = d->n_internal_faces;
         for (int64_t f = 0; f < nF; ++f)
             FV_REQUIRE(d->h_owner[f] >= 0 && d->h_owner[f] < N, "owner out of range");
         for (int64_t f = 0; f < F; ++f)
             FV_REQUIRE(d->h_neighbour[f] >= 0 && d->h_neighbour[f] < N, "neighbour out of range");
-        Graph g = build_graph(N, F, d->h_owner, d->h_neighbour);
-        // order all connected components
         std::vector<int32_t> order;
-        order.reserve(N);
-        std::vector<int32_t> mark(N, -1);
-        int32_t stamp = 0;
-        for (int64_t seed = 0; seed < N; ++seed) {
-            if (mark[seed] >= 0) continue;
-            // pseudo-peripheral start: two BFS sweeps from the component seed
-            std::vector<int32_t> tmp;
-            int32_t far = bfs(g, (int32_t)seed, tmp, mark, stamp++, false);
-            for (int32_t c : tmp) mark[c] = -1;
-            tmp.clear();
-            far = bfs(g, far, tmp, mark, stamp++, false);
-            for (int32_t c : tmp) mark[c] = -1;
-            bfs(g, far, order, mark, stamp++, true);
+        if (method == 1) {
+            FV_REQUIRE(d->h_points && d->h_face_offsets && d->h_face_points, "NULL points/faces");
+            morton_order(d, order);
+        } else {
+            Graph g = build_graph(N, F, d->h_owner, d->h_neighbour);
+            // order all connected components
+            order.reserve(N);
+            std::vector<int32_t> mark(N, -1);
+            int32_t stamp = 0;
+            for (int64_t seed = 0; seed < N; ++seed) {
+                if (mark[seed] >= 0) continue;
+                // pseudo-peripheral start: two BFS sweeps from the component seed
+                std::vector<int32_t> tmp;
+                int32_t far = bfs(g, (int32_t)seed, tmp, mark, stamp++, false);
+                for (int32_t c : tmp) mark[c] = -1;
+                tmp.clear();
+                far = bfs(g, far, tmp, mark, stamp++, false);
+                for (int32_t c : tmp) mark[c] = -1;
+                bfs(g, far, order, mark, stamp++, true);
+            }
+            FV_REQUIRE((int64_t)order.size() == N, "renumbering lost cells");
+            std::reverse(order.begin(), order.end());
         }
-        FV_REQUIRE((int64_t)order.size() == N, "renumbering lost cells");
-        std::reverse(order.begin(), order.end());
         std::vector<int32_t> newOf(N);
         for (int64_t i = 0; i < N; ++i) {
             h_cell_new_to_old[i] = order[i];

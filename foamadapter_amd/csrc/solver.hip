@@ -24,8 +24,13 @@ namespace {
 constexpr int kBlock = 256;
 constexpr int kMaxGrid = 1024;
 
+// Device-resident CG scalars.  rz[it & 1] holds r_it.z_it, so the direction
+// update reads beta = rz[it&1] / rz[(it-1)&1] without a second writer; a
+// residual replacement overwrites rz[it&1] with the replaced residual's value.
+// done: 0 running, 1 stopped (converged estimate, iteration limit, NaN, or the
+// residual-drop trigger of the mixed solver), 2 converged in FP64 (mixed).
 struct CgState {
-    double rz, alpha, beta, bnorm, res, res0;
+    double rz[2], alpha, bnorm, res, res0, rrep, delta;
     int32_t it, done, maxit, pad;
     double tol;
     uint32_t ticket;
@@ -66,36 +71,63 @@ __device__ bool publish(const double (&v)[NV], double* partials, CgState* st, do
     __syncthreads();
     if (!last) return false;
     __threadfence();
-    // sum all partials in slot order (thread 0; at most kMaxGrid slots)
-    if (threadIdx.x == 0) {
-        for (int j = 0; j < NV; ++j) tot[j] = 0.0;
-        for (int b = 0; b < (int)gridDim.x; ++b)
-            for (int j = 0; j < NV; ++j) tot[j] += load_coherent(&partials[b * NV + j]);
-        st->ticket = 0;
+    // sum all partials: thread t adds slots t, t+blockDim, ... in order, then the
+    // fixed-shape block tree; the grouping depends only on the grid size
+    for (int j = 0; j < NV; ++j) {
+        double a = 0.0;
+        for (int b = threadIdx.x; b < (int)gridDim.x; b += blockDim.x)
+            a += load_coherent(&partials[b * NV + j]);
+        a = block_sum(a, sh);
+        if (threadIdx.x == 0) tot[j] = a;
     }
+    if (threadIdx.x == 0) st->ticket = 0;
     return true;
 }
 
-// q = A p over rows [row0 ...) grid-stride, one thread per row
+// q = A p with A in sliced ELL (one 64-row slice per wavefront, entry k of all
+// 64 rows contiguous, so every value/column load is one coalesced 256/512-byte
+// access); rows summed in column order, as the CSR product
+struct Sell {
+    const int64_t* off;
+    const int32_t* col;
+};
+
 template <typename S>
-__global__ __launch_bounds__(kBlock) void k_spmv_pq(int N, const int32_t* __restrict__ rowOffs,
-                                                    const int32_t* __restrict__ col,
-                                                    const S* __restrict__ A,
+__global__ __launch_bounds__(kBlock) void k_spmv_pq(int N, Sell A, const S* __restrict__ val,
                                                     const S* __restrict__ p, S* __restrict__ q,
                                                     double* __restrict__ partials,
                                                     CgState* __restrict__ st) {
     __shared__ double sh[kBlock / 64];
     if (st->done) return;
     double pq = 0.0;
-    for (int r = blockIdx.x * blockDim.x + threadIdx.x; r < N; r += gridDim.x * blockDim.x) {
+    const int npad = (N + 63) & ~63;
+    for (int r = blockIdx.x * blockDim.x + threadIdx.x; r < npad; r += gridDim.x * blockDim.x) {
+        const int sl = r >> 6, l = r & 63;
+        const int64_t b = A.off[sl];
+        const int w = (int)((A.off[sl + 1] - b) >> 6);
         S s = 0;
-        const int e = rowOffs[r + 1];
-        for (int k = rowOffs[r]; k < e; ++k) s += A[k] * p[col[k]];
-        q[r] = s;
-        pq += (double)p[r] * (double)s;
+        for (int k = 0; k < w; ++k) {
+            const int64_t pos = b + 64 * (int64_t)k + l;
+            s += val[pos] * p[A.col[pos]];
+        }
+        if (r < N) {
+            q[r] = s;
+            pq += (double)p[r] * (double)s;
+        }
     }
     double v[1] = {pq}, tot[1];
-    if (publish<1>(v, partials, st, sh, tot) && threadIdx.x == 0) st->alpha = st->rz / tot[0];
+    if (publish<1>(v, partials, st, sh, tot) && threadIdx.x == 0)
+        st->alpha = st->rz[st->it & 1] / tot[0];
+}
+
+template <typename S>
+__global__ void k_sell_values(int64_t n, const int32_t* __restrict__ src,
+                              const double* __restrict__ values, S* __restrict__ out) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const int32_t j = src[i];
+        out[i] = j >= 0 ? (S)values[j] : (S)0;
+    }
 }
 
 template <typename S>
@@ -120,14 +152,14 @@ __global__ __launch_bounds__(kBlock) void k_update(int N, S* __restrict__ x, S* 
     }
     double v[2] = {rr, rz}, tot[2];
     if (publish<2>(v, partials, st, sh, tot) && threadIdx.x == 0) {
-        st->it += 1;
-        st->res = sqrt(tot[0]) / st->bnorm;
-        if (st->res <= st->tol || st->it >= st->maxit || !(st->res == st->res)) {
+        const int it = st->it + 1;
+        st->it = it;
+        st->rz[it & 1] = tot[1];
+        const double rn = sqrt(tot[0]);
+        st->res = rn / st->bnorm;
+        if (st->res <= st->tol || it >= st->maxit || !(st->res == st->res) ||
+            (st->delta > 0.0 && rn < st->delta * st->rrep))
             st->done = 1;
-        } else {
-            st->beta = tot[1] / st->rz;
-            st->rz = tot[1];
-        }
     }
 }
 
@@ -136,9 +168,14 @@ __global__ __launch_bounds__(kBlock) void k_direction(int N, S* __restrict__ p,
                                                       const S* __restrict__ z,
                                                       const CgState* __restrict__ st) {
     if (st->done) return;
-    const S b = (S)st->beta;
+    const int it = st->it;
+    const S b = (S)(st->rz[it & 1] / st->rz[(it - 1) & 1]);
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += gridDim.x * blockDim.x)
         p[i] = z[i] + b * p[i];
+}
+
+__global__ void k_set_delta(CgState* st, double delta) {
+    if (threadIdx.x == 0) st->delta = delta;
 }
 
 // r = b - A x ; z = D^-1 r ; p = z ; partials of b.b, r.r, r.z -> initial state
@@ -174,7 +211,7 @@ __global__ __launch_bounds__(kBlock) void k_init(int N, const int32_t* __restric
         st->bnorm = bn > 0.0 ? bn : 1.0;
         st->res0 = sqrt(tot[1]) / st->bnorm;
         st->res = st->res0;
-        st->rz = tot[2];
+        st->rz[0] = tot[2];
         st->it = 0;
         st->done = (st->res <= st->tol || st->maxit <= 0) ? 1 : 0;
     }
@@ -196,12 +233,6 @@ __global__ void k_cast(int64_t n, const S* __restrict__ s, D* __restrict__ d) {
         d[i] = (D)s[i];
 }
 
-__global__ void k_axpy_f32(int64_t n, const float* __restrict__ d, double* __restrict__ x) {
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-         i += (int64_t)gridDim.x * blockDim.x)
-        x[i] += (double)d[i];
-}
-
 __global__ void k_zero_state(CgState* st, double tol, int maxit) {
     if (threadIdx.x == 0) {
         CgState s{};
@@ -216,44 +247,55 @@ int grid_of(int N) { return grid_for(N, kBlock, kMaxGrid); }
 // Workspace for one CG in value type S.
 template <typename S>
 struct CgWork {
-    DBuf<S> r, z, p, q, dinv;
+    DBuf<S> r, z, p, q, dinv, sval;
     DBuf<double> partials;
     DBuf<CgState> st;
-    void alloc(fv_ctx* c, int N) {
+    void alloc(fv_ctx* c, const fv_sparsity* sp) {
+        const int N = sp->nRows;
         r.alloc(c, N);
         z.alloc(c, N);
         p.alloc(c, N);
         q.alloc(c, N);
         dinv.alloc(c, N);
+        sval.alloc(c, sp->sellTotal);
         partials.alloc(c, 3 * kMaxGrid);
         st.alloc(c, 1);
     }
+    // matrix values (and Jacobi diagonal) of the assembled FP64 system in S
+    void load_matrix(fv_ctx* c, const fv_sparsity* sp, const double* values) {
+        hipStream_t s = c->stream;
+        k_sell_values<S><<<grid_for(sp->sellTotal, kBlock, 8192), kBlock, 0, s>>>(
+            sp->sellTotal, sp->sellSrc.p, values, sval.p);
+        k_dinv<S><<<grid_for(sp->nRows, kBlock), kBlock, 0, s>>>(sp->nRows, sp->rowOffs.p,
+                                                                 sp->diagOffset.p, values, dinv.p);
+        FV_LAUNCH_CHECK();
+    }
 };
 
-struct HostState {
-    CgState s;
-};
-
-// Runs CG on A (values in S) for x (S); returns the final device state.
+// Runs CG for x (S) on the matrix loaded into w; csrA (values in S, CSR) is
+// only read for the initial residual when x is not zero.
 template <typename S>
-CgState run_cg(fv_ctx* ctx, const fv_sparsity* sp, const S* A, const S* b, S* x, bool x_zero,
+CgState run_cg(fv_ctx* ctx, const fv_sparsity* sp, const S* csrA, const S* b, S* x, bool x_zero,
                CgWork<S>& w, double tol, int maxit, int check) {
     const int N = sp->nRows, G = grid_of(N);
     hipStream_t s = ctx->stream;
+    const Sell sell{sp->sellOff.p, sp->sellCol.p};
     k_zero_state<<<1, 64, 0, s>>>(w.st.p, tol, maxit);
-    k_init<S><<<G, kBlock, 0, s>>>(N, sp->rowOffs.p, sp->colIdxs.p, A, b, x, w.dinv.p, w.r.p,
+    k_init<S><<<G, kBlock, 0, s>>>(N, sp->rowOffs.p, sp->colIdxs.p, csrA, b, x, w.dinv.p, w.r.p,
                                    w.z.p, w.p.p, w.partials.p, w.st.p, x_zero ? 1 : 0);
     FV_LAUNCH_CHECK();
     CgState h{};
-    int launched = 0;
+    int launched = 0, batch = std::min(4, check);
     while (true) {
         FV_HIP(hipMemcpyAsync(&h, w.st.p, sizeof(CgState), hipMemcpyDeviceToHost, s));
         FV_HIP(hipStreamSynchronize(s));
         if (h.done || launched >= maxit) break;
-        const int n = std::min(check, maxit - launched);
+        // poll early for short solves, then every `check` iterations
+        const int n = std::min(batch, maxit - launched);
+        batch = std::min(2 * batch, check);
         for (int i = 0; i < n; ++i) {
-            k_spmv_pq<S><<<G, kBlock, 0, s>>>(N, sp->rowOffs.p, sp->colIdxs.p, A, w.p.p, w.q.p,
-                                              w.partials.p, w.st.p);
+            k_spmv_pq<S><<<G, kBlock, 0, s>>>(N, sell, w.sval.p, w.p.p, w.q.p, w.partials.p,
+                                              w.st.p);
             k_update<S><<<G, kBlock, 0, s>>>(N, x, w.r.p, w.z.p, w.p.p, w.q.p, w.dinv.p,
                                              w.partials.p, w.st.p);
             k_direction<S><<<G, kBlock, 0, s>>>(N, w.p.p, w.z.p, w.st.p);
@@ -304,6 +346,123 @@ double true_residual(fv_ctx* ctx, const fv_linsys* ls, const double* x, double* 
     return h.res;
 }
 
+// ---- mixed precision: FP32 CG with FP64 residual replacement ---------------
+//
+// The FP32 CG accumulates its correction d (FP32) for a chunk of iterations; a
+// chunk ends after `chunk` iterations, or earlier when the FP32 residual has
+// dropped by `delta` since the last replacement or its estimate reaches the
+// tolerance.  Then, in FP64: x += d, r = b - A x, ||r||/||b|| tested against the
+// tolerance; the FP32 residual is replaced by the true one and CG continues
+// with its search direction (beta from the replaced residual), so the Krylov
+// space is not restarted.
+
+__global__ void k_ir_accum(int N, float* __restrict__ d, double* __restrict__ x) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += gridDim.x * blockDim.x) {
+        x[i] += (double)d[i];
+        d[i] = 0.0f;
+    }
+}
+
+// r32 = r64; z = D^-1 r32; (start: p = z); partials of r.r and r.z
+__global__ __launch_bounds__(kBlock) void k_ir_replace(int N, const double* __restrict__ r64,
+                                                       const float* __restrict__ dinv,
+                                                       float* __restrict__ r, float* __restrict__ z,
+                                                       float* __restrict__ p,
+                                                       double* __restrict__ partials,
+                                                       CgState* __restrict__ st,
+                                                       const CgState* __restrict__ st64, int start) {
+    __shared__ double sh[kBlock / 64];
+    double rr = 0.0, rz = 0.0;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += gridDim.x * blockDim.x) {
+        const float ri = (float)r64[i];
+        const float zi = dinv[i] * ri;
+        r[i] = ri;
+        z[i] = zi;
+        if (start) p[i] = zi;
+        rr += (double)ri * (double)ri;
+        rz += (double)ri * (double)zi;
+    }
+    double v[2] = {rr, rz}, tot[2];
+    if (publish<2>(v, partials, st, sh, tot) && threadIdx.x == 0) {
+        st->bnorm = st64->bnorm;
+        st->rrep = sqrt(tot[0]);
+        if (start) st->res0 = st64->res;
+        // the replaced residual takes the place of r_it (see CgState)
+        st->rz[st->it & 1] = tot[1];
+        st->res = st64->res;
+        // 2: converged in FP64; 0: continue (direction update follows)
+        st->done = st64->res <= st->tol ? 2 : 0;
+    }
+}
+
+struct IrOut {
+    double init_res, res;
+    int iters, outer;
+};
+
+IrOut run_mixed(fv_ctx* ctx, const fv_linsys* ls, double* x, double tol, int maxit, int chunk,
+                double delta, int max_outer) {
+    const fv_sparsity* sp = ls->sp;
+    const int N = sp->nRows, G = grid_of(N);
+    hipStream_t s = ctx->stream;
+    CgWork<float> w;
+    w.alloc(ctx, sp);
+    w.load_matrix(ctx, sp, ls->values.p);
+    DBuf<float> d32;
+    d32.alloc(ctx, N);
+    FV_HIP(hipMemsetAsync(d32.p, 0, N * sizeof(float), s));
+    DBuf<double> r64, partials64;
+    DBuf<CgState> st64;
+    r64.alloc(ctx, N);
+    partials64.alloc(ctx, 3 * kMaxGrid);
+    st64.alloc(ctx, 1);
+    const Sell sell{sp->sellOff.p, sp->sellCol.p};
+    // FP64 residual of the initial guess, then FP32 CG state from it
+    k_zero_state<<<1, 64, 0, s>>>(w.st.p, tol, maxit);
+    k_zero_state<<<1, 64, 0, s>>>(st64.p, 0.0, 0);
+    k_residual_norms<<<G, kBlock, 0, s>>>(N, sp->rowOffs.p, sp->colIdxs.p, ls->values.p,
+                                          ls->rhs.p, x, r64.p, partials64.p, st64.p);
+    k_ir_replace<<<G, kBlock, 0, s>>>(N, r64.p, w.dinv.p, w.r.p, w.z.p, w.p.p, w.partials.p,
+                                      w.st.p, st64.p, 1);
+    k_set_delta<<<1, 64, 0, s>>>(w.st.p, delta);
+    FV_LAUNCH_CHECK();
+    IrOut out{0, 0, 0, 0};
+    CgState h{};
+    FV_HIP(hipMemcpyAsync(&h, w.st.p, sizeof(CgState), hipMemcpyDeviceToHost, s));
+    FV_HIP(hipStreamSynchronize(s));
+    out.init_res = h.res0;
+    out.res = h.res;
+    int batch = std::min(8, chunk);
+    while (h.done != 2 && h.it < maxit && out.outer < max_outer) {
+        // short chunks first (short solves end early), then `chunk` iterations
+        const int n = std::min(batch, maxit - h.it);
+        batch = std::min(2 * batch, chunk);
+        // the chunk ends after an update: its direction update uses the replaced residual
+        for (int i = 0; i < n; ++i) {
+            k_spmv_pq<float><<<G, kBlock, 0, s>>>(N, sell, w.sval.p, w.p.p, w.q.p, w.partials.p,
+                                                  w.st.p);
+            k_update<float><<<G, kBlock, 0, s>>>(N, d32.p, w.r.p, w.z.p, w.p.p, w.q.p, w.dinv.p,
+                                                 w.partials.p, w.st.p);
+            if (i + 1 < n) k_direction<float><<<G, kBlock, 0, s>>>(N, w.p.p, w.z.p, w.st.p);
+        }
+        // FP64 residual replacement and convergence test
+        k_ir_accum<<<G, kBlock, 0, s>>>(N, d32.p, x);
+        k_zero_state<<<1, 64, 0, s>>>(st64.p, 0.0, 0);
+        k_residual_norms<<<G, kBlock, 0, s>>>(N, sp->rowOffs.p, sp->colIdxs.p, ls->values.p,
+                                              ls->rhs.p, x, r64.p, partials64.p, st64.p);
+        k_ir_replace<<<G, kBlock, 0, s>>>(N, r64.p, w.dinv.p, w.r.p, w.z.p, w.p.p, w.partials.p,
+                                          w.st.p, st64.p, 0);
+        k_direction<float><<<G, kBlock, 0, s>>>(N, w.p.p, w.z.p, w.st.p);
+        FV_LAUNCH_CHECK();
+        FV_HIP(hipMemcpyAsync(&h, w.st.p, sizeof(CgState), hipMemcpyDeviceToHost, s));
+        FV_HIP(hipStreamSynchronize(s));
+        ++out.outer;
+        out.res = h.res;
+    }
+    out.iters = h.it;
+    return out;
+}
+
 }  // namespace
 
 extern "C" {
@@ -315,15 +474,12 @@ int fv_solve_cg(fv_linsys* ls, void* d_x, const fv_solve_opts* o, fv_solver_stat
         FV_REQUIRE(o->rel_tol >= 0.0 && o->max_iter >= 0, "bad solver options");
         fv_sparsity* sp = ls->sp;
         fv_ctx* ctx = sp->mesh->ctx;
-        const int N = sp->nRows;
-        hipStream_t s = ctx->stream;
         const int check = o->check_every > 0 ? o->check_every : 8;
         double* x = (double*)d_x;
         if (o->mode == FV_SOLVE_F64) {
             CgWork<double> w;
-            w.alloc(ctx, N);
-            k_dinv<double><<<grid_for(N, kBlock), kBlock, 0, s>>>(N, sp->rowOffs.p, sp->diagOffset.p,
-                                                                  ls->values.p, w.dinv.p);
+            w.alloc(ctx, sp);
+            w.load_matrix(ctx, sp, ls->values.p);
             CgState h = run_cg<double>(ctx, sp, ls->values.p, ls->rhs.p, x, false, w, o->rel_tol,
                                        o->max_iter, check);
             local.init_res = h.res0;
@@ -332,41 +488,15 @@ int fv_solve_cg(fv_linsys* ls, void* d_x, const fv_solve_opts* o, fv_solver_stat
             local.outer_iters = 1;
             local.true_res = true_residual(ctx, ls, x, w.r.p, w.partials, w.st);
         } else if (o->mode == FV_SOLVE_F32_IR) {
-            // FP32 Jacobi-CG for the correction inside FP64 iterative refinement
-            DBuf<float> A32, r32, d32;
-            A32.alloc(ctx, sp->nnz);
-            r32.alloc(ctx, N);
-            d32.alloc(ctx, N);
-            CgWork<float> w;
-            w.alloc(ctx, N);
-            DBuf<double> r64, partials64;
-            DBuf<CgState> st64;
-            r64.alloc(ctx, N);
-            partials64.alloc(ctx, 3 * kMaxGrid);
-            st64.alloc(ctx, 1);
-            k_cast<float, double><<<grid_for(sp->nnz, kBlock, 8192), kBlock, 0, s>>>(sp->nnz, ls->values.p, A32.p);
-            k_dinv<float><<<grid_for(N, kBlock), kBlock, 0, s>>>(N, sp->rowOffs.p, sp->diagOffset.p,
-                                                                 ls->values.p, w.dinv.p);
-            double res = true_residual(ctx, ls, x, r64.p, partials64, st64);
-            local.init_res = res;
-            const double inner_tol = o->inner_rel_tol > 0.0 ? o->inner_rel_tol : 1e-4;
-            const int max_outer = o->max_outer > 0 ? o->max_outer : 50;
-            int outer = 0, inner_total = 0;
-            while (res > o->rel_tol && outer < max_outer && inner_total < o->max_iter) {
-                k_cast<float, double><<<grid_for(N, kBlock, 4096), kBlock, 0, s>>>(N, r64.p, r32.p);
-                FV_HIP(hipMemsetAsync(d32.p, 0, N * sizeof(float), s));
-                CgState h = run_cg<float>(ctx, sp, A32.p, r32.p, d32.p, true, w, inner_tol,
-                                          o->max_iter - inner_total, check);
-                inner_total += h.it;
-                k_axpy_f32<<<grid_for(N, kBlock, 4096), kBlock, 0, s>>>(N, d32.p, x);
-                res = true_residual(ctx, ls, x, r64.p, partials64, st64);
-                ++outer;
-                if (h.it == 0) break;  // no progress possible in FP32
-            }
-            local.final_res = res;
-            local.true_res = res;
-            local.iters = inner_total;
-            local.outer_iters = outer;
+            IrOut r = run_mixed(ctx, ls, x, o->rel_tol, o->max_iter,
+                                o->check_every > 0 ? o->check_every : 64,
+                                o->inner_rel_tol > 0.0 ? o->inner_rel_tol : 1e-3,
+                                o->max_outer > 0 ? o->max_outer : (1 << 30));
+            local.init_res = r.init_res;
+            local.final_res = r.res;
+            local.true_res = r.res;
+            local.iters = r.iters;
+            local.outer_iters = r.outer;
         } else {
             FV_REQUIRE(false, "unknown solver mode");
         }

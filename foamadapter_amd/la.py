@@ -145,9 +145,12 @@ def solver_opts(solver: dict | None) -> _lib.FvSolveOpts:
     if mode not in ("double", "mixed"):
         raise ValueError(f"unknown precision {mode!r}")
     o.mode = _lib.FV_SOLVE_F32_IR if mode == "mixed" else _lib.FV_SOLVE_F64
-    o.inner_rel_tol = float(d.get("innerRelTol", 1e-4))
-    o.max_outer = int(d.get("maxOuter", 50))
-    o.check_every = int(d.get("checkEvery", 8))
+    # mixed precision: an FP64 residual replacement ends each FP32 chunk of at
+    # most checkEvery iterations, or earlier once the FP32 residual fell by
+    # innerRelTol; FP64: the host polls the stop flag every checkEvery iterations
+    o.inner_rel_tol = float(d.get("innerRelTol", 1e-3))
+    o.max_outer = int(d.get("maxOuter", 0))
+    o.check_every = int(d.get("checkEvery", 32))
     solver_name = d.get("solver", "PCG")
     if solver_name not in ("PCG", "CG"):
         raise ValueError(f"solver {solver_name!r} is not supported (PCG only)")

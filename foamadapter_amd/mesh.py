@@ -259,16 +259,21 @@ class Mesh:
     cell_map = None   # device int32 [N]: device cell i holds caller cell cell_map[i]
     face_map = None   # device int32 [F+B]: device face j holds caller face (~ if flipped)
 
+    RENUMBER = {"rcm": 0, "morton": 1}
+
     @classmethod
-    def from_poly(cls, ctx: Context, poly: PolyMesh, renumber: bool = False) -> "Mesh":
+    def from_poly(cls, ctx: Context, poly: PolyMesh, renumber: bool | str = False) -> "Mesh":
         """Device mesh from polyMesh primitives (geometry computed on the GPU).
 
-        renumber=True stores the mesh in reverse Cuthill-McKee order (what
-        OpenFOAM's renumberMesh does to a case); fields then cross the
-        boundary through to_device()/to_caller(), which apply the permutation."""
+        renumber=True (Z-order, 'morton') or 'rcm' (reverse Cuthill-McKee) stores
+        the mesh renumbered, as OpenFOAM's renumberMesh does to a case; fields
+        then cross the boundary through to_device()/to_caller(), which apply the
+        permutation."""
         cmap = fmap = None
         if renumber:
-            poly, cmap, fmap = poly.renumbered()
+            method = self_method = cls.RENUMBER["morton" if renumber is True else renumber]
+            poly, cmap, fmap = poly.renumbered(self_method)
+            del method
         d, keep = poly.desc()
         h = C.c_void_p()
         call("fv_mesh_create_poly", ctx.handle, C.byref(d), C.byref(h))
@@ -291,6 +296,16 @@ class Mesh:
         call("fv_permute", self.ctx.handle, prec_of(src.dtype), src.shape[0], ncomp, mode,
              1 if oriented else 0, mp.ptr, src.ptr, dst.ptr)
         return dst
+
+    def set_gather(self, mode: int):
+        """0: LDS-staged face blocks when possible (default); 1: per-cell stencil walk."""
+        call("fv_mesh_set_gather", self.handle, int(mode))
+
+    @property
+    def gather_kernel(self) -> str:
+        m = C.c_int()
+        call("fv_mesh_get_gather", self.handle, C.byref(m))
+        return {1: "stencil-walk", 2: "staged"}[m.value]
 
     def to_device(self, src, kind: str = "cell", oriented: bool = False):
         """Caller-numbered device array -> this mesh's numbering (identity if not renumbered).

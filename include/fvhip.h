@@ -158,8 +158,10 @@ int fv_hexmesh_generate(int32_t nx, int32_t ny, int32_t nz, double lx, double ly
                         int32_t* h_patch_size);
 
 /* Locality renumbering of a polyMesh (host), the counterpart of OpenFOAM's
- * renumberMesh (reverse Cuthill-McKee, method 0) that a case runs before the
- * solver: cells reordered by a breadth-first sweep of the face graph, internal
+ * renumberMesh that a case runs before the solver: method 0 reverse
+ * Cuthill-McKee (a breadth-first sweep of the face graph), method 1 Z-order
+ * (Morton code of the cell centres: aligned runs of cells are compact bricks,
+ * which the face kernels exploit); internal
  * faces rewritten in upper-triangular order of the new numbering (a face whose
  * owner/neighbour swap has its point list reversed), boundary faces unchanged.
  * Outputs are sized like the input (points and patches are unchanged):
@@ -201,6 +203,14 @@ typedef enum {
     FV_G_CELL_FACE_OFFSETS, FV_G_CELL_FACES
 } fv_geom;
 int fv_mesh_get(const fv_mesh* mesh, fv_geom which, void* h_out);
+
+/* Kernel choice for the cell-result operators (grad, div, laplacian): mode 0
+ * picks the LDS-staged face-block gather when the faces are in upper-triangular
+ * order, mode 1 forces the per-cell stencil walk (same arithmetic and order,
+ * bit-identical results).  fv_mesh_get_gather reports the kernel in use:
+ * 2 staged, 1 stencil walk.  No counterpart in the reference: a tuning knob. */
+int fv_mesh_set_gather(fv_mesh* mesh, int mode);
+int fv_mesh_get_gather(const fv_mesh* mesh, int* mode);
 
 /* Boundary conditions of a field: one kind per patch
  * (fvcc::VolumeBoundary list built by readVolBoundaryConditions, readers.hpp:31-108). */
