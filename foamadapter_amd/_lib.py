@@ -138,6 +138,13 @@ _SIGS = {
     "fv_set_reference": [_vp, C.c_int, _d],
     "fv_residual": [_vp, _vp, _vp],
     "fv_solve_cg": [_vp, _vp, C.POINTER(FvSolveOpts), C.POINTER(FvSolverStats)],
+    "fv_batch_create": [_vp, C.c_int, C.c_int, C.c_int, _pp],
+    "fv_batch_destroy": [_vp],
+    "fv_batch_add": [_vp, _vp, _vp, _vp, C.POINTER(C.c_int)],
+    "fv_batch_clear": [_vp],
+    "fv_batch_size": [_vp, C.POINTER(C.c_int)],
+    "fv_batch_set_shape": [_vp, C.c_int, C.c_int],
+    "fv_batch_solve": [_vp, C.POINTER(FvSolveOpts), C.c_int, C.POINTER(FvSolverStats)],
 }
 _RESTYPES = {
     "fv_last_error": C.c_char_p,

@@ -13,7 +13,7 @@
 // slot order.  All scalars live on the device, kernels turn into no-ops once
 // the stop flag is set, and the host polls the flag every `check` iterations,
 // so a solve needs no host round trip per iteration.
-#include "implicit.hpp"
+#include "cgstate.hpp"
 
 #include <cmath>
 
@@ -21,68 +21,8 @@ using namespace fvhip;
 
 namespace {
 
-constexpr int kBlock = 256;
+constexpr int kBlock = kCgBlock;
 constexpr int kMaxGrid = 1024;
-
-// Device-resident CG scalars.  rz[it & 1] holds r_it.z_it, so the direction
-// update reads beta = rz[it&1] / rz[(it-1)&1] without a second writer; a
-// residual replacement overwrites rz[it&1] with the replaced residual's value.
-// done: 0 running, 1 stopped (converged estimate, iteration limit, NaN, or the
-// residual-drop trigger of the mixed solver), 2 converged in FP64 (mixed).
-struct CgState {
-    double rz[2], alpha, bnorm, res, res0, rrep, delta;
-    int32_t it, done, maxit, pad;
-    double tol;
-    uint32_t ticket;
-    uint32_t pad2;
-};
-
-__device__ inline double block_sum(double v, double* sh) {
-    // fixed-shape tree: wavefront shuffle then across the block's wavefronts
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
-    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-    if (lane == 0) sh[wid] = v;
-    __syncthreads();
-    double r = 0.0;
-    if (threadIdx.x == 0) {
-        for (int w = 0; w < (int)(blockDim.x >> 6); ++w) r += sh[w];
-    }
-    __syncthreads();
-    return r;  // valid in thread 0
-}
-
-__device__ inline double load_coherent(const double* p) {
-    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-// Write this block's NV partials, then return true in the block that finished last.
-template <int NV>
-__device__ bool publish(const double (&v)[NV], double* partials, CgState* st, double* sh,
-                        double (&tot)[NV]) {
-    __shared__ bool last;
-    double s[NV];
-    for (int j = 0; j < NV; ++j) s[j] = block_sum(v[j], sh);
-    if (threadIdx.x == 0) {
-        for (int j = 0; j < NV; ++j) partials[blockIdx.x * NV + j] = s[j];
-        __threadfence();
-        const uint32_t t = atomicAdd(&st->ticket, 1u);
-        last = (t == gridDim.x - 1);
-    }
-    __syncthreads();
-    if (!last) return false;
-    __threadfence();
-    // sum all partials: thread t adds slots t, t+blockDim, ... in order, then the
-    // fixed-shape block tree; the grouping depends only on the grid size
-    for (int j = 0; j < NV; ++j) {
-        double a = 0.0;
-        for (int b = threadIdx.x; b < (int)gridDim.x; b += blockDim.x)
-            a += load_coherent(&partials[b * NV + j]);
-        a = block_sum(a, sh);
-        if (threadIdx.x == 0) tot[j] = a;
-    }
-    if (threadIdx.x == 0) st->ticket = 0;
-    return true;
-}
 
 // q = A p with A in sliced ELL (one 64-row slice per wavefront, entry k of all
 // 64 rows contiguous, so every value/column load is one coalesced 256/512-byte

@@ -160,6 +160,55 @@ def solver_opts(solver: dict | None) -> _lib.FvSolveOpts:
     return o
 
 
+class Batch:
+    """Many independent systems solved in one launch sequence (fv_batch_*)."""
+
+    def __init__(self, ctx, precision: str = "mixed", tile_rows: int = 1024, chunk: int = 16):
+        mode = _lib.FV_SOLVE_F32_IR if precision == "mixed" else _lib.FV_SOLVE_F64
+        h = C.c_void_p()
+        call("fv_batch_create", ctx.handle, mode, int(tile_rows), int(chunk), C.byref(h))
+        self.handle = h
+        self.ctx = ctx
+        self.members = []
+        ctx.adopt(self)
+
+    def add(self, ls: LinearSystem, x: DeviceArray, xold: DeviceArray | None = None) -> int:
+        i = C.c_int()
+        call("fv_batch_add", self.handle, ls.handle, x.ptr, xold.ptr if xold is not None else None,
+             C.byref(i))
+        self.members.append((ls, x, xold))
+        return i.value
+
+    def clear(self):
+        call("fv_batch_clear", self.handle)
+        self.members = []
+
+    def set_shape(self, tile_rows: int, chunk: int):
+        call("fv_batch_set_shape", self.handle, int(tile_rows), int(chunk))
+
+    def solve(self, solver: dict | None = None, use_graph: bool = True,
+              raise_on_noconverge: bool = True) -> list[SolverStats]:
+        o = solver_opts(solver)
+        n = len(self.members)
+        st = (_lib.FvSolverStats * max(n, 1))()
+        rc = lib.fv_batch_solve(self.handle, C.byref(o), 1 if use_graph else 0, st)
+        if rc not in (_lib.FV_OK, _lib.FV_ENOCONVERGE) or (rc != _lib.FV_OK and raise_on_noconverge):
+            raise _lib.FvError(rc, _lib.last_error())
+        return [SolverStats(s.init_res, s.final_res, s.iters, s.outer_iters, s.true_res, rc)
+                for s in st[:n]]
+
+    def close(self):
+        if self.handle:
+            lib.fv_batch_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 def solve_cg(ls: LinearSystem, x: DeviceArray, solver: dict | None = None,
              raise_on_noconverge: bool = True) -> SolverStats:
     o = solver_opts(solver)

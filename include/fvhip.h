@@ -333,6 +333,30 @@ typedef struct {
  * the tolerance is not reached. */
 int fv_solve_cg(fv_linsys* ls, void* d_x, const fv_solve_opts* opts, fv_solver_stats* stats);
 
+/* ---- batched solves (many independent systems in one launch sequence) ----
+ * No counterpart in the reference (each NeoN solve is a separate Ginkgo call);
+ * this is the batching layer the served and batched workloads run on.  A batch
+ * holds assembled systems of any sizes; fv_batch_solve runs Jacobi-CG on all of
+ * them together (row tiles of tile_rows rows over every system, per-system
+ * deterministic reductions, FP64 residual replacement every `chunk`
+ * iterations, a chunk captured once as a hipGraph when use_graph != 0).  A
+ * system's result does not depend on the other members of the batch.
+ * mode: FV_SOLVE_F64 or FV_SOLVE_F32_IR (FP32 CG, FP64 residual and test). */
+typedef struct fv_batch fv_batch;
+int fv_batch_create(fv_ctx* ctx, int mode, int tile_rows, int chunk, fv_batch** out);
+int fv_batch_destroy(fv_batch* b);
+/* d_x: FP64 initial guess and result [N]; d_xold: optional [N] that receives x
+ * after the solve (Euler T_old <- T), or NULL.  The system must stay alive
+ * while it is in the batch. */
+int fv_batch_add(fv_batch* b, fv_linsys* ls, void* d_x, void* d_xold, int* id);
+int fv_batch_clear(fv_batch* b);
+int fv_batch_size(const fv_batch* b, int* n);
+int fv_batch_set_shape(fv_batch* b, int tile_rows, int chunk);
+/* stats: array of fv_batch_size entries (in insertion order) or NULL.
+ * opts->inner_rel_tol: FP32 residual drop that ends a chunk early (mixed). */
+int fv_batch_solve(fv_batch* b, const fv_solve_opts* opts, int use_graph,
+                   fv_solver_stats* stats);
+
 #ifdef __cplusplus
 }
 #endif
