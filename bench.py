@@ -278,6 +278,217 @@ def bench_poisson(ctx, n=128, seed=1003, repeats=3):
     return {"mesh": f"{n}^3 jittered, permuted, renumbered", "cells": N, **out}
 
 
+def _loguniform(u, lo, hi):
+    return np.exp(np.log(lo) + u * (np.log(hi) - np.log(lo)))
+
+
+def bench_batched(ctx, P=512, seed=1004, steps=5):
+    """512 independent heat problems on cubic meshes n ~ logU[12, 96], kappa ~
+    logU[1e-4, 1e-1], dt = 1e-3, T0 ~ U[300,400], walls ~ U[300,400], 5 Euler
+    steps, to 1e-8.  Plain: one problem at a time, FP64 fv_solve_cg per step.
+    New: all problems in one batched launch sequence per step (FP32 CG with FP64
+    residual replacement, hipGraph chunks, autotuned tile shape)."""
+    import foamadapter_amd as fa
+    from foamadapter_amd import _lib, la
+    from foamadapter_amd.la import solver_opts
+    import ctypes as C
+
+    sys.path.insert(0, str(ROOT / "tests"))
+    u = lambda n, s: _uniform_host(n, 0.0, 1.0, seed, s)  # noqa: E731
+    ns = np.rint(_loguniform(u(P, 0), 12, 96)).astype(int)
+    kappas = _loguniform(u(P, 1), 1e-4, 1e-1)
+    t0 = time.time()
+    meshes, sps = {}, {}
+    for n in sorted(set(ns)):
+        m = fa.Mesh.from_poly(ctx, fa.PolyMesh.hex(int(n)))
+        meshes[n] = m
+        sps[n] = la.SparsityPattern(m)
+    log(f"batched: {len(meshes)} distinct meshes built in {time.time() - t0:.1f}s")
+    probs = []
+    for i in range(P):
+        m = meshes[ns[i]]
+        N, F, B = m.n_cells, m.n_internal_faces, m.n_boundary_faces
+        walls = 300.0 + 100.0 * _uniform_host(6, 0.0, 1.0, seed + 1 + i, 9)
+        po = m.get("patchOffsets") if i == 0 or ns[i] != ns[i - 1] else po
+        Tb = np.concatenate([np.full(po[k + 1] - po[k], walls[k]) for k in range(6)])
+        bc = getattr(m, "_bc_fv", None) or fa.BoundaryConditions(m, ["fixedValue"] * 6)
+        m._bc_fv = bc
+        T0 = ctx.uniform(N, 300.0, 400.0, seed + 1 + i, 0)
+        probs.append({"m": m, "bc": bc, "Tb": ctx.upload(Tb), "T0": T0,
+                      "kap": ctx.upload(np.full(F + B, kappas[i])),
+                      "ls": la.LinearSystem(sps[ns[i]]), "N": N})
+    ctx.sync()
+    total_cells = int(sum(p["N"] for p in probs))
+    log(f"batched: {P} problems, {total_cells} cells, set up in {time.time() - t0:.1f}s")
+    solver = {"relTol": 1e-8, "maxIter": 5000}
+
+    def run_plain():
+        o = solver_opts(solver)
+        st = _lib.FvSolverStats()
+        iters = 0
+        for p in probs:
+            x = p["x"]
+            _lib.call("fv_memcpy_d2d", ctx.handle, x.ptr, p["T0"].ptr, x.nbytes)
+            for s in range(steps):
+                _lib.call("fv_memcpy_d2d", ctx.handle, p["xo"].ptr, x.ptr, x.nbytes)
+                _lib.call("fv_assemble_ddt_laplacian", p["ls"].handle, p["xo"].ptr, 1e-3,
+                          p["kap"].ptr, p["bc"].handle, p["Tb"].ptr, None, -1.0)
+                _lib.call("fv_solve_cg", p["ls"].handle, x.ptr, C.byref(o), C.byref(st))
+                iters += st.iters
+        ctx.sync()
+        return iters
+
+    def run_batched(batch):
+        for p in probs:
+            _lib.call("fv_memcpy_d2d", ctx.handle, p["x"].ptr, p["T0"].ptr, p["x"].nbytes)
+            _lib.call("fv_memcpy_d2d", ctx.handle, p["xo"].ptr, p["T0"].ptr, p["x"].nbytes)
+        iters = 0
+        for s in range(steps):
+            for p in probs:
+                _lib.call("fv_assemble_ddt_laplacian", p["ls"].handle, p["xo"].ptr, 1e-3,
+                          p["kap"].ptr, p["bc"].handle, p["Tb"].ptr, None, -1.0)
+            stats = batch.solve(solver, use_graph=True)   # x_old <- x after the solve
+            iters += sum(st.numIter for st in stats)
+        ctx.sync()
+        return iters
+
+    for p in probs:
+        p["x"] = ctx.empty((p["N"],))
+        p["xo"] = ctx.empty((p["N"],))
+    out = {"problems": P, "steps": steps, "cells_total": total_cells,
+           "mesh_edges": [int(ns.min()), int(ns.max())], "solves": P * steps}
+    # plain FP64 per-problem drop-in
+    t = time.perf_counter()
+    it_plain = run_plain()
+    wall_plain = time.perf_counter() - t
+    x_plain = [p["x"].numpy() for p in probs[:8]]
+    out["plain_fp64"] = {"wall_s": round(wall_plain, 3), "solves_per_s": round(P * steps / wall_plain, 1),
+                         "cg_iters": it_plain}
+    # batched, mixed precision, autotuned shape
+    batch = la.Batch(ctx, "mixed")
+    for p in probs:
+        batch.add(p["ls"], p["x"], p["xo"])
+    for p in probs:
+        _lib.call("fv_memcpy_d2d", ctx.handle, p["x"].ptr, p["T0"].ptr, p["x"].nbytes)
+        _lib.call("fv_memcpy_d2d", ctx.handle, p["xo"].ptr, p["T0"].ptr, p["x"].nbytes)
+        _lib.call("fv_assemble_ddt_laplacian", p["ls"].handle, p["xo"].ptr, 1e-3, p["kap"].ptr,
+                  p["bc"].handle, p["Tb"].ptr, None, -1.0)
+    t = time.perf_counter()
+    shape = batch.autotune(solver, str(ROOT / "foamadapter_amd" / "batch_tuning.json"))
+    out["autotune"] = {"tile_rows": shape[0], "chunk": shape[1],
+                       "seconds": round(time.perf_counter() - t, 2)}
+    run_batched(batch)                                    # warm (graph capture)
+    t = time.perf_counter()
+    it_b = run_batched(batch)
+    wall_b = time.perf_counter() - t
+    err = max(float(np.max(np.abs(p["x"].numpy() - xp)) / np.max(np.abs(xp)))
+              for p, xp in zip(probs[:8], x_plain))
+    out["batched_mixed"] = {"wall_s": round(wall_b, 3), "solves_per_s": round(P * steps / wall_b, 1),
+                            "cg_iters": it_b, "max_rel_diff_vs_plain_first8": err}
+    out["speedup"] = round(wall_plain / wall_b, 2)
+    batch.close()
+    for p in probs:
+        for k in ("Tb", "T0", "kap", "x", "xo"):
+            p[k].close()
+        p["ls"].close()
+    for m in meshes.values():
+        m.close()
+    return out
+
+
+def _uniform_host(n, lo, hi, seed, stream):
+    """Same counter-based generator as fv_fill_uniform."""
+    M = np.uint64(0xFFFFFFFFFFFFFFFF)
+
+    def mix(x):
+        with np.errstate(over="ignore"):
+            x = (x + np.uint64(0x9E3779B97F4A7C15)) & M
+            x = ((x ^ (x >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)) & M
+            x = ((x ^ (x >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)) & M
+            return x ^ (x >> np.uint64(31))
+
+    i = np.arange(n, dtype=np.uint64)
+    with np.errstate(over="ignore"):
+        k = mix(np.uint64(seed) ^ mix(np.uint64(stream)))
+        v = (mix(k + i) >> np.uint64(11)).astype(np.float64) * (1.0 / 9007199254740992.0)
+    return lo + (hi - lo) * v
+
+
+def bench_served(n_req=96, seed=1005, rate=None):
+    """Served request stream: Poisson arrivals, 70% heat (5 Euler steps, kappa ~
+    logU[1e-4,1e-1]) / 30% pressure Poisson, on jittered permuted hex cubes with
+    edge n ~ logU[16,160] (snapped to a catalogue of resident meshes), 200 GB
+    HBM budget, 1e-8 relative residual.  The same stream is run through the
+    plain FP64 per-request path and through continuous batching (mixed)."""
+    import foamadapter_amd as fa
+    from foamadapter_amd.serve import Server
+
+    cat = [16, 20, 25, 32, 40, 50, 64, 80, 100, 128, 160]
+    u = lambda k: _uniform_host(n_req, 0.0, 1.0, seed, k)  # noqa: E731
+    n_draw = _loguniform(u(0), 16, 160)
+    mesh_idx = [int(np.argmin(np.abs(np.log(cat) - np.log(v)))) for v in n_draw]
+    kind = (u(1) < 0.3).astype(int)          # 1: Poisson
+    kappa = _loguniform(u(2), 1e-4, 1e-1)
+    gaps = -np.log(1.0 - u(3))               # unit-rate exponential inter-arrival gaps
+    t0 = time.time()
+    polys = [fa.PolyMesh.hex(n, jitter=0.2, jitter_seed=seed + n, perm_seed=seed + n) for n in cat]
+    log(f"served: catalogue of {len(cat)} meshes generated in {time.time() - t0:.1f}s")
+    results = {}
+    budget = 200 << 30
+
+    def run(label, **opts):
+        srv = Server(0, hbm_budget=budget, **opts)
+        t1 = time.time()
+        ids = [srv.add_mesh(p, renumber=True) for p in polys]
+        log(f"served[{label}]: meshes resident in {time.time() - t1:.1f}s")
+        # warm-up request per kind
+        for r in (srv.submit_heat(ids[0], 1e-2, steps=1, seed=1), srv.submit_poisson(ids[0], seed=2)):
+            srv.wait(r)
+        srv_stats0 = srv.stats()
+        rids = []
+        start = time.perf_counter()
+        tnext = start
+        for i in range(n_req):
+            if rate:
+                tnext += gaps[i] / rate
+                d = tnext - time.perf_counter()
+                if d > 0:
+                    time.sleep(d)
+            mid = ids[mesh_idx[i]]
+            if kind[i]:
+                rids.append(srv.submit_poisson(mid, seed=seed + 10 * i, rel_tol=1e-8))
+            else:
+                rids.append(srv.submit_heat(mid, kappa=float(kappa[i]), dt=1e-3, steps=5,
+                                            seed=seed + 10 * i, rel_tol=1e-8))
+        lat, bad = [], 0
+        for r in rids:
+            _, st = srv.wait(r)
+            if st.status == 0:
+                lat.append(st.latency_ms)
+            else:
+                bad += 1
+        wall = time.perf_counter() - start
+        s = srv.stats()
+        srv.close()
+        lat = np.array(lat)
+        return {"solves_per_s": round(len(lat) / wall, 3), "wall_s": round(wall, 2),
+                "p50_ms": round(float(np.percentile(lat, 50)), 2),
+                "p99_ms": round(float(np.percentile(lat, 99)), 2),
+                "mean_ms": round(float(lat.mean()), 2), "failed": bad,
+                "peak_active": s["peak_active"], "peak_gb": round(s["peak_bytes"] / 1e9, 2),
+                "warmup_completed": srv_stats0["completed"]}
+
+    results["plain_fp64"] = run("plain", precision="double", batched=False)
+    results["batched_mixed"] = run("batched", precision="mixed", batched=True, tile_rows=1024,
+                                   chunk=16)
+    results["speedup_solves_per_s"] = round(results["batched_mixed"]["solves_per_s"] /
+                                            results["plain_fp64"]["solves_per_s"], 2)
+    results["stream"] = {"requests": n_req, "heat": int((kind == 0).sum()),
+                         "poisson": int(kind.sum()), "rate_per_s": rate,
+                         "mesh_edges": cat}
+    return results
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -288,6 +499,13 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-implicit", action="store_true")
     ap.add_argument("--no-as-given", action="store_true")
+    ap.add_argument("--batched", action="store_true",
+                    help="also run the 512-problem batched heat workload")
+    ap.add_argument("--batch-problems", type=int, default=512)
+    ap.add_argument("--served", action="store_true", help="also run the served request stream")
+    ap.add_argument("--requests", type=int, default=96)
+    ap.add_argument("--rate", type=float, default=None,
+                    help="Poisson arrival rate (requests/s) of the served stream; default: all at once")
     ap.add_argument("--copy-probe", action="store_true",
                     help="first copy 2 GiB device-to-device 3 times (a known-size reference "
                          "for hardware traffic counters)")
@@ -368,6 +586,12 @@ def main():
         rec["implicit_heat"] = bench_heat(ctx)
         log("pressure Poisson 128^3, FP64 vs mixed")
         rec["implicit_poisson"] = bench_poisson(ctx)
+    if args.batched:
+        log("batched heat solves")
+        rec["batched_heat"] = bench_batched(ctx, P=args.batch_problems)
+    if args.served:
+        log("served request stream")
+        rec["served"] = bench_served(n_req=args.requests, rate=args.rate)
     print(json.dumps(rec), flush=True)
     ctx.close()
     return 0

@@ -135,6 +135,7 @@ _SIGS = {
     "fv_assemble_explicit": [_vp, _vp, _d],
     "fv_assemble_laplacian": [_vp, _vp, _vp, _vp, _vp, _d],
     "fv_assemble_div": [_vp, C.c_int, _vp, _vp, _vp, _vp, _d],
+    "fv_assemble_ddt_laplacian": [_vp, _vp, _d, _vp, _vp, _vp, _vp, _d],
     "fv_set_reference": [_vp, C.c_int, _d],
     "fv_residual": [_vp, _vp, _vp],
     "fv_solve_cg": [_vp, _vp, C.POINTER(FvSolveOpts), C.POINTER(FvSolverStats)],
@@ -145,6 +146,8 @@ _SIGS = {
     "fv_batch_size": [_vp, C.POINTER(C.c_int)],
     "fv_batch_set_shape": [_vp, C.c_int, C.c_int],
     "fv_batch_solve": [_vp, C.POINTER(FvSolveOpts), C.c_int, C.POINTER(FvSolverStats)],
+    "fv_batch_autotune": [_vp, C.POINTER(FvSolveOpts), C.c_char_p, C.POINTER(C.c_int),
+                          C.POINTER(C.c_int)],
 }
 _RESTYPES = {
     "fv_last_error": C.c_char_p,

@@ -15,11 +15,15 @@
 // replacement (x += d, r = b - A x in FP64, convergence test in FP64, CG
 // continues from the replaced residual without restarting its direction), so
 // each system stops on its true FP64 relative residual.  A chunk is captured
-// once as a hipGraph and replayed until every system has stopped.
-#include "cgstate.hpp"
+// once as a hipGraph (per membership) and replayed until every system has
+// stopped.  Members can join and leave between chunks (continuous batching):
+// joining members are started by their own tiles, the others carry on.
+#include "batch.hpp"
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
+#include <string>
 #include <vector>
 
 using namespace fvhip;
@@ -208,18 +212,6 @@ __global__ __launch_bounds__(kB) void kb_replace(const BTile* __restrict__ tiles
     }
 }
 
-// after convergence: xold = x (Euler T_old <- T), once
-template <typename S>
-__global__ __launch_bounds__(kB) void kb_store_old(const BTile* __restrict__ tiles, int nTiles,
-                                                   const BProb<S>* __restrict__ probs) {
-    if ((int)blockIdx.x >= nTiles) return;
-    const BTile t = tiles[blockIdx.x];
-    const BProb<S>& P = probs[t.prob];
-    if (!P.xold) return;
-    const int r1 = min(t.row1, P.N);
-    for (int i = t.row0 + threadIdx.x; i < r1; i += kB) P.xold[i] = P.x[i];
-}
-
 template <typename S>
 __global__ void kb_sell_values(int64_t n, const int32_t* __restrict__ src,
                                const double* __restrict__ values, S* __restrict__ out) {
@@ -256,48 +248,24 @@ __global__ void kb_zero(int64_t n, uint8_t* __restrict__ p) {
 }
 
 }  // namespace
-
 // ---- host side ----------------------------------------------------------------
 
-struct BMember {
-    fv_linsys* ls = nullptr;
-    double* x = nullptr;
-    double* xold = nullptr;
-    int id = -1;
-    int N = 0;
-    DBuf<uint8_t> vec;   // d, r, z, p, q, dinv (S) + r64 (double)
-    DBuf<uint8_t> sval;  // sliced-ELL values in S
-    fv_solver_stats stats{};
-    bool finished = false;
-};
-
-struct fv_batch {
-    fv_ctx* ctx = nullptr;
-    int prec = FV_SOLVE_F32_IR;  // inner precision of the whole batch
-    int tileRows = 1024;         // rows per tile (multiple of 64)
-    int chunk = 16;              // CG iterations per replacement
-    int nextId = 0;
-    std::vector<BMember*> members;
-    // device tables, rebuilt when membership changes
-    DBuf<uint8_t> probs;
-    DBuf<BTile> tiles;
-    DBuf<CgState> states;
-    DBuf<double> partials;
-    int nTiles = 0;
-    bool dirty = true;
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t graphExec = nullptr;
-    int graphTiles = -1, graphChunk = -1;
-    ~fv_batch() {
-        if (graphExec) (void)hipGraphExecDestroy(graphExec);
-        if (graph) (void)hipGraphDestroy(graph);
-        for (auto* m : members) delete m;
-    }
-};
+fv_batch::~fv_batch() {
+    if (graphExec) (void)hipGraphExecDestroy(graphExec);
+    if (graph) (void)hipGraphDestroy(graph);
+    for (auto* m : members) delete m;
+}
 
 namespace {
 
 size_t elem(int prec) { return prec == FV_SOLVE_F32_IR ? sizeof(float) : sizeof(double); }
+
+void drop_graph(fv_batch* b) {
+    if (b->graphExec) (void)hipGraphExecDestroy(b->graphExec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    b->graphExec = nullptr;
+    b->graph = nullptr;
+}
 
 template <typename S>
 void build_tables(fv_batch* b) {
@@ -305,7 +273,6 @@ void build_tables(fv_batch* b) {
     const int P = (int)b->members.size();
     std::vector<BProb<S>> hp(P);
     std::vector<BTile> ht;
-    b->states.alloc(ctx, std::max(P, 1));
     for (int i = 0; i < P; ++i) {
         BMember* m = b->members[i];
         const fv_sparsity* sp = m->ls->sp;
@@ -339,120 +306,199 @@ void build_tables(fv_batch* b) {
         p.q = v + 4 * (int64_t)N;
         p.dinv = v + 5 * (int64_t)N;
         p.r64 = (double*)(m->vec.p + 6 * (int64_t)N * sizeof(S));
-        p.st = b->states.p + i;
+        p.st = b->states.p + m->slot;
     }
     b->nTiles = (int)ht.size();
     b->probs.alloc(ctx, std::max<size_t>(1, P * sizeof(BProb<S>)));
-    if (P) FV_HIP(hipMemcpyAsync(b->probs.p, hp.data(), P * sizeof(BProb<S>), hipMemcpyHostToDevice, ctx->stream));
-    b->tiles.alloc(ctx, std::max<size_t>(1, ht.size()));
-    if (!ht.empty()) FV_HIP(hipMemcpyAsync(b->tiles.p, ht.data(), ht.size() * sizeof(BTile), hipMemcpyHostToDevice, ctx->stream));
+    if (P)
+        FV_HIP(hipMemcpyAsync(b->probs.p, hp.data(), P * sizeof(BProb<S>), hipMemcpyHostToDevice,
+                              ctx->stream));
+    b->tiles.alloc(ctx, std::max<size_t>(1, ht.size() * sizeof(BTile)));
+    if (!ht.empty())
+        FV_HIP(hipMemcpyAsync(b->tiles.p, ht.data(), ht.size() * sizeof(BTile),
+                              hipMemcpyHostToDevice, ctx->stream));
     b->partials.alloc(ctx, 2 * std::max<size_t>(1, ht.size()));
     FV_HIP(hipStreamSynchronize(ctx->stream));
     b->dirty = false;
-    if (b->graphExec) {
-        (void)hipGraphExecDestroy(b->graphExec);
-        b->graphExec = nullptr;
-    }
-    if (b->graph) {
-        (void)hipGraphDestroy(b->graph);
-        b->graph = nullptr;
-    }
+    drop_graph(b);
 }
 
 template <typename S>
-void enqueue_chunk(fv_batch* b, int chunk) {
+void enqueue_chunk(fv_batch* b) {
     hipStream_t s = b->ctx->stream;
     const auto* probs = (const BProb<S>*)b->probs.p;
+    const auto* tiles = (const BTile*)b->tiles.p;
     const int nt = b->nTiles;
-    for (int i = 0; i < chunk; ++i) {
-        kb_spmv<S><<<nt, kB, 0, s>>>(b->tiles.p, nt, probs, b->partials.p);
-        kb_update<S><<<nt, kB, 0, s>>>(b->tiles.p, nt, probs, b->partials.p);
-        if (i + 1 < chunk) kb_direction<S><<<nt, kB, 0, s>>>(b->tiles.p, nt, probs);
+    for (int i = 0; i < b->chunk; ++i) {
+        kb_spmv<S><<<nt, kB, 0, s>>>(tiles, nt, probs, b->partials.p);
+        kb_update<S><<<nt, kB, 0, s>>>(tiles, nt, probs, b->partials.p);
+        // the chunk ends after an update: its direction update uses the replaced residual
+        if (i + 1 < b->chunk) kb_direction<S><<<nt, kB, 0, s>>>(tiles, nt, probs);
     }
-    kb_accum<S><<<nt, kB, 0, s>>>(b->tiles.p, nt, probs);
-    kb_resid<S><<<nt, kB, 0, s>>>(b->tiles.p, nt, probs, b->partials.p, 0);
-    kb_replace<S><<<nt, kB, 0, s>>>(b->tiles.p, nt, probs, b->partials.p, 0);
-    kb_direction<S><<<nt, kB, 0, s>>>(b->tiles.p, nt, probs);
+    kb_accum<S><<<nt, kB, 0, s>>>(tiles, nt, probs);
+    kb_resid<S><<<nt, kB, 0, s>>>(tiles, nt, probs, b->partials.p, 0);
+    kb_replace<S><<<nt, kB, 0, s>>>(tiles, nt, probs, b->partials.p, 0);
+    kb_direction<S><<<nt, kB, 0, s>>>(tiles, nt, probs);
 }
 
-// Capture one chunk as a graph (kernels only) and replay it.
 template <typename S>
-void run_chunk(fv_batch* b, bool use_graph) {
+void run_chunk(fv_batch* b) {
     hipStream_t s = b->ctx->stream;
-    if (!use_graph) {
-        enqueue_chunk<S>(b, b->chunk);
+    if (!b->useGraph) {
+        enqueue_chunk<S>(b);
         FV_LAUNCH_CHECK();
         return;
     }
-    if (!b->graphExec || b->graphTiles != b->nTiles || b->graphChunk != b->chunk) {
-        if (b->graphExec) (void)hipGraphExecDestroy(b->graphExec);
-        if (b->graph) (void)hipGraphDestroy(b->graph);
-        b->graphExec = nullptr;
-        b->graph = nullptr;
+    if (!b->graphExec) {
+        // the chunk holds kernels only; it is captured once per membership
         FV_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-        enqueue_chunk<S>(b, b->chunk);
-        FV_HIP(hipStreamEndCapture(s, &b->graph));
+        enqueue_chunk<S>(b);
+        hipError_t e = hipStreamEndCapture(s, &b->graph);
+        if (e != hipSuccess) {
+            b->graph = nullptr;
+            throw Error{FV_EDEVICE, std::string("hipStreamEndCapture: ") + hipGetErrorString(e)};
+        }
         FV_HIP(hipGraphInstantiate(&b->graphExec, b->graph, nullptr, nullptr, 0));
-        b->graphTiles = b->nTiles;
-        b->graphChunk = b->chunk;
     }
     FV_HIP(hipGraphLaunch(b->graphExec, s));
 }
 
 template <typename S>
-void prepare_members(fv_batch* b) {
-    hipStream_t s = b->ctx->stream;
-    for (BMember* m : b->members) {
-        const fv_sparsity* sp = m->ls->sp;
-        kb_sell_values<S><<<grid_for(sp->sellTotal, kB, 8192), kB, 0, s>>>(
-            sp->sellTotal, sp->sellSrc.p, m->ls->values.p, (S*)m->sval.p);
-        kb_dinv<S><<<grid_for(m->N, kB), kB, 0, s>>>(m->N, sp->rowOffs.p, sp->diagOffset.p,
-                                                     m->ls->values.p,
-                                                     (S*)m->vec.p + 5 * (int64_t)m->N);
-        kb_zero<<<grid_for(m->N * (int64_t)sizeof(S), kB, 4096), kB, 0, s>>>(
-            m->N * (int64_t)sizeof(S), m->vec.p);  // d = 0
+void start_members(fv_batch* b) {
+    fv_ctx* ctx = b->ctx;
+    hipStream_t s = ctx->stream;
+    std::vector<BTile> st;
+    // tile list of the members that (re)start, with their global slots
+    int slot = 0;
+    for (int i = 0; i < (int)b->members.size(); ++i) {
+        BMember* m = b->members[i];
+        const int nt = (m->N + b->tileRows - 1) / b->tileRows;
+        if (m->needStart) {
+            for (int k = 0; k < nt; ++k) {
+                BTile t;
+                t.prob = i;
+                t.row0 = k * b->tileRows;
+                t.row1 = std::min(t.row0 + b->tileRows, ((m->N + 63) / 64) * 64);
+                t.slot = slot + k;
+                st.push_back(t);
+            }
+            const fv_sparsity* sp = m->ls->sp;
+            kb_sell_values<S><<<grid_for(sp->sellTotal, kB, 8192), kB, 0, s>>>(
+                sp->sellTotal, sp->sellSrc.p, m->ls->values.p, (S*)m->sval.p);
+            kb_dinv<S><<<grid_for(m->N, kB), kB, 0, s>>>(m->N, sp->rowOffs.p, sp->diagOffset.p,
+                                                         m->ls->values.p,
+                                                         (S*)m->vec.p + 5 * (int64_t)m->N);
+            kb_zero<<<grid_for(m->N * (int64_t)sizeof(S), kB, 4096), kB, 0, s>>>(
+                m->N * (int64_t)sizeof(S), m->vec.p);  // d = 0
+            kb_init_states<<<1, 64, 0, s>>>(b->states.p + m->slot, 1, b->tol, b->maxit, b->delta);
+            m->needStart = false;
+        }
+        slot += nt;
     }
+    if (st.empty()) return;
+    b->startTiles.alloc(ctx, st.size() * sizeof(BTile));
+    FV_HIP(hipMemcpyAsync(b->startTiles.p, st.data(), st.size() * sizeof(BTile),
+                          hipMemcpyHostToDevice, s));
+    const auto* probs = (const BProb<S>*)b->probs.p;
+    const auto* tiles = (const BTile*)b->startTiles.p;
+    const int nt = (int)st.size();
+    kb_resid<S><<<nt, kB, 0, s>>>(tiles, nt, probs, b->partials.p, 1);
+    kb_replace<S><<<nt, kB, 0, s>>>(tiles, nt, probs, b->partials.p, 1);
     FV_LAUNCH_CHECK();
+    // the upload buffer must outlive the launches
+    FV_HIP(hipStreamSynchronize(s));
 }
 
 template <typename S>
-void solve_all(fv_batch* b, double tol, int maxit, double delta, bool use_graph) {
-    fv_ctx* ctx = b->ctx;
-    hipStream_t s = ctx->stream;
+void step_t(fv_batch* b, std::vector<CgState>& out) {
     if (b->dirty) build_tables<S>(b);
-    const int P = (int)b->members.size();
-    if (!P) return;
-    prepare_members<S>(b);
-    const auto* probs = (const BProb<S>*)b->probs.p;
-    const int nt = b->nTiles;
-    kb_init_states<<<grid_for(P, 64), 64, 0, s>>>(b->states.p, P, tol, maxit, delta);
-    kb_resid<S><<<nt, kB, 0, s>>>(b->tiles.p, nt, probs, b->partials.p, 1);
-    kb_replace<S><<<nt, kB, 0, s>>>(b->tiles.p, nt, probs, b->partials.p, 1);
-    FV_LAUNCH_CHECK();
-    std::vector<CgState> hs(P);
-    while (true) {
-        FV_HIP(hipMemcpyAsync(hs.data(), b->states.p, P * sizeof(CgState), hipMemcpyDeviceToHost, s));
-        FV_HIP(hipStreamSynchronize(s));
-        bool running = false;
-        for (const CgState& st : hs)
-            if (st.done != 2 && st.it < st.maxit) running = true;
-        if (!running) break;
-        run_chunk<S>(b, use_graph);
+    out.clear();
+    if (b->members.empty()) return;
+    start_members<S>(b);
+    run_chunk<S>(b);
+    std::vector<CgState> all(b->slotCap);
+    hipStream_t s = b->ctx->stream;
+    FV_HIP(hipMemcpyAsync(all.data(), b->states.p, b->slotCap * sizeof(CgState),
+                          hipMemcpyDeviceToHost, s));
+    FV_HIP(hipStreamSynchronize(s));
+    for (BMember* m : b->members) out.push_back(all[m->slot]);
+}
+
+int grow_slots(fv_batch* b) {
+    if (b->freeSlots.empty()) {
+        const int cap = std::max(8, 2 * b->slotCap);
+        DBuf<CgState> ns;
+        ns.alloc(b->ctx, cap);
+        if (b->slotCap)
+            FV_HIP(hipMemcpyAsync(ns.p, b->states.p, b->slotCap * sizeof(CgState),
+                                  hipMemcpyDeviceToDevice, b->ctx->stream));
+        FV_HIP(hipStreamSynchronize(b->ctx->stream));
+        b->states = std::move(ns);
+        for (int i = cap - 1; i >= b->slotCap; --i) b->freeSlots.push_back(i);
+        b->slotCap = cap;
+        b->dirty = true;
     }
-    for (int i = 0; i < P; ++i) {
-        BMember* m = b->members[i];
-        m->stats.init_res = hs[i].res0;
-        m->stats.final_res = hs[i].res;
-        m->stats.true_res = hs[i].res;
-        m->stats.iters = hs[i].it;
-        m->stats.outer_iters = 0;
-        m->finished = true;
-    }
-    kb_store_old<S><<<nt, kB, 0, s>>>(b->tiles.p, nt, probs);
-    FV_LAUNCH_CHECK();
+    const int s = b->freeSlots.back();
+    b->freeSlots.pop_back();
+    return s;
 }
 
 }  // namespace
+
+namespace fvhip {
+
+BMember* batch_add(fv_batch* b, fv_linsys* ls, double* x, double* xold, void* user) {
+    FV_REQUIRE(ls && ls->sp && x, "NULL system or vector");
+    FV_REQUIRE(ls->sp->mesh->ctx == b->ctx, "linear system belongs to another context");
+    auto* m = new BMember();
+    try {
+        m->ls = ls;
+        m->x = x;
+        m->xold = xold;
+        m->N = ls->sp->nRows;
+        m->user = user;
+        const size_t e = elem(b->prec);
+        m->vec.alloc(b->ctx, 6 * (size_t)m->N * e + (size_t)m->N * sizeof(double));
+        m->sval.alloc(b->ctx, (size_t)ls->sp->sellTotal * e);
+        m->slot = grow_slots(b);
+    } catch (...) {
+        delete m;
+        throw;
+    }
+    m->id = b->nextId++;
+    m->needStart = true;
+    b->members.push_back(m);
+    b->dirty = true;
+    return m;
+}
+
+void batch_remove(fv_batch* b, BMember* m) {
+    for (size_t i = 0; i < b->members.size(); ++i)
+        if (b->members[i] == m) {
+            b->members.erase(b->members.begin() + i);
+            b->freeSlots.push_back(m->slot);
+            delete m;
+            b->dirty = true;
+            return;
+        }
+}
+
+void batch_restart(fv_batch*, BMember* m) { m->needStart = true; }
+
+void batch_step(fv_batch* b, std::vector<CgState>& states) {
+    if (b->prec == FV_SOLVE_F32_IR)
+        step_t<float>(b, states);
+    else
+        step_t<double>(b, states);
+}
+
+void batch_copy_old(fv_batch* b, BMember* m) {
+    if (!m->xold) return;
+    FV_HIP(hipMemcpyAsync(m->xold, m->x, m->N * sizeof(double), hipMemcpyDeviceToDevice,
+                          b->ctx->stream));
+}
+
+}  // namespace fvhip
 
 extern "C" {
 
@@ -481,24 +527,8 @@ int fv_batch_destroy(fv_batch* b) {
 
 int fv_batch_add(fv_batch* b, fv_linsys* ls, void* d_x, void* d_xold, int* id) {
     return guarded([&] {
-        FV_REQUIRE(b && ls && ls->sp && d_x && id, "NULL argument");
-        FV_REQUIRE(ls->sp->mesh->ctx == b->ctx, "linear system belongs to another context");
-        auto* m = new BMember();
-        try {
-            m->ls = ls;
-            m->x = (double*)d_x;
-            m->xold = (double*)d_xold;
-            m->N = ls->sp->nRows;
-            const size_t e = elem(b->prec);
-            m->vec.alloc(b->ctx, 6 * (size_t)m->N * e + (size_t)m->N * sizeof(double));
-            m->sval.alloc(b->ctx, (size_t)ls->sp->sellTotal * e);
-        } catch (...) {
-            delete m;
-            throw;
-        }
-        m->id = b->nextId++;
-        b->members.push_back(m);
-        b->dirty = true;
+        FV_REQUIRE(b && id, "NULL argument");
+        BMember* m = batch_add(b, ls, (double*)d_x, (double*)d_xold, nullptr);
         *id = m->id;
     });
 }
@@ -506,9 +536,7 @@ int fv_batch_add(fv_batch* b, fv_linsys* ls, void* d_x, void* d_xold, int* id) {
 int fv_batch_clear(fv_batch* b) {
     return guarded([&] {
         FV_REQUIRE(b, "NULL batch");
-        for (auto* m : b->members) delete m;
-        b->members.clear();
-        b->dirty = true;
+        while (!b->members.empty()) batch_remove(b, b->members.back());
     });
 }
 
@@ -522,11 +550,161 @@ int fv_batch_size(const fv_batch* b, int* n) {
 int fv_batch_set_shape(fv_batch* b, int tile_rows, int chunk) {
     return guarded([&] {
         FV_REQUIRE(b, "NULL batch");
-        FV_REQUIRE(tile_rows >= 64 && tile_rows % 64 == 0, "tile_rows must be a multiple of 64");
+        FV_REQUIRE(tile_rows >= 64 && tile_rows % 64 == 0 && tile_rows <= (1 << 20),
+                   "tile_rows must be a positive multiple of 64");
         FV_REQUIRE(chunk >= 1 && chunk <= 4096, "chunk must be in [1, 4096]");
         if (tile_rows != b->tileRows) b->dirty = true;
+        if (chunk != b->chunk) drop_graph(b);
         b->tileRows = tile_rows;
         b->chunk = chunk;
+    });
+}
+
+}  // extern "C"
+
+// ---- autotuning of the batch shape ---------------------------------------------
+
+namespace {
+
+std::string tune_key(const fv_batch* b) {
+    int64_t total = 0, mx = 0;
+    for (const BMember* m : b->members) {
+        total += m->N;
+        mx = std::max<int64_t>(mx, m->N);
+    }
+    auto lg = [](int64_t v) {
+        int l = 0;
+        while ((int64_t(1) << (l + 1)) <= v) ++l;
+        return l;
+    };
+    return std::string(b->prec == FV_SOLVE_F32_IR ? "mixed" : "fp64") + "_P" +
+           std::to_string(lg((int64_t)b->members.size())) + "_N" + std::to_string(lg(total)) +
+           "_M" + std::to_string(lg(mx));
+}
+
+// minimal JSON cache: {"<key>": {"tile_rows": T, "chunk": C, "ms": M}, ...}
+bool cache_lookup(const char* path, const std::string& key, int* tile, int* chunk) {
+    if (!path) return false;
+    FILE* f = fopen(path, "r");
+    if (!f) return false;
+    std::string s;
+    char buf[4096];
+    size_t n;
+    while ((n = fread(buf, 1, sizeof buf, f)) > 0) s.append(buf, n);
+    fclose(f);
+    const size_t k = s.find("\"" + key + "\"");
+    if (k == std::string::npos) return false;
+    const size_t t = s.find("\"tile_rows\":", k), c = s.find("\"chunk\":", k);
+    if (t == std::string::npos || c == std::string::npos) return false;
+    *tile = atoi(s.c_str() + t + 12);
+    *chunk = atoi(s.c_str() + c + 8);
+    return *tile >= 64 && *tile % 64 == 0 && *chunk >= 1;
+}
+
+void cache_store(const char* path, const std::string& key, int tile, int chunk, double ms) {
+    if (!path) return;
+    std::string s;
+    if (FILE* f = fopen(path, "r")) {
+        char buf[4096];
+        size_t n;
+        while ((n = fread(buf, 1, sizeof buf, f)) > 0) s.append(buf, n);
+        fclose(f);
+    }
+    const size_t e = s.rfind('}');
+    char entry[256];
+    snprintf(entry, sizeof entry, "\"%s\": {\"tile_rows\": %d, \"chunk\": %d, \"ms\": %.4f}",
+             key.c_str(), tile, chunk, ms);
+    if (e == std::string::npos || s.find('{') == std::string::npos) {
+        s = std::string("{\n  ") + entry + "\n}\n";
+    } else {
+        const bool empty = s.find(':') == std::string::npos;
+        s = s.substr(0, e);
+        while (!s.empty() && (s.back() == '\n' || s.back() == ' ')) s.pop_back();
+        s += std::string(empty ? "\n  " : ",\n  ") + entry + "\n}\n";
+    }
+    if (FILE* f = fopen(path, "w")) {
+        fwrite(s.data(), 1, s.size(), f);
+        fclose(f);
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+int fv_batch_autotune(fv_batch* b, const fv_solve_opts* o, const char* cache_path, int* tile_rows,
+                      int* chunk) {
+    return guarded([&] {
+        FV_REQUIRE(b && o && tile_rows && chunk, "NULL argument");
+        FV_REQUIRE(!b->members.empty(), "empty batch");
+        const std::string key = tune_key(b);
+        int t = 0, c = 0;
+        if (cache_lookup(cache_path, key, &t, &c)) {
+            FV_HIP(hipStreamSynchronize(b->ctx->stream));
+            *tile_rows = t;
+            *chunk = c;
+            int rc = fv_batch_set_shape(b, t, c);
+            if (rc != FV_OK) throw Error{rc, fv_last_error()};
+            return;
+        }
+        fv_ctx* ctx = b->ctx;
+        hipStream_t s = ctx->stream;
+        // every trial starts from the same x (and x_old)
+        std::vector<DBuf<double>> save(2 * b->members.size());
+        for (size_t i = 0; i < b->members.size(); ++i) {
+            BMember* m = b->members[i];
+            save[2 * i].alloc(ctx, m->N);
+            FV_HIP(hipMemcpyAsync(save[2 * i].p, m->x, m->N * sizeof(double), hipMemcpyDeviceToDevice, s));
+            if (m->xold) {
+                save[2 * i + 1].alloc(ctx, m->N);
+                FV_HIP(hipMemcpyAsync(save[2 * i + 1].p, m->xold, m->N * sizeof(double),
+                                      hipMemcpyDeviceToDevice, s));
+            }
+        }
+        auto restore = [&] {
+            for (size_t i = 0; i < b->members.size(); ++i) {
+                BMember* m = b->members[i];
+                FV_HIP(hipMemcpyAsync(m->x, save[2 * i].p, m->N * sizeof(double), hipMemcpyDeviceToDevice, s));
+                if (m->xold)
+                    FV_HIP(hipMemcpyAsync(m->xold, save[2 * i + 1].p, m->N * sizeof(double),
+                                          hipMemcpyDeviceToDevice, s));
+            }
+        };
+        hipEvent_t e0, e1;
+        FV_HIP(hipEventCreate(&e0));
+        FV_HIP(hipEventCreate(&e1));
+        double best = 1e300;
+        int bt = b->tileRows, bc = b->chunk;
+        const int tiles[] = {256, 512, 1024, 2048, 4096};
+        const int chunks[] = {8, 16, 32};
+        for (int tr : tiles)
+            for (int ch : chunks) {
+                restore();
+                if (fv_batch_set_shape(b, tr, ch) != FV_OK) continue;
+                // first run builds tables and captures the graph, second is timed
+                (void)fv_batch_solve(b, o, b->useGraph ? 1 : 0, nullptr);
+                restore();
+                FV_HIP(hipEventRecord(e0, s));
+                const int rc = fv_batch_solve(b, o, b->useGraph ? 1 : 0, nullptr);
+                FV_HIP(hipEventRecord(e1, s));
+                FV_HIP(hipEventSynchronize(e1));
+                float ms = 0;
+                FV_HIP(hipEventElapsedTime(&ms, e0, e1));
+                if (rc == FV_OK && ms < best) {
+                    best = ms;
+                    bt = tr;
+                    bc = ch;
+                }
+            }
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        restore();
+        FV_HIP(hipStreamSynchronize(s));
+        int rc = fv_batch_set_shape(b, bt, bc);
+        if (rc != FV_OK) throw Error{rc, fv_last_error()};
+        cache_store(cache_path, key, bt, bc, best);
+        *tile_rows = bt;
+        *chunk = bc;
     });
 }
 
@@ -534,16 +712,36 @@ int fv_batch_solve(fv_batch* b, const fv_solve_opts* o, int use_graph, fv_solver
     return guarded([&] {
         FV_REQUIRE(b && o, "NULL argument");
         FV_REQUIRE(o->rel_tol >= 0.0 && o->max_iter >= 0, "bad solver options");
-        const double delta = o->inner_rel_tol > 0.0 ? o->inner_rel_tol : 1e-3;
-        if (b->prec == FV_SOLVE_F32_IR)
-            solve_all<float>(b, o->rel_tol, o->max_iter, delta, use_graph != 0);
-        else
-            solve_all<double>(b, o->rel_tol, o->max_iter, 0.0, use_graph != 0);
+        b->tol = o->rel_tol;
+        b->maxit = o->max_iter;
+        b->delta = b->prec == FV_SOLVE_F32_IR ? (o->inner_rel_tol > 0.0 ? o->inner_rel_tol : 1e-3)
+                                              : 0.0;
+        if (b->useGraph != (use_graph != 0)) drop_graph(b);
+        b->useGraph = use_graph != 0;
+        for (BMember* m : b->members) m->needStart = true;
+        std::vector<CgState> hs;
+        int chunks = 0;
+        while (true) {
+            batch_step(b, hs);
+            ++chunks;
+            bool running = false;
+            for (const CgState& st : hs)
+                if (st.done != 2 && st.it < st.maxit) running = true;
+            if (!running) break;
+        }
         int bad = 0;
         for (size_t i = 0; i < b->members.size(); ++i) {
-            if (stats) stats[i] = b->members[i]->stats;
-            if (!(b->members[i]->stats.final_res <= o->rel_tol)) ++bad;
+            BMember* m = b->members[i];
+            m->stats.init_res = hs[i].res0;
+            m->stats.final_res = hs[i].res;
+            m->stats.true_res = hs[i].res;
+            m->stats.iters = hs[i].it;
+            m->stats.outer_iters = chunks;
+            batch_copy_old(b, m);
+            if (stats) stats[i] = m->stats;
+            if (!(hs[i].res <= o->rel_tol)) ++bad;
         }
+        FV_HIP(hipStreamSynchronize(b->ctx->stream));
         if (bad)
             throw Error{FV_ENOCONVERGE, std::to_string(bad) + " system(s) of the batch did not converge"};
     });

@@ -1,0 +1,63 @@
+// batch.hpp -- the batched CG engine's host interface, shared by the C ABI
+// (fv_batch_*) and the request server.
+#pragma once
+
+#include "cgstate.hpp"
+
+#include <vector>
+
+struct BMember {
+    fv_linsys* ls = nullptr;
+    double* x = nullptr;
+    double* xold = nullptr;
+    int id = -1;
+    int N = 0;
+    int slot = -1;           // index of the member's CgState (stable while it is in the batch)
+    bool needStart = true;   // matrix/right-hand side (re)loaded, CG must (re)start
+    fvhip::DBuf<uint8_t> vec;   // d, r, z, p, q, dinv (S) + r64 (double)
+    fvhip::DBuf<uint8_t> sval;  // sliced-ELL values in S
+    fv_solver_stats stats{};
+    void* user = nullptr;    // owner's tag (the server's request)
+};
+
+struct fv_batch {
+    fv_ctx* ctx = nullptr;
+    int prec = FV_SOLVE_F32_IR;  // inner precision of the whole batch
+    int tileRows = 1024;         // rows per tile (multiple of 64)
+    int chunk = 16;              // CG iterations per FP64 residual replacement
+    double tol = 1e-8, delta = 1e-3;
+    int maxit = 1000;
+    int nextId = 0;
+    std::vector<BMember*> members;
+    std::vector<int> freeSlots;
+    int slotCap = 0;
+    // device tables, rebuilt when membership changes
+    fvhip::DBuf<uint8_t> probs;
+    fvhip::DBuf<uint8_t> tiles;
+    fvhip::DBuf<uint8_t> startTiles;
+    fvhip::DBuf<fvhip::CgState> states;
+    fvhip::DBuf<double> partials;
+    int nTiles = 0, nStartTiles = 0;
+    bool dirty = true;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graphExec = nullptr;
+    bool useGraph = true;
+    ~fv_batch();
+};
+
+namespace fvhip {
+
+BMember* batch_add(fv_batch* b, fv_linsys* ls, double* x, double* xold, void* user);
+void batch_remove(fv_batch* b, BMember* m);
+// Mark a member whose system was re-assembled (and x possibly changed): its
+// CG restarts from x at the next step.
+void batch_restart(fv_batch* b, BMember* m);
+// One step: (re)start members that need it, then one chunk of CG iterations
+// for every member followed by the FP64 residual replacement.  Returns the
+// host copy of every member's state (in member order).
+void batch_step(fv_batch* b, std::vector<CgState>& states);
+// Launch-only part of a step (no host synchronisation), for callers that
+// overlap the next step's host work.
+void batch_copy_old(fv_batch* b, BMember* m);
+
+}  // namespace fvhip

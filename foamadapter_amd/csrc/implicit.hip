@@ -89,6 +89,47 @@ __global__ __launch_bounds__(kBlock) void k_laplacian(
     rhs[c] += coeff * bsum;
 }
 
+// Fused Euler ddt + coeff * laplacian, overwriting values and rhs: the same
+// arithmetic as zero -> ddt -> laplacian in that order (0 + x is exact), in one
+// pass over the cell's stencil.
+__global__ __launch_bounds__(kBlock) void k_ddt_laplacian(
+    int N, int F, Stencil st, const int32_t* __restrict__ rowOffs,
+    const int32_t* __restrict__ diag, const double* __restrict__ V,
+    const double* __restrict__ Told, double rdt, const double* __restrict__ magSf,
+    const double* __restrict__ nodc, const double* __restrict__ magSfb,
+    const double* __restrict__ dcb, const int8_t* __restrict__ kind,
+    const double* __restrict__ gamma, const double* __restrict__ Tb,
+    const double* __restrict__ Tg, double coeff, double* __restrict__ values,
+    double* __restrict__ rhs) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= N) return;
+    const int row = rowOffs[c];
+    double dsum = 0.0, bsum = 0.0;
+    const int e1 = st.off[c + 1];
+    for (int k = st.off[c]; k < e1; ++k) {
+        const int e = st.idx[k];
+        if (e >= F) {
+            const int b = e - F;
+            const double gm = gamma[e] * magSfb[b];
+            if (kind[b] == FV_BC_FIXED_GRADIENT) {
+                bsum -= gm * (Tg ? Tg[b] : 0.0);
+            } else {
+                const double ic = gm * dcb[b];
+                dsum -= ic;
+                bsum -= ic * Tb[b];
+            }
+        } else {
+            const int f = e >= 0 ? e : ~e;
+            const double up = nodc[f] * (gamma[f] * magSf[f]);
+            values[row + st.slot[k]] = coeff * up;
+            dsum -= up;
+        }
+    }
+    const double v = V[c];
+    values[row + diag[c]] = (rdt * v) + coeff * dsum;
+    rhs[c] = ((rdt * Told[c]) * v) + coeff * bsum;
+}
+
 __global__ __launch_bounds__(kBlock) void k_div(
     int N, int F, int scheme, Stencil st, const int32_t* __restrict__ rowOffs,
     const int32_t* __restrict__ diag, const double* __restrict__ w,
@@ -379,6 +420,24 @@ int fv_assemble_laplacian(fv_linsys* ls, const void* gamma, const fv_bc* bc, con
             m->nCells, m->nInt, st, sp->rowOffs.p, sp->diagOffset.p, m->magSf.p, m->nodc.p,
             m->magSfb.p, m->dcb.p, bc->faceKind.p, (const double*)gamma, (const double*)Tb,
             (const double*)Tg, coeff, ls->values.p, ls->rhs.p);
+        FV_LAUNCH_CHECK();
+    });
+}
+
+int fv_assemble_ddt_laplacian(fv_linsys* ls, const void* Told, double dt, const void* gamma,
+                              const fv_bc* bc, const void* Tb, const void* Tg, double coeff) {
+    return guarded([&] {
+        check_ls(ls);
+        check_bc(ls, bc);
+        fv_mesh* m = mesh_of(ls);
+        FV_REQUIRE(Told && gamma && (Tb || m->nB == 0) && dt > 0.0, "bad arguments");
+        fv_sparsity* sp = ls->sp;
+        Stencil st{m->cfOff.p, m->cfIdx.p, sp->entrySlot.p};
+        k_ddt_laplacian<<<grid_for(m->nCells, kBlock), kBlock, 0, m->ctx->stream>>>(
+            m->nCells, m->nInt, st, sp->rowOffs.p, sp->diagOffset.p, m->V.p, (const double*)Told,
+            1.0 / dt, m->magSf.p, m->nodc.p, m->magSfb.p, m->dcb.p, bc->faceKind.p,
+            (const double*)gamma, (const double*)Tb, (const double*)Tg, coeff, ls->values.p,
+            ls->rhs.p);
         FV_LAUNCH_CHECK();
     });
 }

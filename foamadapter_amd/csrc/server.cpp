@@ -1,0 +1,605 @@
+// server.cpp -- resident request loop for implicit solves (continuous batching).
+//
+// Requests arrive from any host thread (fv_server_submit) and queue.  A worker
+// thread owns the server's context: it admits queued requests while their
+// estimated device memory fits the HBM budget (and the in-flight limit),
+// builds each admitted request's system on its resident mesh (fields,
+// assembly), and then either
+//   * batched: steps the batched CG engine -- one chunk of iterations for every
+//     system in flight, followed by the FP64 residual replacement -- and after
+//     each step retires converged systems (a heat request re-assembles for its
+//     next Euler step and rejoins; a finished request is read back and its
+//     memory freed), so requests that arrive while others run join at the next
+//     chunk; or
+//   * plain: runs one request at a time through fv_solve_cg (the drop-in path).
+// Latency is measured from submission to the result being on the host.
+#include "batch.hpp"
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <condition_variable>
+#include <deque>
+#include <memory>
+#include <thread>
+#include <unordered_map>
+
+using namespace fvhip;
+
+namespace {
+
+using Clock = std::chrono::steady_clock;
+
+double ms_between(Clock::time_point a, Clock::time_point b) {
+    return std::chrono::duration<double, std::milli>(b - a).count();
+}
+
+struct MeshEntry {
+    fv_mesh* mesh = nullptr;
+    fv_sparsity* sp = nullptr;
+    fv_bc* bcHeat = nullptr;     // fixedValue on every patch
+    fv_bc* bcPoisson = nullptr;  // fixedValue on patch 0, zeroGradient elsewhere
+    DBuf<int32_t> cellMap;       // device cell -> caller cell (renumbered meshes)
+    bool renumbered = false;
+    int32_t N = 0, F = 0, B = 0;
+    int64_t nnz = 0, sellTotal = 0;
+};
+
+struct Req {
+    int64_t id = 0;
+    fv_request d{};
+    std::vector<double> wall, T0;  // owned copies of the caller's arrays
+    int state = FV_REQ_QUEUED;
+    bool cancel = false;
+    int status = FV_OK;
+    Clock::time_point tSubmit, tStart, tDone;
+    size_t estimate = 0;
+    // device resources while running
+    fv_linsys* ls = nullptr;
+    DBuf<double> x, xold, Tb, gamma, scratch;
+    BMember* member = nullptr;
+    int step = 0, iters = 0;
+    double res = 0.0;
+    std::vector<double> result;
+};
+
+}  // namespace
+
+struct fv_server {
+    fv_ctx* ctx = nullptr;
+    fv_server_opts opts{};
+    std::vector<std::unique_ptr<MeshEntry>> meshes;
+    std::mutex qmu;               // queue, request table, statistics
+    std::condition_variable qcv;  // work arrived / request finished
+    std::mutex gpumu;             // users of the context's stream
+    std::deque<Req*> pending;
+    std::unordered_map<int64_t, Req*> reqs;
+    std::vector<Req*> active;
+    int64_t nextId = 1;
+    bool stop = false;
+    std::thread worker;
+    fv_batch* batch = nullptr;
+    // statistics
+    int64_t submitted = 0, completed = 0, failed = 0, rejected = 0, cancelled = 0;
+    std::vector<double> latencies;
+    bool haveFirst = false;
+    Clock::time_point tFirst, tLast;
+    size_t peakBytes = 0;
+    int peakActive = 0;
+};
+
+namespace {
+
+size_t estimate_bytes(const MeshEntry& me, int kind) {
+    const size_t N = me.N, F = me.F, B = me.B, nnz = me.nnz;
+    size_t b = 8 * nnz + 8 * N            // linear system
+               + 8 * N * 2 + 8 * B         // x, x_old, boundary values
+               + 8 * (F + B)               // face coefficient (kappa or 1)
+               + 4 * (size_t)me.sellTotal  // FP32 matrix copy (batched)
+               + 6 * 4 * N + 8 * N;        // Krylov vectors
+    if (kind == FV_REQ_POISSON) b += 24 * N + 24 * B + 8 * (F + B) + 8 * N;  // U, phi, div
+    return b + (1 << 16);
+}
+
+void free_request_device(Req* r) {
+    if (r->ls) {
+        fv_linsys_destroy(r->ls);
+        r->ls = nullptr;
+    }
+    r->x.release();
+    r->xold.release();
+    r->Tb.release();
+    r->gamma.release();
+    r->scratch.release();
+}
+
+void check(int rc) {
+    if (rc != FV_OK) throw Error{rc, fv_last_error()};
+}
+
+// Assemble the request's system for its current step.
+void assemble(fv_server* s, Req* r) {
+    MeshEntry& me = *s->meshes[r->d.mesh_id];
+    if (r->d.kind == FV_REQ_HEAT) {
+        // ddt(T) - laplacian(kappa, T), one fused pass
+        check(fv_assemble_ddt_laplacian(r->ls, r->xold.p, r->d.dt, r->gamma.p, me.bcHeat, r->Tb.p,
+                                        nullptr, -1.0));
+    } else {
+        check(fv_linsys_zero(r->ls));
+        check(fv_assemble_laplacian(r->ls, r->gamma.p, me.bcPoisson, r->Tb.p, nullptr, 1.0));
+        // explicit div(phi) computed into scratch (after U, phi)
+        double* E = r->scratch.p + 3 * (size_t)me.N + 3 * (size_t)me.B + (me.F + me.B);
+        check(fv_assemble_explicit(r->ls, E, -1.0));
+    }
+}
+
+// Build a request's fields and system on its mesh (server context stream).
+void setup(fv_server* s, Req* r) {
+    MeshEntry& me = *s->meshes[r->d.mesh_id];
+    fv_ctx* ctx = s->ctx;
+    const int32_t N = me.N, F = me.F, B = me.B;
+    check(fv_linsys_create(me.sp, &r->ls));
+    r->x.alloc(ctx, N);
+    r->xold.alloc(ctx, N);
+    r->Tb.alloc(ctx, B);
+    r->gamma.alloc(ctx, F + B);
+    if (r->d.kind == FV_REQ_HEAT) {
+        // initial field: caller's (mapped into the mesh numbering) or seeded
+        if (!r->T0.empty()) {
+            DBuf<double> tmp;
+            tmp.upload(ctx, r->T0.data(), N);
+            if (me.renumbered)
+                check(fv_permute(ctx, FV_F64, N, 1, 0, 0, me.cellMap.p, tmp.p, r->x.p));
+            else
+                check(fv_memcpy_d2d(ctx, r->x.p, tmp.p, N * sizeof(double)));
+            check(fv_sync(ctx));
+        } else {
+            check(fv_fill_uniform(ctx, FV_F64, r->x.p, N, 300.0, 400.0, r->d.seed, 0));
+        }
+        check(fv_memcpy_d2d(ctx, r->xold.p, r->x.p, N * sizeof(double)));
+        // wall values, patch by patch
+        std::vector<int32_t> poff(me.mesh->nPatches + 1);
+        check(fv_mesh_get(me.mesh, FV_G_PATCH_OFFSETS, poff.data()));
+        std::vector<double> tb(B, 350.0);
+        for (int p = 0; p < me.mesh->nPatches; ++p)
+            for (int b = poff[p]; b < poff[p + 1]; ++b)
+                tb[b] = r->wall.empty() ? 350.0 : r->wall[p];
+        if (B) check(fv_memcpy_h2d(ctx, r->Tb.p, tb.data(), B * sizeof(double)));
+        check(fv_fill(ctx, FV_F64, r->gamma.p, F + B, r->d.kappa));
+    } else {
+        check(fv_fill(ctx, FV_F64, r->x.p, N, 0.0));
+        check(fv_fill(ctx, FV_F64, r->xold.p, N, 0.0));
+        check(fv_fill(ctx, FV_F64, r->Tb.p, B, 0.0));
+        check(fv_fill(ctx, FV_F64, r->gamma.p, F + B, 1.0));
+        // velocity U ~ U[-1,1) in cells and on boundary faces -> phi = Sf.U_f -> div(phi)
+        r->scratch.alloc(ctx, 3 * (size_t)N + 3 * (size_t)B + (F + B) + N);
+        double* U = r->scratch.p;
+        double* Ub = U + 3 * (size_t)N;
+        double* phi = Ub + 3 * (size_t)B;
+        double* E = phi + (F + B);
+        check(fv_fill_uniform(ctx, FV_F64, U, 3 * (int64_t)N, -1.0, 1.0, r->d.seed, 1));
+        check(fv_fill_uniform(ctx, FV_F64, Ub, 3 * (int64_t)B, -1.0, 1.0, r->d.seed, 2));
+        check(fv_flux(ctx, me.mesh, FV_F64, U, Ub, phi));
+        check(fv_div_gauss(ctx, me.mesh, FV_LINEAR, FV_F64, 1.0, phi, nullptr, nullptr, E));
+    }
+    r->step = 0;
+    assemble(s, r);
+}
+
+void finish(fv_server* s, Req* r, int status) {
+    MeshEntry& me = *s->meshes[r->d.mesh_id];
+    fv_ctx* ctx = s->ctx;
+    if (status == FV_OK) {
+        r->result.resize(me.N);
+        if (me.renumbered) {
+            DBuf<double> tmp;
+            tmp.alloc(ctx, me.N);
+            check(fv_permute(ctx, FV_F64, me.N, 1, 1, 0, me.cellMap.p, r->x.p, tmp.p));
+            tmp.download(r->result.data());
+        } else {
+            r->x.download(r->result.data());
+        }
+    }
+    free_request_device(r);
+    std::lock_guard<std::mutex> lk(s->qmu);
+    r->status = status;
+    r->state = FV_REQ_DONE;
+    r->tDone = Clock::now();
+    s->tLast = r->tDone;
+    if (status == FV_OK) {
+        ++s->completed;
+        s->latencies.push_back(ms_between(r->tSubmit, r->tDone));
+    } else if (status == FV_ECANCELLED) {
+        ++s->cancelled;
+    } else {
+        ++s->failed;
+    }
+    s->qcv.notify_all();
+}
+
+// Admit queued requests that fit; returns them (already moved to RUNNING).
+std::vector<Req*> admit(fv_server* s, bool one_at_a_time) {
+    std::vector<Req*> out;
+    std::lock_guard<std::mutex> lk(s->qmu);
+    size_t inUse = s->ctx->in_use;
+    while (!s->pending.empty()) {
+        Req* r = s->pending.front();
+        if (r->cancel) {
+            s->pending.pop_front();
+            r->state = FV_REQ_DONE;
+            r->status = FV_ECANCELLED;
+            ++s->cancelled;
+            s->qcv.notify_all();
+            continue;
+        }
+        const int nActive = (int)s->active.size() + (int)out.size();
+        if (one_at_a_time && nActive >= 1) break;
+        if (s->opts.max_active > 0 && nActive >= s->opts.max_active) break;
+        if (s->opts.hbm_budget && inUse + r->estimate > s->opts.hbm_budget && nActive > 0) break;
+        s->pending.pop_front();
+        r->state = FV_REQ_RUNNING;
+        r->tStart = Clock::now();
+        inUse += r->estimate;
+        out.push_back(r);
+    }
+    return out;
+}
+
+void run_plain(fv_server* s, Req* r) {
+    fv_solve_opts o{};
+    o.rel_tol = r->d.rel_tol;
+    o.max_iter = r->d.max_iter;
+    o.mode = s->opts.mode;
+    o.inner_rel_tol = 1e-3;
+    o.check_every = s->opts.chunk > 0 ? s->opts.chunk : 32;
+    const int steps = r->d.kind == FV_REQ_HEAT ? std::max(1, r->d.steps) : 1;
+    for (int st = 0; st < steps; ++st) {
+        if (st > 0) {
+            check(fv_memcpy_d2d(s->ctx, r->xold.p, r->x.p, r->x.n * sizeof(double)));
+            assemble(s, r);
+        }
+        fv_solver_stats ss{};
+        int rc = fv_solve_cg(r->ls, r->x.p, &o, &ss);
+        r->iters += ss.iters;
+        r->res = ss.final_res;
+        if (rc != FV_OK) throw Error{rc, fv_last_error()};
+        if (r->cancel) throw Error{FV_ECANCELLED, "cancelled"};
+    }
+}
+
+void worker_loop(fv_server* s) {
+    const bool batched = s->opts.batched != 0;
+    std::vector<CgState> states;
+    while (true) {
+        {
+            std::unique_lock<std::mutex> lk(s->qmu);
+            s->qcv.wait(lk, [&] { return s->stop || !s->pending.empty() || !s->active.empty(); });
+            if (s->stop) return;
+        }
+        std::lock_guard<std::mutex> g(s->gpumu);
+        // admission
+        std::vector<Req*> adm = admit(s, !batched);
+        for (Req* r : adm) {
+            try {
+                setup(s, r);
+                if (batched) {
+                    r->member = batch_add(s->batch, r->ls, r->x.p, r->xold.p, r);
+                }
+                std::lock_guard<std::mutex> lk(s->qmu);
+                s->active.push_back(r);
+            } catch (const Error& e) {
+                set_error(e.msg);
+                finish(s, r, e.code);
+            }
+        }
+        {
+            std::lock_guard<std::mutex> lk(s->qmu);
+            s->peakActive = std::max(s->peakActive, (int)s->active.size());
+            s->peakBytes = std::max(s->peakBytes, s->ctx->in_use);
+        }
+        if (!batched) {
+            Req* r = nullptr;
+            {
+                std::lock_guard<std::mutex> lk(s->qmu);
+                if (!s->active.empty()) r = s->active.front();
+            }
+            if (!r) continue;
+            int status = FV_OK;
+            try {
+                run_plain(s, r);
+            } catch (const Error& e) {
+                set_error(e.msg);
+                status = e.code;
+            }
+            {
+                std::lock_guard<std::mutex> lk(s->qmu);
+                s->active.erase(s->active.begin());
+            }
+            finish(s, r, status);
+            continue;
+        }
+        if (s->batch->members.empty()) continue;
+        try {
+            batch_step(s->batch, states);
+        } catch (const Error& e) {
+            // a device error fails everything in flight
+            set_error(e.msg);
+            std::vector<Req*> act;
+            {
+                std::lock_guard<std::mutex> lk(s->qmu);
+                act.swap(s->active);
+            }
+            for (Req* r : act) {
+                if (r->member) batch_remove(s->batch, r->member);
+                r->member = nullptr;
+                finish(s, r, e.code);
+            }
+            continue;
+        }
+        // retire / advance members (states are in member order)
+        std::vector<BMember*> members = s->batch->members;
+        for (size_t i = 0; i < members.size(); ++i) {
+            BMember* m = members[i];
+            Req* r = (Req*)m->user;
+            const CgState& st = states[i];
+            const bool converged = st.done == 2;
+            const bool failed = !converged && (st.it >= st.maxit || !std::isfinite(st.res));
+            if (!converged && !failed && !r->cancel) continue;
+            r->iters += st.it;
+            r->res = st.res;
+            int status = FV_OK;
+            bool done = true;
+            if (r->cancel) {
+                status = FV_ECANCELLED;
+            } else if (failed) {
+                status = FV_ENOCONVERGE;
+            } else if (r->d.kind == FV_REQ_HEAT && r->step + 1 < std::max(1, r->d.steps)) {
+                // next Euler step: T_old <- T, re-assemble, restart this member
+                r->step += 1;
+                try {
+                    check(fv_memcpy_d2d(s->ctx, r->xold.p, r->x.p, r->x.n * sizeof(double)));
+                    assemble(s, r);
+                    batch_restart(s->batch, m);
+                    done = false;
+                } catch (const Error& e) {
+                    set_error(e.msg);
+                    status = e.code;
+                }
+            }
+            if (done) {
+                batch_remove(s->batch, m);
+                r->member = nullptr;
+                {
+                    std::lock_guard<std::mutex> lk(s->qmu);
+                    s->active.erase(std::find(s->active.begin(), s->active.end(), r));
+                }
+                try {
+                    finish(s, r, status);
+                } catch (const Error& e) {
+                    set_error(e.msg);
+                    finish(s, r, e.code);
+                }
+            }
+        }
+    }
+}
+
+MeshEntry& mesh_of(fv_server* s, int id) {
+    FV_REQUIRE(id >= 0 && id < (int)s->meshes.size(), "unknown mesh id");
+    return *s->meshes[id];
+}
+
+}  // namespace
+
+extern "C" {
+
+int fv_server_create(int device, const fv_server_opts* opts, fv_server** out) {
+    fv_server* s = nullptr;
+    int rc = guarded([&] {
+        FV_REQUIRE(opts && out, "NULL argument");
+        FV_REQUIRE(opts->mode == FV_SOLVE_F64 || opts->mode == FV_SOLVE_F32_IR, "unknown mode");
+        s = new fv_server();
+        s->opts = *opts;
+        check(fv_ctx_create(device, 0, &s->ctx));
+        const int tr = opts->tile_rows > 0 ? opts->tile_rows : 1024;
+        const int ch = opts->chunk > 0 ? opts->chunk : 16;
+        check(fv_batch_create(s->ctx, opts->mode, tr, ch, &s->batch));
+        s->batch->useGraph = opts->use_graph != 0;
+        s->worker = std::thread(worker_loop, s);
+        *out = s;
+    });
+    if (rc != FV_OK && s) {
+        if (s->batch) fv_batch_destroy(s->batch);
+        if (s->ctx) fv_ctx_destroy(s->ctx);
+        delete s;
+    }
+    return rc;
+}
+
+int fv_server_add_mesh(fv_server* s, const fv_polymesh_desc* d, int renumber, int* mesh_id) {
+    return guarded([&] {
+        FV_REQUIRE(s && d && mesh_id, "NULL argument");
+        auto me = std::make_unique<MeshEntry>();
+        std::lock_guard<std::mutex> g(s->gpumu);
+        if (renumber) {
+            const int64_t nF = d->n_faces, N = d->n_cells;
+            std::vector<int32_t> foff(nF + 1), fpts(d->h_face_offsets[nF]), own(nF),
+                nbr(d->n_internal_faces), cmap(N), fmap(nF);
+            check(fv_polymesh_renumber(d, 1, foff.data(), fpts.data(), own.data(), nbr.data(),
+                                       cmap.data(), fmap.data()));
+            fv_polymesh_desc r = *d;
+            r.h_face_offsets = foff.data();
+            r.h_face_points = fpts.data();
+            r.h_owner = own.data();
+            r.h_neighbour = nbr.data();
+            check(fv_mesh_create_poly(s->ctx, &r, &me->mesh));
+            me->cellMap.upload(s->ctx, cmap.data(), N);
+            check(fv_sync(s->ctx));
+            me->renumbered = true;
+        } else {
+            check(fv_mesh_create_poly(s->ctx, d, &me->mesh));
+        }
+        check(fv_sparsity_create(me->mesh, &me->sp));
+        const int np = me->mesh->nPatches;
+        std::vector<int32_t> kh(np, FV_BC_FIXED_VALUE), kp(np, FV_BC_FIXED_GRADIENT);
+        if (np) kp[0] = FV_BC_FIXED_VALUE;
+        check(fv_bc_create(me->mesh, kh.data(), &me->bcHeat));
+        check(fv_bc_create(me->mesh, kp.data(), &me->bcPoisson));
+        me->N = me->mesh->nCells;
+        me->F = me->mesh->nInt;
+        me->B = me->mesh->nB;
+        me->nnz = me->sp->nnz;
+        me->sellTotal = me->sp->sellTotal;
+        std::lock_guard<std::mutex> lk(s->qmu);
+        *mesh_id = (int)s->meshes.size();
+        s->meshes.push_back(std::move(me));
+    });
+}
+
+int fv_server_mesh_cells(fv_server* s, int mesh_id, int32_t* n) {
+    return guarded([&] {
+        FV_REQUIRE(s && n, "NULL argument");
+        std::lock_guard<std::mutex> lk(s->qmu);
+        *n = mesh_of(s, mesh_id).N;
+    });
+}
+
+int fv_server_submit(fv_server* s, const fv_request* q, int64_t* id) {
+    return guarded([&] {
+        FV_REQUIRE(s && q && id, "NULL argument");
+        FV_REQUIRE(q->kind == FV_REQ_HEAT || q->kind == FV_REQ_POISSON, "unknown request kind");
+        FV_REQUIRE(q->rel_tol > 0.0 && q->max_iter > 0, "bad solver settings");
+        FV_REQUIRE(q->kind != FV_REQ_HEAT || (q->dt > 0.0 && q->steps >= 1), "bad time step");
+        auto* r = new Req();
+        r->d = *q;
+        std::unique_lock<std::mutex> lk(s->qmu);
+        MeshEntry& me = mesh_of(s, q->mesh_id);
+        if (q->h_wall_values) r->wall.assign(q->h_wall_values, q->h_wall_values + me.mesh->nPatches);
+        if (q->h_T0) r->T0.assign(q->h_T0, q->h_T0 + me.N);
+        r->d.h_wall_values = nullptr;
+        r->d.h_T0 = nullptr;
+        r->estimate = estimate_bytes(me, q->kind);
+        r->tSubmit = Clock::now();
+        r->id = s->nextId++;
+        if (!s->haveFirst) {
+            s->haveFirst = true;
+            s->tFirst = r->tSubmit;
+        }
+        ++s->submitted;
+        if (s->opts.hbm_budget && r->estimate > s->opts.hbm_budget) {
+            ++s->rejected;
+            delete r;
+            throw Error{FV_ENOMEM_BUDGET, "request needs more device memory than the server budget"};
+        }
+        s->reqs[r->id] = r;
+        s->pending.push_back(r);
+        *id = r->id;
+        s->qcv.notify_all();
+    });
+}
+
+int fv_server_poll(fv_server* s, int64_t id, int* state) {
+    return guarded([&] {
+        FV_REQUIRE(s && state, "NULL argument");
+        std::lock_guard<std::mutex> lk(s->qmu);
+        auto it = s->reqs.find(id);
+        FV_REQUIRE(it != s->reqs.end(), "unknown request id");
+        *state = it->second->state;
+    });
+}
+
+int fv_server_wait(fv_server* s, int64_t id, double* h_result, fv_request_stats* stats) {
+    int status = FV_OK;
+    int rc = guarded([&] {
+        FV_REQUIRE(s, "NULL server");
+        std::unique_lock<std::mutex> lk(s->qmu);
+        auto it = s->reqs.find(id);
+        FV_REQUIRE(it != s->reqs.end(), "unknown request id");
+        Req* r = it->second;
+        s->qcv.wait(lk, [&] { return r->state == FV_REQ_DONE; });
+        if (stats) {
+            stats->status = r->status;
+            stats->iters = r->iters;
+            stats->final_res = r->res;
+            stats->queue_ms = r->tStart.time_since_epoch().count() ? ms_between(r->tSubmit, r->tStart) : 0.0;
+            stats->latency_ms = ms_between(r->tSubmit, r->tDone);
+        }
+        if (h_result && r->status == FV_OK) std::copy(r->result.begin(), r->result.end(), h_result);
+        status = r->status;
+        s->reqs.erase(it);
+        delete r;
+    });
+    if (rc != FV_OK) return rc;
+    if (status != FV_OK) set_error("request finished with status " + std::to_string(status));
+    return status;
+}
+
+int fv_server_cancel(fv_server* s, int64_t id) {
+    return guarded([&] {
+        FV_REQUIRE(s, "NULL server");
+        std::lock_guard<std::mutex> lk(s->qmu);
+        auto it = s->reqs.find(id);
+        FV_REQUIRE(it != s->reqs.end(), "unknown request id");
+        it->second->cancel = true;
+        s->qcv.notify_all();
+    });
+}
+
+int fv_server_get_stats(fv_server* s, fv_server_stats* o) {
+    return guarded([&] {
+        FV_REQUIRE(s && o, "NULL argument");
+        std::lock_guard<std::mutex> lk(s->qmu);
+        *o = fv_server_stats{};
+        o->submitted = s->submitted;
+        o->completed = s->completed;
+        o->failed = s->failed;
+        o->rejected = s->rejected;
+        o->cancelled = s->cancelled;
+        o->peak_bytes = s->peakBytes;
+        o->peak_active = s->peakActive;
+        if (s->haveFirst && s->completed) o->wall_s = ms_between(s->tFirst, s->tLast) * 1e-3;
+        if (o->wall_s > 0) o->solves_per_s = s->completed / o->wall_s;
+        if (!s->latencies.empty()) {
+            std::vector<double> l = s->latencies;
+            std::sort(l.begin(), l.end());
+            auto q = [&](double p) {
+                const double pos = p * (l.size() - 1);
+                const size_t i = (size_t)pos;
+                const double f = pos - i;
+                return i + 1 < l.size() ? l[i] * (1 - f) + l[i + 1] * f : l[i];
+            };
+            o->p50_ms = q(0.50);
+            o->p99_ms = q(0.99);
+            double sum = 0;
+            for (double v : l) sum += v;
+            o->mean_ms = sum / l.size();
+            o->max_ms = l.back();
+        }
+    });
+}
+
+int fv_server_destroy(fv_server* s) {
+    return guarded([&] {
+        if (!s) return;
+        {
+            std::lock_guard<std::mutex> lk(s->qmu);
+            s->stop = true;
+            s->qcv.notify_all();
+        }
+        if (s->worker.joinable()) s->worker.join();
+        for (Req* r : s->active) free_request_device(r);
+        for (auto& kv : s->reqs) delete kv.second;
+        if (s->batch) fv_batch_destroy(s->batch);
+        for (auto& me : s->meshes) {
+            if (me->bcHeat) fv_bc_destroy(me->bcHeat);
+            if (me->bcPoisson) fv_bc_destroy(me->bcPoisson);
+            if (me->sp) fv_sparsity_destroy(me->sp);
+            me->cellMap.release();
+            if (me->mesh) fv_mesh_destroy(me->mesh);
+        }
+        fv_ctx_destroy(s->ctx);
+        delete s;
+    });
+}
+
+}  // extern "C"

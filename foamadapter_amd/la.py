@@ -186,6 +186,14 @@ class Batch:
     def set_shape(self, tile_rows: int, chunk: int):
         call("fv_batch_set_shape", self.handle, int(tile_rows), int(chunk))
 
+    def autotune(self, solver: dict | None = None, cache_path: str | None = None) -> tuple[int, int]:
+        """Pick tile rows and chunk length by timing solves (cached per size bucket)."""
+        o = solver_opts(solver)
+        t, c = C.c_int(), C.c_int()
+        call("fv_batch_autotune", self.handle, C.byref(o),
+             cache_path.encode() if cache_path else None, C.byref(t), C.byref(c))
+        return t.value, c.value
+
     def solve(self, solver: dict | None = None, use_graph: bool = True,
               raise_on_noconverge: bool = True) -> list[SolverStats]:
         o = solver_opts(solver)

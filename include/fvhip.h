@@ -299,6 +299,13 @@ int fv_assemble_explicit(fv_linsys* ls, const void* d_E, double coeff);
  * diagonal minus their sum, boundary faces folded into diag/rhs. */
 int fv_assemble_laplacian(fv_linsys* ls, const void* d_gamma, const fv_bc* bc, const void* d_Tb,
                           const void* d_Tgrad, double coeff);
+/* The heat-transfer system dsl::Expression(imp::ddt(T) + coeff*imp::laplacian(gamma, T))
+ * (heatTransfer.cpp:67, coeff = -1) assembled in one pass, OVERWRITING values and
+ * rhs: bit-identical to fv_linsys_zero + fv_assemble_ddt_euler(.., 1) +
+ * fv_assemble_laplacian(.., coeff). */
+int fv_assemble_ddt_laplacian(fv_linsys* ls, const void* d_Told, double dt, const void* d_gamma,
+                              const fv_bc* bc, const void* d_Tb, const void* d_Tgrad,
+                              double coeff);
 /* DivOperator(Implicit, phi, T, "Gauss <scheme>").implicitOperation(ls)
  * (test_implicitOperators.cpp:188-195): lower = -w phi, upper = lower + phi. */
 int fv_assemble_div(fv_linsys* ls, fv_scheme scheme, const void* d_phi, const fv_bc* bc,
@@ -356,6 +363,77 @@ int fv_batch_set_shape(fv_batch* b, int tile_rows, int chunk);
  * opts->inner_rel_tol: FP32 residual drop that ends a chunk early (mixed). */
 int fv_batch_solve(fv_batch* b, const fv_solve_opts* opts, int use_graph,
                    fv_solver_stats* stats);
+/* Autotuner: times full batch solves for tile_rows in {256..4096} x chunk in
+ * {8,16,32} from the batch's current x (restored after every trial), applies
+ * the fastest shape and records it in the JSON file cache_path (may be NULL)
+ * under a key built from the precision, the member count and the row counts;
+ * a later call with a matching key applies the cached shape without trials. */
+int fv_batch_autotune(fv_batch* b, const fv_solve_opts* opts, const char* cache_path,
+                      int* tile_rows, int* chunk);
+
+/* ---- serving: a resident request loop for implicit solves ----------------
+ * No counterpart in the reference (its solves run inside an application's time
+ * loop).  The server owns a context (one HIP stream) on `device` and a worker
+ * thread.  Meshes are added once and stay resident; requests (heat transfer or
+ * pressure Poisson on a resident mesh) are admitted while their estimated
+ * device memory fits the HBM budget, and run either by continuous batching on
+ * the batched CG engine (systems join and leave between chunks) or one at a
+ * time through fv_solve_cg (the plain drop-in path).  Results are returned in
+ * the caller's cell numbering. */
+typedef struct fv_server fv_server;
+enum { FV_REQ_HEAT = 0, FV_REQ_POISSON = 1 };
+enum { FV_REQ_QUEUED = 0, FV_REQ_RUNNING = 1, FV_REQ_DONE = 2 };
+typedef struct {
+    size_t hbm_budget;   /* admission budget in bytes (0: unlimited) */
+    int32_t mode;        /* FV_SOLVE_F32_IR or FV_SOLVE_F64 */
+    int32_t batched;     /* 1: continuous batching, 0: one request at a time via fv_solve_cg */
+    int32_t max_active;  /* systems in flight (batched), 0: no limit */
+    int32_t tile_rows;   /* batch tile rows (multiple of 64), 0: 1024 */
+    int32_t chunk;       /* CG iterations per FP64 residual replacement, 0: 16 */
+    int32_t use_graph;   /* replay chunks as hipGraphs */
+} fv_server_opts;
+typedef struct {
+    int32_t mesh_id;
+    int32_t kind;        /* FV_REQ_HEAT: Euler steps of ddt(T) - laplacian(kappa,T), all walls
+                            fixedValue; FV_REQ_POISSON: laplacian(1,p) = div(Sf.U_f), p = 0 on
+                            patch 0, zeroGradient elsewhere */
+    int32_t steps;       /* heat: Euler steps */
+    int32_t max_iter;    /* CG iterations per solve */
+    double dt, kappa;    /* heat */
+    double rel_tol;      /* FP64 relative residual per solve */
+    const double* h_wall_values; /* heat: one value per patch, or NULL (350) */
+    const double* h_T0;  /* heat: initial field [N] in the caller's numbering, or NULL */
+    uint64_t seed;       /* synthetic inputs: T0 ~ U[300,400) (heat), U ~ U[-1,1) (Poisson) */
+} fv_request;
+typedef struct {
+    int32_t status;      /* FV_OK, FV_ENOCONVERGE, FV_ECANCELLED, FV_ENOMEM_BUDGET, FV_EDEVICE */
+    int32_t iters;       /* CG iterations summed over the request's solves */
+    double final_res;    /* FP64 relative residual of the last solve */
+    double queue_ms;     /* submit -> admission */
+    double latency_ms;   /* submit -> result */
+} fv_request_stats;
+typedef struct {
+    int64_t submitted, completed, failed, rejected, cancelled;
+    double wall_s;       /* first submission -> last completion */
+    double solves_per_s; /* completed requests per second of wall_s */
+    double p50_ms, p99_ms, mean_ms, max_ms;
+    size_t peak_bytes;   /* peak device bytes in use */
+    int32_t peak_active; /* peak requests in flight */
+} fv_server_stats;
+int fv_server_create(int device, const fv_server_opts* opts, fv_server** out);
+/* Adds a resident mesh (renumber != 0: Z-order renumbered internally). */
+int fv_server_add_mesh(fv_server* s, const fv_polymesh_desc* mesh, int renumber, int* mesh_id);
+int fv_server_mesh_cells(fv_server* s, int mesh_id, int32_t* n_cells);
+/* Queues a request; FV_ENOMEM_BUDGET if it can never fit the budget. */
+int fv_server_submit(fv_server* s, const fv_request* req, int64_t* req_id);
+int fv_server_poll(fv_server* s, int64_t req_id, int* state);
+/* Waits for a request, copies its result field [N] (caller numbering; may be
+ * NULL) and statistics, and releases it.  Returns the request's status. */
+int fv_server_wait(fv_server* s, int64_t req_id, double* h_result, fv_request_stats* stats);
+/* Cancels a queued or running request (its wait returns FV_ECANCELLED). */
+int fv_server_cancel(fv_server* s, int64_t req_id);
+int fv_server_get_stats(fv_server* s, fv_server_stats* out);
+int fv_server_destroy(fv_server* s);
 
 #ifdef __cplusplus
 }

@@ -99,3 +99,46 @@ def test_iteration_limit_reported(ctx, systems):
         b.solve({"relTol": 1e-14, "maxIter": 3})
     assert e.value.code == _lib.FV_ENOCONVERGE
     b.close()
+
+
+def test_fused_heat_assembly_bitwise(ctx, systems):
+    """fv_assemble_ddt_laplacian == zero + ddt + laplacian, bit for bit."""
+    from foamadapter_amd import _lib, la
+    s = systems[3]
+    m, T = s["mesh"], s["T"]
+    F, B = m.n_internal_faces, m.n_boundary_faces
+    kap = ctx.upload(np.full(F + B, 3e-2))
+    a = la.LinearSystem(la.SparsityPattern.readOrCreate(m))
+    b = la.LinearSystem(la.SparsityPattern.readOrCreate(m))
+    b.zero()
+    _lib.call("fv_assemble_ddt_euler", b.handle, T.internal.ptr, 1e-3, 1.0)
+    _lib.call("fv_assemble_laplacian", b.handle, kap.ptr, T.bcs.handle, T.boundary.ptr, None, -1.0)
+    _lib.call("fv_assemble_ddt_laplacian", a.handle, T.internal.ptr, 1e-3, kap.ptr, T.bcs.handle,
+              T.boundary.ptr, None, -1.0)
+    assert np.array_equal(a.values(), b.values()) and np.array_equal(a.rhs(), b.rhs())
+
+
+def test_autotune_picks_and_caches(ctx, systems, tmp_path):
+    from foamadapter_amd import la
+    b = la.Batch(ctx, "mixed", 1024, 16)
+    xs = [ctx.upload(s["x0"]) for s in systems]
+    for s, x in zip(systems, xs):
+        b.add(s["ls"], x)
+    before = [x.numpy() for x in xs]
+    cache = str(tmp_path / "tune.json")
+    t, c = b.autotune({"relTol": 1e-8, "maxIter": 2000}, cache)
+    assert t in (256, 512, 1024, 2048, 4096) and c in (8, 16, 32)
+    for x, x0 in zip(xs, before):          # trials restore the initial guess
+        assert np.array_equal(x.numpy(), x0)
+    import json
+    entries = json.load(open(cache))
+    assert len(entries) == 1 and list(entries.values())[0]["tile_rows"] == t
+    # a second batch of the same shape takes the cached shape
+    b2 = la.Batch(ctx, "mixed", 1024, 16)
+    for s in systems:
+        b2.add(s["ls"], ctx.upload(s["x0"]))
+    assert b2.autotune({"relTol": 1e-8, "maxIter": 2000}, cache) == (t, c)
+    stats = b.solve({"relTol": 1e-8, "maxIter": 2000})
+    assert all(st.finalResNorm <= 1e-8 for st in stats)
+    b.close()
+    b2.close()
