@@ -11,8 +11,8 @@ sections).  Throughput counts algorithmic bytes: every API-level input array
 once at its stored precision plus every output once (per internal face / cell
 / boundary face coefficients in BYTES).
 
-The device mesh is renumbered at import (reverse Cuthill-McKee, as OpenFOAM's
-renumberMesh); the caller's fields are mapped into that numbering once when
+The device mesh is renumbered at import (Z-order of cell centres, the role of
+OpenFOAM's renumberMesh); the caller's fields are mapped into that numbering once when
 they are uploaded (cost reported as boundary_mapping).  The same sweep on the
 numbering as given is reported beside it (as_given), and the results of the
 two are compared.
@@ -552,7 +552,7 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"explicit operator sweep: {args.n}^3 hex cells, interior vertices "
                                f"jittered +-0.2h, cell numbering randomly permuted (seed "
-                               f"{args.seed}); mesh renumbered at import (reverse Cuthill-McKee), "
+                               f"{args.seed}); mesh renumbered at import (Z-order of cell centres), "
                                f"fields mapped once at upload",
                    "cells": N, "internal_faces": F, "boundary_faces": B, "ops": OPS,
                    "precisions": ["f64", "f32"]},

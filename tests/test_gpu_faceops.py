@@ -269,11 +269,12 @@ def test_renumbered_mesh_roundtrip(ctx):
     # bitwise against the oracle on the renumbered mesh itself
     cm = m.cell_map.numpy()
     assert np.array_equal(fvcc.GaussGreenGrad(ctx, m).grad(Tf).numpy(), orc.grad(g_ren, T[cm], Tb))
-    # the renumbering is a permutation and shrinks the bandwidth
+    # the renumbering is a permutation and brings face neighbours close in index
+    # (Z-order bounds the mean owner-neighbour distance, not the maximum)
     assert sorted(cm) == list(range(N))
-    bw0 = np.max(np.abs(g["owner"] - g["neighbour"]))
-    bw1 = np.max(np.abs(g_ren["owner"] - g_ren["neighbour"]))
-    assert bw1 < bw0 / 4
+    d0 = np.mean(np.abs(g["owner"] - g["neighbour"]))
+    d1 = np.mean(np.abs(g_ren["owner"] - g_ren["neighbour"]))
+    assert d1 < d0 / 4
 
 
 @pytest.mark.parametrize("dt", [np.float64, np.float32])
