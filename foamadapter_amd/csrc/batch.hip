@@ -1,14 +1,14 @@
 // batch.hip -- many independent implicit solves in one launch sequence.
 //
 // A batch holds assembled linear systems (fv_linsys, any sizes) and their
-// solution vectors.  Their rows are cut into tiles of a fixed number of rows
-// (a multiple of 64, never straddling two systems); every kernel of the CG
-// iteration runs over the tile list of the whole batch, one workgroup per tile,
-// so small and large systems share the machine and one launch serves all of
-// them.  Dot products are reduced per system: each tile writes its partials to
-// its own slot and the last tile of a system (integer ticket in that system's
-// state) sums the system's slots in tile order -- so a system's results depend
-// only on its own tiling, bit for bit the same alone or in any mix.
+// solution vectors.  Their rows are cut into tiles (a multiple of 64 rows, never
+// straddling two systems, at most kCgMaxTiles per system); every kernel of the
+// CG iteration (cgkernels.hpp) runs over the tile list of the whole batch, one
+// workgroup per tile, so small and large systems share the machine and one
+// launch serves all of them.  Dot products are reduced per system: each tile
+// writes its partials to its own slot and every consuming tile sums its
+// system's slots in one fixed order -- so a system's results depend only on its
+// own tiling, bit for bit the same alone or in any mix.
 //
 // Precision: the CG runs in S (float: mixed, double: FP64) on a sliced-ELL copy
 // of the matrix; every chunk of iterations ends with an FP64 residual
@@ -19,6 +19,7 @@
 // stopped.  Members can join and leave between chunks (continuous batching):
 // joining members are started by their own tiles, the others carry on.
 #include "batch.hpp"
+#include "cgkernels.hpp"
 
 #include <algorithm>
 #include <cmath>
@@ -31,215 +32,6 @@ using namespace fvhip;
 namespace {
 
 constexpr int kB = kCgBlock;
-
-template <typename S>
-struct BProb {
-    int32_t N, tile0, ntiles, pad;
-    const int64_t* sellOff;
-    const int32_t* sellCol;
-    const S* sval;
-    const int32_t* rowOffs;
-    const int32_t* colIdxs;
-    const double* vals;
-    const double* rhs;
-    double* x;
-    double* xold;  // optional copy target after convergence (Euler T_old), may be null
-    S *d, *r, *z, *p, *q, *dinv;
-    double* r64;
-    CgState* st;
-};
-
-struct BTile {
-    int32_t prob, row0, row1, slot;
-};
-
-template <typename S>
-__global__ __launch_bounds__(kB) void kb_spmv(const BTile* __restrict__ tiles, int nTiles,
-                                              const BProb<S>* __restrict__ probs,
-                                              double* __restrict__ partials) {
-    __shared__ double sh[kB / 64];
-    if ((int)blockIdx.x >= nTiles) return;
-    const BTile t = tiles[blockIdx.x];
-    const BProb<S>& P = probs[t.prob];
-    CgState* st = P.st;
-    if (st->done) return;
-    double pq = 0.0;
-    for (int r = t.row0 + threadIdx.x; r < t.row1; r += kB) {
-        const int sl = r >> 6, l = r & 63;
-        const int64_t b = P.sellOff[sl];
-        const int w = (int)((P.sellOff[sl + 1] - b) >> 6);
-        S s = 0;
-        for (int k = 0; k < w; ++k) {
-            const int64_t pos = b + 64 * (int64_t)k + l;
-            s += P.sval[pos] * P.p[P.sellCol[pos]];
-        }
-        if (r < P.N) {
-            P.q[r] = s;
-            pq += (double)P.p[r] * (double)s;
-        }
-    }
-    double v[1] = {pq}, tot[1];
-    if (publish_slots<1>(v, partials, st, sh, P.tile0, P.ntiles, t.slot, tot) && threadIdx.x == 0)
-        st->alpha = st->rz[st->it & 1] / tot[0];
-}
-
-template <typename S>
-__global__ __launch_bounds__(kB) void kb_update(const BTile* __restrict__ tiles, int nTiles,
-                                                const BProb<S>* __restrict__ probs,
-                                                double* __restrict__ partials) {
-    __shared__ double sh[kB / 64];
-    if ((int)blockIdx.x >= nTiles) return;
-    const BTile t = tiles[blockIdx.x];
-    const BProb<S>& P = probs[t.prob];
-    CgState* st = P.st;
-    if (st->done) return;
-    const S a = (S)st->alpha;
-    double rr = 0.0, rz = 0.0;
-    const int r1 = min(t.row1, P.N);
-    for (int i = t.row0 + threadIdx.x; i < r1; i += kB) {
-        P.d[i] += a * P.p[i];
-        const S ri = P.r[i] - a * P.q[i];
-        P.r[i] = ri;
-        const S zi = P.dinv[i] * ri;
-        P.z[i] = zi;
-        rr += (double)ri * (double)ri;
-        rz += (double)ri * (double)zi;
-    }
-    double v[2] = {rr, rz}, tot[2];
-    if (publish_slots<2>(v, partials, st, sh, P.tile0, P.ntiles, t.slot, tot) && threadIdx.x == 0) {
-        const int it = st->it + 1;
-        st->it = it;
-        st->rz[it & 1] = tot[1];
-        const double rn = sqrt(tot[0]);
-        const double est = rn / st->bnorm;
-        if (est <= st->tol || it >= st->maxit || !(est == est) ||
-            (st->delta > 0.0 && rn < st->delta * st->rrep))
-            st->done = 1;
-    }
-}
-
-template <typename S>
-__global__ __launch_bounds__(kB) void kb_direction(const BTile* __restrict__ tiles, int nTiles,
-                                                   const BProb<S>* __restrict__ probs) {
-    if ((int)blockIdx.x >= nTiles) return;
-    const BTile t = tiles[blockIdx.x];
-    const BProb<S>& P = probs[t.prob];
-    const CgState* st = P.st;
-    if (st->done) return;
-    const int it = st->it;
-    const S b = (S)(st->rz[it & 1] / st->rz[(it - 1) & 1]);
-    const int r1 = min(t.row1, P.N);
-    for (int i = t.row0 + threadIdx.x; i < r1; i += kB) P.p[i] = P.z[i] + b * P.p[i];
-}
-
-// x += d, d = 0 (FP64 accumulation of the chunk's correction)
-template <typename S>
-__global__ __launch_bounds__(kB) void kb_accum(const BTile* __restrict__ tiles, int nTiles,
-                                               const BProb<S>* __restrict__ probs) {
-    if ((int)blockIdx.x >= nTiles) return;
-    const BTile t = tiles[blockIdx.x];
-    const BProb<S>& P = probs[t.prob];
-    if (P.st->done == 2) return;
-    const int r1 = min(t.row1, P.N);
-    for (int i = t.row0 + threadIdx.x; i < r1; i += kB) {
-        P.x[i] += (double)P.d[i];
-        P.d[i] = S(0);
-    }
-}
-
-// r64 = b - A x in FP64 (CSR rows in column order); start: also ||b||
-template <typename S>
-__global__ __launch_bounds__(kB) void kb_resid(const BTile* __restrict__ tiles, int nTiles,
-                                               const BProb<S>* __restrict__ probs,
-                                               double* __restrict__ partials, int start) {
-    __shared__ double sh[kB / 64];
-    if ((int)blockIdx.x >= nTiles) return;
-    const BTile t = tiles[blockIdx.x];
-    const BProb<S>& P = probs[t.prob];
-    CgState* st = P.st;
-    if (!start && st->done == 2) return;
-    double rr = 0.0, bb = 0.0;
-    const int r1 = min(t.row1, P.N);
-    for (int i = t.row0 + threadIdx.x; i < r1; i += kB) {
-        double s = 0.0;
-        const int e = P.rowOffs[i + 1];
-        for (int k = P.rowOffs[i]; k < e; ++k) s += P.vals[k] * P.x[P.colIdxs[k]];
-        const double ri = P.rhs[i] - s;
-        P.r64[i] = ri;
-        rr += ri * ri;
-        bb += P.rhs[i] * P.rhs[i];
-    }
-    double v[2] = {rr, bb}, tot[2];
-    if (publish_slots<2>(v, partials, st, sh, P.tile0, P.ntiles, t.slot, tot) && threadIdx.x == 0) {
-        if (start) {
-            const double bn = sqrt(tot[1]);
-            st->bnorm = bn > 0.0 ? bn : 1.0;
-        }
-        st->res = sqrt(tot[0]) / st->bnorm;
-        if (start) st->res0 = st->res;
-    }
-}
-
-// r = (S) r64; z = D^-1 r; start: p = z; sets rz[it&1], rrep, done (2: converged)
-template <typename S>
-__global__ __launch_bounds__(kB) void kb_replace(const BTile* __restrict__ tiles, int nTiles,
-                                                 const BProb<S>* __restrict__ probs,
-                                                 double* __restrict__ partials, int start) {
-    __shared__ double sh[kB / 64];
-    if ((int)blockIdx.x >= nTiles) return;
-    const BTile t = tiles[blockIdx.x];
-    const BProb<S>& P = probs[t.prob];
-    CgState* st = P.st;
-    if (!start && st->done == 2) return;
-    double rr = 0.0, rz = 0.0;
-    const int r1 = min(t.row1, P.N);
-    for (int i = t.row0 + threadIdx.x; i < r1; i += kB) {
-        const S ri = (S)P.r64[i];
-        const S zi = P.dinv[i] * ri;
-        P.r[i] = ri;
-        P.z[i] = zi;
-        if (start) P.p[i] = zi;
-        rr += (double)ri * (double)ri;
-        rz += (double)ri * (double)zi;
-    }
-    double v[2] = {rr, rz}, tot[2];
-    if (publish_slots<2>(v, partials, st, sh, P.tile0, P.ntiles, t.slot, tot) && threadIdx.x == 0) {
-        st->rrep = sqrt(tot[0]);
-        st->rz[st->it & 1] = tot[1];
-        const bool conv = st->res <= st->tol;
-        st->done = conv ? 2 : 0;
-        if (!conv && st->it >= st->maxit) st->done = 1;
-    }
-}
-
-template <typename S>
-__global__ void kb_sell_values(int64_t n, const int32_t* __restrict__ src,
-                               const double* __restrict__ values, S* __restrict__ out) {
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-         i += (int64_t)gridDim.x * blockDim.x) {
-        const int32_t j = src[i];
-        out[i] = j >= 0 ? (S)values[j] : (S)0;
-    }
-}
-
-template <typename S>
-__global__ void kb_dinv(int N, const int32_t* __restrict__ rowOffs,
-                        const int32_t* __restrict__ diag, const double* __restrict__ values,
-                        S* __restrict__ dinv) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < N) dinv[i] = (S)(1.0 / values[rowOffs[i] + diag[i]]);
-}
-
-__global__ void kb_init_states(CgState* st, int n, double tol, int maxit, double delta) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) {
-        CgState s{};
-        s.tol = tol;
-        s.maxit = maxit;
-        s.delta = delta;
-        st[i] = s;
-    }
-}
 
 __global__ void kb_zero(int64_t n, uint8_t* __restrict__ p) {
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
@@ -267,27 +59,47 @@ void drop_graph(fv_batch* b) {
     b->graph = nullptr;
 }
 
+// rows per tile of a member and its tile count
+int member_rows(const fv_batch* b, const BMember* m) { return cg_tile_rows(m->N, b->tileRows); }
+int member_tiles(const fv_batch* b, const BMember* m) {
+    const int R = member_rows(b, m);
+    return (m->N + R - 1) / R;
+}
+
+template <typename S>
+void member_tile_list(const fv_batch* b, int i, int slot0, std::vector<CgTile>& out) {
+    const BMember* m = b->members[i];
+    const int R = member_rows(b, m), npad = ((m->N + 63) / 64) * 64;
+    for (int r0 = 0, k = 0; r0 < m->N; r0 += R, ++k) {
+        CgTile t;
+        t.prob = i;
+        t.row0 = r0;
+        t.row1 = std::min(r0 + R, npad);
+        t.slot = slot0 + k;
+        out.push_back(t);
+    }
+}
+
+template <typename S>
+CgPartials partials_of(fv_batch* b) {
+    const int64_t n = std::max(1, b->nTiles);
+    return CgPartials{b->partials.p, b->partials.p + n, b->partials.p + 3 * n};
+}
+
 template <typename S>
 void build_tables(fv_batch* b) {
     fv_ctx* ctx = b->ctx;
     const int P = (int)b->members.size();
-    std::vector<BProb<S>> hp(P);
-    std::vector<BTile> ht;
+    std::vector<CgSys<S>> hp(P);
+    std::vector<CgTile> ht;
     for (int i = 0; i < P; ++i) {
         BMember* m = b->members[i];
         const fv_sparsity* sp = m->ls->sp;
         const int N = m->N;
-        BProb<S>& p = hp[i];
+        CgSys<S>& p = hp[i];
         p.N = N;
         p.tile0 = (int)ht.size();
-        for (int r0 = 0; r0 < N; r0 += b->tileRows) {
-            BTile t;
-            t.prob = i;
-            t.row0 = r0;
-            t.row1 = std::min(r0 + b->tileRows, ((N + 63) / 64) * 64);
-            t.slot = (int)ht.size();
-            ht.push_back(t);
-        }
+        member_tile_list<S>(b, i, p.tile0, ht);
         p.ntiles = (int)ht.size() - p.tile0;
         p.sellOff = sp->sellOff.p;
         p.sellCol = sp->sellCol.p;
@@ -309,15 +121,15 @@ void build_tables(fv_batch* b) {
         p.st = b->states.p + m->slot;
     }
     b->nTiles = (int)ht.size();
-    b->probs.alloc(ctx, std::max<size_t>(1, P * sizeof(BProb<S>)));
+    b->probs.alloc(ctx, std::max<size_t>(1, P * sizeof(CgSys<S>)));
     if (P)
-        FV_HIP(hipMemcpyAsync(b->probs.p, hp.data(), P * sizeof(BProb<S>), hipMemcpyHostToDevice,
+        FV_HIP(hipMemcpyAsync(b->probs.p, hp.data(), P * sizeof(CgSys<S>), hipMemcpyHostToDevice,
                               ctx->stream));
-    b->tiles.alloc(ctx, std::max<size_t>(1, ht.size() * sizeof(BTile)));
+    b->tiles.alloc(ctx, std::max<size_t>(1, ht.size() * sizeof(CgTile)));
     if (!ht.empty())
-        FV_HIP(hipMemcpyAsync(b->tiles.p, ht.data(), ht.size() * sizeof(BTile),
+        FV_HIP(hipMemcpyAsync(b->tiles.p, ht.data(), ht.size() * sizeof(CgTile),
                               hipMemcpyHostToDevice, ctx->stream));
-    b->partials.alloc(ctx, 2 * std::max<size_t>(1, ht.size()));
+    b->partials.alloc(ctx, 5 * std::max<size_t>(1, ht.size()));
     FV_HIP(hipStreamSynchronize(ctx->stream));
     b->dirty = false;
     drop_graph(b);
@@ -326,19 +138,19 @@ void build_tables(fv_batch* b) {
 template <typename S>
 void enqueue_chunk(fv_batch* b) {
     hipStream_t s = b->ctx->stream;
-    const auto* probs = (const BProb<S>*)b->probs.p;
-    const auto* tiles = (const BTile*)b->tiles.p;
-    const int nt = b->nTiles;
+    const CgGrid<S> g{(const CgTile*)b->tiles.p, (const CgSys<S>*)b->probs.p, b->nTiles, 0};
+    const CgPartials red = partials_of<S>(b);
+    const int G = cg_grid(b->nTiles);
     for (int i = 0; i < b->chunk; ++i) {
-        kb_spmv<S><<<nt, kB, 0, s>>>(tiles, nt, probs, b->partials.p);
-        kb_update<S><<<nt, kB, 0, s>>>(tiles, nt, probs, b->partials.p);
+        kc_spmv<S><<<G, kB, 0, s>>>(g, red);
+        kc_update<S><<<G, kB, 0, s>>>(g, red);
         // the chunk ends after an update: its direction update uses the replaced residual
-        if (i + 1 < b->chunk) kb_direction<S><<<nt, kB, 0, s>>>(tiles, nt, probs);
+        if (i + 1 < b->chunk) kc_direction<S, 0><<<G, kB, 0, s>>>(g, red);
     }
-    kb_accum<S><<<nt, kB, 0, s>>>(tiles, nt, probs);
-    kb_resid<S><<<nt, kB, 0, s>>>(tiles, nt, probs, b->partials.p, 0);
-    kb_replace<S><<<nt, kB, 0, s>>>(tiles, nt, probs, b->partials.p, 0);
-    kb_direction<S><<<nt, kB, 0, s>>>(tiles, nt, probs);
+    kc_accum<S><<<G, kB, 0, s>>>(g);
+    kc_resid<S, 0><<<G, kB, 0, s>>>(g, red);
+    kc_replace<S, 0><<<G, kB, 0, s>>>(g, red);
+    kc_direction<S, 1><<<G, kB, 0, s>>>(g, red);
 }
 
 template <typename S>
@@ -367,43 +179,39 @@ template <typename S>
 void start_members(fv_batch* b) {
     fv_ctx* ctx = b->ctx;
     hipStream_t s = ctx->stream;
-    std::vector<BTile> st;
+    std::vector<CgTile> st;
     // tile list of the members that (re)start, with their global slots
     int slot = 0;
     for (int i = 0; i < (int)b->members.size(); ++i) {
         BMember* m = b->members[i];
-        const int nt = (m->N + b->tileRows - 1) / b->tileRows;
         if (m->needStart) {
-            for (int k = 0; k < nt; ++k) {
-                BTile t;
-                t.prob = i;
-                t.row0 = k * b->tileRows;
-                t.row1 = std::min(t.row0 + b->tileRows, ((m->N + 63) / 64) * 64);
-                t.slot = slot + k;
-                st.push_back(t);
-            }
+            member_tile_list<S>(b, i, slot, st);
             const fv_sparsity* sp = m->ls->sp;
-            kb_sell_values<S><<<grid_for(sp->sellTotal, kB, 8192), kB, 0, s>>>(
+            kc_sell_values<S><<<grid_for(sp->sellTotal, kB, 8192), kB, 0, s>>>(
                 sp->sellTotal, sp->sellSrc.p, m->ls->values.p, (S*)m->sval.p);
-            kb_dinv<S><<<grid_for(m->N, kB), kB, 0, s>>>(m->N, sp->rowOffs.p, sp->diagOffset.p,
+            kc_dinv<S><<<grid_for(m->N, kB), kB, 0, s>>>(m->N, sp->rowOffs.p, sp->diagOffset.p,
                                                          m->ls->values.p,
                                                          (S*)m->vec.p + 5 * (int64_t)m->N);
             kb_zero<<<grid_for(m->N * (int64_t)sizeof(S), kB, 4096), kB, 0, s>>>(
                 m->N * (int64_t)sizeof(S), m->vec.p);  // d = 0
-            kb_init_states<<<1, 64, 0, s>>>(b->states.p + m->slot, 1, b->tol, b->maxit, b->delta);
+            kc_init_state<<<1, 64, 0, s>>>(b->states.p + m->slot, 1,
+                                           m->tol > 0.0 ? m->tol : b->tol,
+                                           m->maxit > 0 ? m->maxit : b->maxit, b->delta);
             m->needStart = false;
         }
-        slot += nt;
+        slot += member_tiles(b, m);
     }
     if (st.empty()) return;
-    b->startTiles.alloc(ctx, st.size() * sizeof(BTile));
-    FV_HIP(hipMemcpyAsync(b->startTiles.p, st.data(), st.size() * sizeof(BTile),
+    b->startTiles.alloc(ctx, st.size() * sizeof(CgTile));
+    FV_HIP(hipMemcpyAsync(b->startTiles.p, st.data(), st.size() * sizeof(CgTile),
                           hipMemcpyHostToDevice, s));
-    const auto* probs = (const BProb<S>*)b->probs.p;
-    const auto* tiles = (const BTile*)b->startTiles.p;
-    const int nt = (int)st.size();
-    kb_resid<S><<<nt, kB, 0, s>>>(tiles, nt, probs, b->partials.p, 1);
-    kb_replace<S><<<nt, kB, 0, s>>>(tiles, nt, probs, b->partials.p, 1);
+    const CgGrid<S> g{(const CgTile*)b->startTiles.p, (const CgSys<S>*)b->probs.p,
+                      (int)st.size(), 0};
+    const CgPartials red = partials_of<S>(b);
+    const int G = cg_grid((int)st.size());
+    kc_resid<S, 1><<<G, kB, 0, s>>>(g, red);
+    kc_replace<S, 1><<<G, kB, 0, s>>>(g, red);
+    kc_direction<S, 2><<<G, kB, 0, s>>>(g, red);
     FV_LAUNCH_CHECK();
     // the upload buffer must outlive the launches
     FV_HIP(hipStreamSynchronize(s));
@@ -726,7 +534,7 @@ int fv_batch_solve(fv_batch* b, const fv_solve_opts* o, int use_graph, fv_solver
             ++chunks;
             bool running = false;
             for (const CgState& st : hs)
-                if (st.done != 2 && st.it < st.maxit) running = true;
+                if (st.stop < 2 && st.it < st.maxit) running = true;
             if (!running) break;
         }
         int bad = 0;

@@ -14,6 +14,8 @@ struct BMember {
     int N = 0;
     int slot = -1;           // index of the member's CgState (stable while it is in the batch)
     bool needStart = true;   // matrix/right-hand side (re)loaded, CG must (re)start
+    double tol = 0.0;        // FP64 relative residual target (0: the batch's)
+    int maxit = 0;           // CG iteration limit per solve (0: the batch's)
     fvhip::DBuf<uint8_t> vec;   // d, r, z, p, q, dinv (S) + r64 (double)
     fvhip::DBuf<uint8_t> sval;  // sliced-ELL values in S
     fv_solver_stats stats{};

@@ -1,0 +1,331 @@
+// cgkernels.hpp -- the Jacobi-preconditioned CG iteration as row-tile kernels,
+// shared by the single-system solver (fv_solve_cg) and the batched engine.
+//
+// A launch covers a list of tiles; a tile is a contiguous range of one
+// system's rows (a multiple of 64) and owns one partial-sum slot.  The single-
+// system solver uses implicit tiles (logical block b covers rows [b*R, b*R+R));
+// the batched engine passes an explicit tile table over many systems.  Logical
+// blocks are numbered XCD-contiguously (hardware block i runs on XCD i % 8), so
+// neighbouring row tiles -- and the SpMV gathers between them -- share an L2.
+//
+// One iteration is three kernels:
+//   spmv       q = A p (sliced ELL), partials of p.q
+//   update     alpha from the p.q partials; d += alpha p, r -= alpha q,
+//              z = D^-1 r, partials of r.r and r.z
+//   direction  beta and the stop decision from the r.r/r.z partials; p = z + beta p
+// and the FP64 residual replacement of the mixed solver is
+//   accum      x += d, d = 0
+//   resid      r64 = b - A x (FP64 CSR), partials of r.r and b.b
+//   replace    relative residual from those partials; r = r64, z = D^-1 r,
+//              partials of r.r and r.z (a direction kernel in mode 1 follows)
+// Every dot product is summed by each consuming block from the producer's
+// slots in a fixed order: deterministic, no atomics, no fences.
+#pragma once
+
+#include "cgstate.hpp"
+
+namespace fvhip {
+// internal linkage: every translation unit that launches these owns its copies
+namespace {
+
+template <typename S>
+struct CgSys {
+    int32_t N, tile0, ntiles, pad;  // rows; the system's slots [tile0, tile0 + ntiles)
+    const int64_t* sellOff;         // sliced ELL, 64-row slices, entries column-major
+    const int32_t* sellCol;
+    const S* sval;
+    const int32_t* rowOffs;         // FP64 CSR (residual replacement)
+    const int32_t* colIdxs;
+    const double* vals;
+    const double* rhs;
+    double* x;
+    double* xold;                   // optional copy target after the solve (may be null)
+    S *d, *r, *z, *p, *q, *dinv;    // d: correction (the solution itself in plain FP64)
+    double* r64;
+    CgState* st;
+};
+
+struct CgTile {
+    int32_t prob, row0, row1, slot;
+};
+
+template <typename S>
+struct CgGrid {
+    const CgTile* tiles;   // null: implicit tiles of probs[0]
+    const CgSys<S>* probs;
+    int32_t nTiles;        // tiles (explicit) or blocks with rows (implicit)
+    int32_t rowsPerBlock;  // implicit tiles
+};
+
+// Per-slot partial sums: p.q (1 per slot), r.r and r.z (2), FP64 r.r and b.b (2)
+struct CgPartials {
+    double *pq, *rz, *rb;
+};
+
+template <typename S>
+__device__ inline bool cg_locate(const CgGrid<S>& g, const CgSys<S>*& P, int& row0, int& row1,
+                                 int& slot) {
+    const int per = gridDim.x >> 3;  // the grid is a multiple of 8 blocks
+    const int lb = (int)(blockIdx.x & 7) * per + (int)(blockIdx.x >> 3);
+    if (lb >= g.nTiles) return false;
+    if (g.tiles) {
+        const CgTile t = g.tiles[lb];
+        P = g.probs + t.prob;
+        row0 = t.row0;
+        row1 = t.row1;
+        slot = t.slot;
+    } else {
+        P = g.probs;
+        row0 = lb * g.rowsPerBlock;
+        row1 = min(row0 + g.rowsPerBlock, (P->N + 63) & ~63);
+        slot = lb;
+    }
+    return true;
+}
+
+template <typename S>
+__global__ __launch_bounds__(kCgBlock) void kc_spmv(CgGrid<S> g, CgPartials red) {
+    __shared__ double sh[kCgBlock / 64];
+    const CgSys<S>* Pp;
+    int row0, row1, slot;
+    if (!cg_locate(g, Pp, row0, row1, slot)) return;
+    const CgSys<S>& P = *Pp;
+    CgState* st = P.st;
+    const int stop = st->stop;
+    if (row0 == 0 && threadIdx.x == 0) {
+        st->done = stop;
+        if (!stop) st->it += 1;
+    }
+    if (stop) return;
+    double pq = 0.0;
+    for (int r = row0 + threadIdx.x; r < row1; r += kCgBlock) {
+        const int sl = r >> 6, l = r & 63;
+        const int64_t b = P.sellOff[sl];
+        const int w = (int)((P.sellOff[sl + 1] - b) >> 6);
+        S s = 0;
+        for (int k = 0; k < w; ++k) {
+            const int64_t pos = b + 64 * (int64_t)k + l;
+            s += P.sval[pos] * P.p[P.sellCol[pos]];
+        }
+        if (r < P.N) {
+            P.q[r] = s;
+            pq += (double)P.p[r] * (double)s;
+        }
+    }
+    double v[1] = {pq};
+    write_slot<1>(v, red.pq, slot, sh);
+}
+
+template <typename S>
+__global__ __launch_bounds__(kCgBlock) void kc_update(CgGrid<S> g, CgPartials red) {
+    __shared__ double sh[kCgBlock / 64];
+    const CgSys<S>* Pp;
+    int row0, row1, slot;
+    if (!cg_locate(g, Pp, row0, row1, slot)) return;
+    const CgSys<S>& P = *Pp;
+    CgState* st = P.st;
+    if (st->done) return;
+    double tot[1];
+    sum_slots<1>(red.pq, P.tile0, P.ntiles, sh, tot);
+    const int it = st->it;
+    const double alpha = st->rz[(it - 1) & 1] / tot[0];
+    if (row0 == 0 && threadIdx.x == 0) st->alpha = alpha;
+    const S a = (S)alpha;
+    double rr = 0.0, rz = 0.0;
+    const int r1 = min(row1, P.N);
+    for (int i = row0 + threadIdx.x; i < r1; i += kCgBlock) {
+        P.d[i] += a * P.p[i];
+        const S ri = P.r[i] - a * P.q[i];
+        P.r[i] = ri;
+        const S zi = P.dinv[i] * ri;
+        P.z[i] = zi;
+        rr += (double)ri * (double)ri;
+        rz += (double)ri * (double)zi;
+    }
+    double v[2] = {rr, rz};
+    write_slot<2>(v, red.rz, slot, sh);
+}
+
+// MODE 0: after an update (stop on the recurrence estimate, the iteration limit,
+//         NaN or the residual-drop trigger);
+// MODE 1: after a residual replacement (stop on the FP64 residual: 2 converged,
+//         3 non-finite, 1 iteration limit);
+// MODE 2: after the start (replacement of the initial guess; p = z already).
+template <typename S, int MODE>
+__global__ __launch_bounds__(kCgBlock) void kc_direction(CgGrid<S> g, CgPartials red) {
+    __shared__ double sh[kCgBlock / 64];
+    const CgSys<S>* Pp;
+    int row0, row1, slot;
+    if (!cg_locate(g, Pp, row0, row1, slot)) return;
+    const CgSys<S>& P = *Pp;
+    CgState* st = P.st;
+    const int done = st->done;
+    if ((MODE == 0 && done) || (MODE == 1 && done >= 2)) return;
+    double tot[2];
+    sum_slots<2>(red.rz, P.tile0, P.ntiles, sh, tot);
+    const int it = st->it;
+    const double rn = sqrt(tot[0]);
+    int stop;
+    if (MODE == 0) {
+        const double est = rn / st->bnorm;
+        stop = (est <= st->tol || it >= st->maxit || !(est == est) ||
+                (st->delta > 0.0 && rn < st->delta * st->rrep))
+                   ? 1
+                   : 0;
+    } else {
+        const double res = st->res;
+        stop = res <= st->tol ? 2 : !(res <= 1.7976931348623157e308) ? 3 : it >= st->maxit ? 1 : 0;
+    }
+    if (row0 == 0 && threadIdx.x == 0) {
+        st->rz[it & 1] = tot[1];
+        if (MODE == 0)
+            st->res = rn / st->bnorm;
+        else
+            st->rrep = rn;
+        st->stop = stop;
+    }
+    if (MODE == 2 || stop != 0 || it == 0) return;
+    const S beta = (S)(tot[1] / st->rz[(it - 1) & 1]);
+    const int r1 = min(row1, P.N);
+    for (int i = row0 + threadIdx.x; i < r1; i += kCgBlock) P.p[i] = P.z[i] + beta * P.p[i];
+}
+
+template <typename S>
+__global__ __launch_bounds__(kCgBlock) void kc_accum(CgGrid<S> g) {
+    const CgSys<S>* Pp;
+    int row0, row1, slot;
+    if (!cg_locate(g, Pp, row0, row1, slot)) return;
+    const CgSys<S>& P = *Pp;
+    if (P.st->stop >= 2) return;
+    const int r1 = min(row1, P.N);
+    for (int i = row0 + threadIdx.x; i < r1; i += kCgBlock) {
+        P.x[i] += (double)P.d[i];
+        P.d[i] = S(0);
+    }
+}
+
+// r64 = b - A x in FP64 (CSR rows in column order), partials of r.r and b.b
+template <typename S, int START>
+__global__ __launch_bounds__(kCgBlock) void kc_resid(CgGrid<S> g, CgPartials red) {
+    __shared__ double sh[kCgBlock / 64];
+    const CgSys<S>* Pp;
+    int row0, row1, slot;
+    if (!cg_locate(g, Pp, row0, row1, slot)) return;
+    const CgSys<S>& P = *Pp;
+    if (!START && P.st->stop >= 2) return;
+    double rr = 0.0, bb = 0.0;
+    const int r1 = min(row1, P.N);
+    for (int i = row0 + threadIdx.x; i < r1; i += kCgBlock) {
+        double s = 0.0;
+        const int e = P.rowOffs[i + 1];
+        for (int k = P.rowOffs[i]; k < e; ++k) s += P.vals[k] * P.x[P.colIdxs[k]];
+        const double ri = P.rhs[i] - s;
+        P.r64[i] = ri;
+        rr += ri * ri;
+        bb += P.rhs[i] * P.rhs[i];
+    }
+    double v[2] = {rr, bb};
+    write_slot<2>(v, red.rb, slot, sh);
+}
+
+// relative FP64 residual from the resid partials; r = (S) r64, z = D^-1 r,
+// START: p = z and ||b||; partials of r.r and r.z
+template <typename S, int START>
+__global__ __launch_bounds__(kCgBlock) void kc_replace(CgGrid<S> g, CgPartials red) {
+    __shared__ double sh[kCgBlock / 64];
+    const CgSys<S>* Pp;
+    int row0, row1, slot;
+    if (!cg_locate(g, Pp, row0, row1, slot)) return;
+    const CgSys<S>& P = *Pp;
+    CgState* st = P.st;
+    if (!START && st->stop >= 2) return;
+    double tot[2];
+    sum_slots<2>(red.rb, P.tile0, P.ntiles, sh, tot);
+    double bn;
+    if (START) {
+        bn = sqrt(tot[1]);
+        if (!(bn > 0.0)) bn = 1.0;
+    } else {
+        bn = st->bnorm;
+    }
+    const double res = sqrt(tot[0]) / bn;
+    if (row0 == 0 && threadIdx.x == 0) {
+        st->res = res;
+        if (START) {
+            st->bnorm = bn;
+            st->res0 = res;
+        }
+    }
+    double rr = 0.0, rz = 0.0;
+    const int r1 = min(row1, P.N);
+    for (int i = row0 + threadIdx.x; i < r1; i += kCgBlock) {
+        const S ri = (S)P.r64[i];
+        const S zi = P.dinv[i] * ri;
+        P.r[i] = ri;
+        P.z[i] = zi;
+        if (START) P.p[i] = zi;
+        rr += (double)ri * (double)ri;
+        rz += (double)ri * (double)zi;
+    }
+    double v[2] = {rr, rz};
+    write_slot<2>(v, red.rz, slot, sh);
+}
+
+// One block: ||r64|| / ||b|| of system 0 from the resid partials into st.res
+template <typename S>
+__global__ __launch_bounds__(kCgBlock) void kc_resid_norm(const CgSys<S>* __restrict__ P,
+                                                          CgPartials red, CgState* out) {
+    __shared__ double sh[kCgBlock / 64];
+    double tot[2];
+    sum_slots<2>(red.rb, P->tile0, P->ntiles, sh, tot);
+    if (threadIdx.x == 0) {
+        double bn = sqrt(tot[1]);
+        if (!(bn > 0.0)) bn = 1.0;
+        out->bnorm = bn;
+        out->res = sqrt(tot[0]) / bn;
+    }
+}
+
+template <int = 0>
+__global__ void kc_init_state(CgState* st, int n, double tol, int maxit, double delta) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) {
+        CgState s{};
+        s.tol = tol;
+        s.maxit = maxit;
+        s.delta = delta;
+        st[i] = s;
+    }
+}
+
+template <typename S>
+__global__ void kc_sell_values(int64_t n, const int32_t* __restrict__ src,
+                               const double* __restrict__ values, S* __restrict__ out) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const int32_t j = src[i];
+        out[i] = j >= 0 ? (S)values[j] : (S)0;
+    }
+}
+
+template <typename S>
+__global__ void kc_dinv(int N, const int32_t* __restrict__ rowOffs, const int32_t* __restrict__ diag,
+                        const double* __restrict__ values, S* __restrict__ dinv) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < N) dinv[i] = (S)(1.0 / values[rowOffs[i] + diag[i]]);
+}
+
+// Grid for `tiles` logical blocks: a multiple of 8 (one share per XCD)
+inline int cg_grid(int tiles) { return ((tiles + 7) / 8) * 8; }
+
+// Rows per tile of an N-row system: at least `minRows`, and at most kCgMaxTiles
+// tiles per system (bounds the partial sums every block re-reads)
+constexpr int kCgMaxTiles = 1024;
+inline int cg_tile_rows(int N, int minRows) {
+    const int64_t need = ((int64_t)N + kCgMaxTiles - 1) / kCgMaxTiles;
+    const int r = (int)(((need + 63) / 64) * 64);
+    return r > minRows ? r : minRows;
+}
+
+}  // namespace
+}  // namespace fvhip

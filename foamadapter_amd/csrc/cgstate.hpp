@@ -1,5 +1,5 @@
-// cgstate.hpp -- device-resident CG scalars and the deterministic reductions
-// shared by the single-system solver and the batched engine.
+// cgstate.hpp -- device-resident CG scalars and the deterministic, fence-free
+// block reductions shared by the single-system solver and the batched engine.
 #pragma once
 
 #include "implicit.hpp"
@@ -8,17 +8,23 @@ namespace fvhip {
 
 constexpr int kCgBlock = 256;
 
-// Device-resident CG scalars.  rz[it & 1] holds r_it.z_it, so the direction
-// update reads beta = rz[it&1] / rz[(it-1)&1] without a second writer; a
-// residual replacement overwrites rz[it&1] with the replaced residual's value.
-// done: 0 running, 1 stopped (converged estimate, iteration limit, NaN, or the
-// residual-drop trigger of the mixed solver), 2 converged in FP64.
+// Device-resident CG scalars of one system.  Each field has one writer per
+// kernel (the thread 0 of the system's first tile) and is never read by the
+// kernel that writes it, so no fence or atomic is needed: the kernel boundary
+// orders a write before every read of it.
+//   stop  decision of the latest direction/restart kernel: 0 running, 1 stopped
+//         (recurrence estimate at the tolerance, iteration limit, NaN, or the
+//         residual-drop trigger of the mixed solver), 2 converged in FP64,
+//         3 failed (non-finite FP64 residual)
+//   done  copy of stop taken by the SpMV kernel at the start of an iteration;
+//         the rest of the iteration reads it
+//   it    completed iterations; the SpMV kernel of iteration k sets it = k + 1
+//   rz    rz[it & 1] holds r_it.z_it, so beta = rz[it&1] / rz[(it-1)&1]; a
+//         residual replacement overwrites rz[it & 1] with the replaced value
 struct CgState {
     double rz[2], alpha, bnorm, res, res0, rrep, delta;
-    int32_t it, done, maxit, pad;
+    int32_t it, done, maxit, stop;
     double tol;
-    uint32_t ticket;
-    uint32_t pad2;
 };
 
 __device__ inline double block_sum(double v, double* sh) {
@@ -35,44 +41,34 @@ __device__ inline double block_sum(double v, double* sh) {
     return r;  // valid in thread 0
 }
 
-__device__ inline double load_coherent(const double* p) {
-    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-// Each of `count` participating blocks writes its NV partials to slot `slot`
-// (slots [base, base+count)); the block that finishes last (integer ticket in
-// st) sums the slots in slot order and returns true with the totals in thread 0.
+// The block's NV partial sums go to slot `slot` of `partials` (NV doubles per
+// slot).  Plain stores: the consumer is a later kernel.
 template <int NV>
-__device__ bool publish_slots(const double (&v)[NV], double* partials, CgState* st, double* sh,
-                              int base, int count, int slot, double (&tot)[NV]) {
-    __shared__ bool last;
+__device__ inline void write_slot(const double (&v)[NV], double* partials, int slot, double* sh) {
     double s[NV];
     for (int j = 0; j < NV; ++j) s[j] = block_sum(v[j], sh);
-    if (threadIdx.x == 0) {
+    if (threadIdx.x == 0)
         for (int j = 0; j < NV; ++j) partials[(int64_t)slot * NV + j] = s[j];
-        __threadfence();
-        const uint32_t t = atomicAdd(&st->ticket, 1u);
-        last = (t == (uint32_t)count - 1);
-    }
-    __syncthreads();
-    if (!last) return false;
-    __threadfence();
+}
+
+// Every block of the consuming kernel sums the producer's slots [base,
+// base+count) itself, in one fixed order (strided per thread, then the
+// fixed-shape block tree), so all blocks obtain bitwise the same totals and the
+// result depends only on the system's own slot count.
+template <int NV>
+__device__ inline void sum_slots(const double* partials, int base, int count, double* sh,
+                                 double (&tot)[NV]) {
+    __shared__ double bc[NV];
     for (int j = 0; j < NV; ++j) {
         double a = 0.0;
         for (int b = threadIdx.x; b < count; b += blockDim.x)
-            a += load_coherent(&partials[(int64_t)(base + b) * NV + j]);
+            a += partials[(int64_t)(base + b) * NV + j];
         a = block_sum(a, sh);
-        if (threadIdx.x == 0) tot[j] = a;
+        if (threadIdx.x == 0) bc[j] = a;
     }
-    if (threadIdx.x == 0) st->ticket = 0;
-    return true;
-}
-
-// The single-system grid: every block of the grid participates.
-template <int NV>
-__device__ bool publish(const double (&v)[NV], double* partials, CgState* st, double* sh,
-                        double (&tot)[NV]) {
-    return publish_slots<NV>(v, partials, st, sh, 0, (int)gridDim.x, (int)blockIdx.x, tot);
+    __syncthreads();
+    for (int j = 0; j < NV; ++j) tot[j] = bc[j];
+    __syncthreads();
 }
 
 }  // namespace fvhip

@@ -9,6 +9,7 @@
 #include <string>
 #include <vector>
 #include <mutex>
+#include <map>
 #include <unordered_map>
 
 #include "../../include/fvhip.h"
@@ -77,11 +78,14 @@ struct fv_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
     size_t budget = 0;  // 0 = unlimited
-    size_t in_use = 0;
-    std::unordered_map<void*, size_t> sizes;
+    size_t in_use = 0;                                       // live bytes (as requested)
+    std::unordered_map<void*, std::pair<size_t, size_t>> sizes;  // live: requested, block size
+    std::multimap<size_t, void*> cache;                      // freed blocks by block size
+    size_t cached = 0;                                       // bytes held in the cache
     std::mutex mu;
     static constexpr int kEvents = 64;
     hipEvent_t events[kEvents] = {};
+    uint64_t gen = 0;  // unique per context of the process
 };
 
 namespace fvhip {
@@ -89,25 +93,38 @@ namespace fvhip {
 // Allocate device memory accounted against the context's HBM budget.
 void* ctx_alloc(fv_ctx* ctx, size_t bytes);
 void ctx_free(fv_ctx* ctx, void* p);
+// Generation of a live context (0 if it was destroyed); a buffer released after
+// its context is gone was freed with the context and is dropped.
+uint64_t ctx_generation(fv_ctx* ctx);
 
 // RAII device buffer owned by a library handle.
 template <typename T>
 struct DBuf {
     fv_ctx* ctx = nullptr;
+    uint64_t gen = 0;
     T* p = nullptr;
     size_t n = 0;
     DBuf() = default;
     DBuf(const DBuf&) = delete;
     DBuf& operator=(const DBuf&) = delete;
-    DBuf(DBuf&& o) noexcept : ctx(o.ctx), p(o.p), n(o.n) { o.p = nullptr; o.n = 0; }
+    DBuf(DBuf&& o) noexcept : ctx(o.ctx), gen(o.gen), p(o.p), n(o.n) { o.p = nullptr; o.n = 0; }
     DBuf& operator=(DBuf&& o) noexcept {
-        if (this != &o) { release(); ctx = o.ctx; p = o.p; n = o.n; o.p = nullptr; o.n = 0; }
+        if (this != &o) {
+            release();
+            ctx = o.ctx;
+            gen = o.gen;
+            p = o.p;
+            n = o.n;
+            o.p = nullptr;
+            o.n = 0;
+        }
         return *this;
     }
     ~DBuf() { release(); }
     void alloc(fv_ctx* c, size_t count) {
         release();
         ctx = c;
+        gen = ctx_generation(c);
         n = count;
         p = count ? static_cast<T*>(ctx_alloc(c, count * sizeof(T))) : nullptr;
     }
@@ -122,7 +139,7 @@ struct DBuf {
         }
     }
     void release() {
-        if (p && ctx) ctx_free(ctx, p);
+        if (p && ctx && gen && ctx_generation(ctx) == gen) ctx_free(ctx, p);
         p = nullptr;
         n = 0;
     }

@@ -3,6 +3,8 @@
 
 #include "mesh.hpp"
 
+#include <memory>
+
 // la::SparsityPattern (expression.hpp:35; test_sparsityPattern.cpp:36-62):
 // CSR of the face graph, columns ascending per row; row c is
 // [lower neighbours | c | upper neighbours] in the cell's stencil order.
@@ -23,9 +25,15 @@ struct fv_sparsity {
     fvhip::DBuf<int32_t> sellCol, sellSrc;
 };
 
+// Solver workspace kept with a linear system between solves (one per precision).
+struct fv_solver_cache {
+    virtual ~fv_solver_cache() = default;
+};
+
 // la::LinearSystem<scalar, localIdx>: values[nnz], rhs[N] (FP64), with the
 // convention A.x - b = (operator applied to x) * V.
 struct fv_linsys {
     fv_sparsity* sp = nullptr;
     fvhip::DBuf<double> values, rhs;
+    std::unique_ptr<fv_solver_cache> cg[2];  // FP64 CG, FP32 CG in FP64 refinement
 };

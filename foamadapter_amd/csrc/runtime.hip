@@ -6,8 +6,46 @@ namespace fvhip {
 
 static thread_local std::string g_last_error;
 
+// live contexts and their generations
+static std::mutex g_ctx_mu;
+static std::unordered_map<fv_ctx*, uint64_t> g_live;
+static uint64_t g_next_gen = 1;
+
+uint64_t ctx_generation(fv_ctx* ctx) {
+    std::lock_guard<std::mutex> lk(g_ctx_mu);
+    auto it = g_live.find(ctx);
+    return it == g_live.end() ? 0 : it->second;
+}
+
 void set_error(const std::string& msg) { g_last_error = msg; }
 void clear_error() { g_last_error.clear(); }
+
+// Device memory is pooled per context: freed blocks are kept, keyed by their
+// size class (multiples of 1/8 of the power of two below the request, at least
+// 256 bytes), and handed out again for a request of the same class.  Reuse
+// needs no synchronisation: a context's work is ordered on its one stream.  The
+// cache is trimmed (largest blocks first) beyond kCacheLimit and emptied before
+// a failed hipMalloc is retried.
+constexpr size_t kCacheLimit = size_t(8) << 30;
+
+size_t size_class(size_t bytes) {
+    if (bytes <= 256) return 256;
+    size_t p = 1;
+    while (p <= bytes / 2) p <<= 1;  // largest power of two <= bytes
+    const size_t step = std::max<size_t>(256, p / 8);
+    return (bytes + step - 1) / step * step;
+}
+
+void trim_cache(fv_ctx* ctx, size_t keep) {
+    if (ctx->cached <= keep) return;
+    (void)hipStreamSynchronize(ctx->stream);
+    while (ctx->cached > keep && !ctx->cache.empty()) {
+        auto it = std::prev(ctx->cache.end());
+        ctx->cached -= it->first;
+        (void)hipFree(it->second);
+        ctx->cache.erase(it);
+    }
+}
 
 void* ctx_alloc(fv_ctx* ctx, size_t bytes) {
     std::lock_guard<std::mutex> lk(ctx->mu);
@@ -15,13 +53,26 @@ void* ctx_alloc(fv_ctx* ctx, size_t bytes) {
         throw Error{FV_ENOMEM_BUDGET, "HBM budget exceeded: in use " + std::to_string(ctx->in_use) +
                                           " + request " + std::to_string(bytes) + " > budget " +
                                           std::to_string(ctx->budget)};
+    const size_t cls = size_class(bytes);
     void* p = nullptr;
-    FV_HIP(hipSetDevice(ctx->device));
-    hipError_t e = hipMalloc(&p, bytes ? bytes : 1);
-    if (e != hipSuccess)
-        throw Error{FV_EDEVICE, std::string("hipMalloc(") + std::to_string(bytes) + "): " +
-                                    hipGetErrorString(e)};
-    ctx->sizes[p] = bytes;
+    auto hit = ctx->cache.find(cls);
+    if (hit != ctx->cache.end()) {
+        p = hit->second;
+        ctx->cached -= cls;
+        ctx->cache.erase(hit);
+    } else {
+        FV_HIP(hipSetDevice(ctx->device));
+        hipError_t e = hipMalloc(&p, cls);
+        if (e != hipSuccess && !ctx->cache.empty()) {
+            (void)hipGetLastError();
+            trim_cache(ctx, 0);
+            e = hipMalloc(&p, cls);
+        }
+        if (e != hipSuccess)
+            throw Error{FV_EDEVICE, std::string("hipMalloc(") + std::to_string(cls) + "): " +
+                                        hipGetErrorString(e)};
+    }
+    ctx->sizes[p] = {bytes, cls};
     ctx->in_use += bytes;
     return p;
 }
@@ -30,11 +81,12 @@ void ctx_free(fv_ctx* ctx, void* p) {
     std::lock_guard<std::mutex> lk(ctx->mu);
     auto it = ctx->sizes.find(p);
     if (it == ctx->sizes.end()) return;
-    ctx->in_use -= it->second;
+    ctx->in_use -= it->second.first;
+    const size_t cls = it->second.second;
     ctx->sizes.erase(it);
-    // device-wide sync keeps a free from racing work still queued on the stream
-    (void)hipStreamSynchronize(ctx->stream);
-    (void)hipFree(p);
+    ctx->cache.emplace(cls, p);
+    ctx->cached += cls;
+    trim_cache(ctx, kCacheLimit);
 }
 
 template <typename S>
@@ -105,6 +157,11 @@ int fv_ctx_create(int device, size_t hbm_budget_bytes, fv_ctx** out) {
             delete c;
             throw Error{FV_EDEVICE, std::string("hipStreamCreate: ") + hipGetErrorString(e)};
         }
+        {
+            std::lock_guard<std::mutex> lk(g_ctx_mu);
+            c->gen = g_next_gen++;
+            g_live[c] = c->gen;
+        }
         *out = c;
     });
 }
@@ -112,10 +169,17 @@ int fv_ctx_create(int device, size_t hbm_budget_bytes, fv_ctx** out) {
 int fv_ctx_destroy(fv_ctx* ctx) {
     return guarded([&] {
         if (!ctx) return;
+        {
+            std::lock_guard<std::mutex> lk(g_ctx_mu);
+            g_live.erase(ctx);
+        }
         (void)hipSetDevice(ctx->device);
         (void)hipStreamSynchronize(ctx->stream);
         for (auto& kv : ctx->sizes) (void)hipFree(kv.first);
         ctx->sizes.clear();
+        for (auto& kv : ctx->cache) (void)hipFree(kv.second);
+        ctx->cache.clear();
+        ctx->cached = 0;
         for (auto& e : ctx->events)
             if (e) (void)hipEventDestroy(e);
         (void)hipStreamDestroy(ctx->stream);

@@ -1,8 +1,9 @@
 // server.cpp -- resident request loop for implicit solves (continuous batching).
 //
 // Requests arrive from any host thread (fv_server_submit) and queue.  A worker
-// thread owns the server's context: it admits queued requests while their
-// estimated device memory fits the HBM budget (and the in-flight limit),
+// thread owns the server's context: it admits queued requests while the
+// resident meshes plus the estimated device memory of the requests in flight
+// fit the HBM budget (and the in-flight limit),
 // builds each admitted request's system on its resident mesh (fields,
 // assembly), and then either
 //   * batched: steps the batched CG engine -- one chunk of iterations for every
@@ -86,17 +87,22 @@ struct fv_server {
     Clock::time_point tFirst, tLast;
     size_t peakBytes = 0;
     int peakActive = 0;
+    size_t residentBytes = 0;  // meshes, their sparsity patterns and boundary conditions
+    size_t inFlightBytes = 0;  // estimates of the admitted, unfinished requests
 };
 
 namespace {
 
-size_t estimate_bytes(const MeshEntry& me, int kind) {
+// Device bytes of one request: its linear system, fields and face coefficient,
+// and the solver's matrix copy and Krylov vectors in the server's precision.
+size_t estimate_bytes(const fv_server* s, const MeshEntry& me, int kind) {
     const size_t N = me.N, F = me.F, B = me.B, nnz = me.nnz;
-    size_t b = 8 * nnz + 8 * N            // linear system
-               + 8 * N * 2 + 8 * B         // x, x_old, boundary values
-               + 8 * (F + B)               // face coefficient (kappa or 1)
-               + 4 * (size_t)me.sellTotal  // FP32 matrix copy (batched)
-               + 6 * 4 * N + 8 * N;        // Krylov vectors
+    const size_t e = s->opts.mode == FV_SOLVE_F32_IR ? 4 : 8;
+    size_t b = 8 * nnz + 8 * N              // linear system
+               + 8 * N * 2 + 8 * B           // x, x_old, boundary values
+               + 8 * (F + B)                 // face coefficient (kappa or 1)
+               + e * (size_t)me.sellTotal    // sliced-ELL matrix copy
+               + 7 * e * N + 8 * N;          // Krylov vectors, correction, FP64 residual
     if (kind == FV_REQ_POISSON) b += 24 * N + 24 * B + 8 * (F + B) + 8 * N;  // U, phi, div
     return b + (1 << 16);
 }
@@ -202,6 +208,7 @@ void finish(fv_server* s, Req* r, int status) {
     }
     free_request_device(r);
     std::lock_guard<std::mutex> lk(s->qmu);
+    s->inFlightBytes -= r->estimate;
     r->status = status;
     r->state = FV_REQ_DONE;
     r->tDone = Clock::now();
@@ -221,7 +228,7 @@ void finish(fv_server* s, Req* r, int status) {
 std::vector<Req*> admit(fv_server* s, bool one_at_a_time) {
     std::vector<Req*> out;
     std::lock_guard<std::mutex> lk(s->qmu);
-    size_t inUse = s->ctx->in_use;
+    size_t inUse = s->residentBytes + s->inFlightBytes;
     while (!s->pending.empty()) {
         Req* r = s->pending.front();
         if (r->cancel) {
@@ -240,6 +247,7 @@ std::vector<Req*> admit(fv_server* s, bool one_at_a_time) {
         r->state = FV_REQ_RUNNING;
         r->tStart = Clock::now();
         inUse += r->estimate;
+        s->inFlightBytes += r->estimate;
         out.push_back(r);
     }
     return out;
@@ -284,6 +292,8 @@ void worker_loop(fv_server* s) {
                 setup(s, r);
                 if (batched) {
                     r->member = batch_add(s->batch, r->ls, r->x.p, r->xold.p, r);
+                    r->member->tol = r->d.rel_tol;
+                    r->member->maxit = r->d.max_iter;
                 }
                 std::lock_guard<std::mutex> lk(s->qmu);
                 s->active.push_back(r);
@@ -342,8 +352,9 @@ void worker_loop(fv_server* s) {
             BMember* m = members[i];
             Req* r = (Req*)m->user;
             const CgState& st = states[i];
-            const bool converged = st.done == 2;
-            const bool failed = !converged && (st.it >= st.maxit || !std::isfinite(st.res));
+            const bool converged = st.stop == 2;
+            const bool failed =
+                !converged && (st.stop == 3 || st.it >= st.maxit || !std::isfinite(st.res));
             if (!converged && !failed && !r->cancel) continue;
             r->iters += st.it;
             r->res = st.res;
@@ -421,6 +432,7 @@ int fv_server_add_mesh(fv_server* s, const fv_polymesh_desc* d, int renumber, in
         FV_REQUIRE(s && d && mesh_id, "NULL argument");
         auto me = std::make_unique<MeshEntry>();
         std::lock_guard<std::mutex> g(s->gpumu);
+        const size_t used0 = s->ctx->in_use;
         if (renumber) {
             const int64_t nF = d->n_faces, N = d->n_cells;
             std::vector<int32_t> foff(nF + 1), fpts(d->h_face_offsets[nF]), own(nF),
@@ -451,6 +463,7 @@ int fv_server_add_mesh(fv_server* s, const fv_polymesh_desc* d, int renumber, in
         me->nnz = me->sp->nnz;
         me->sellTotal = me->sp->sellTotal;
         std::lock_guard<std::mutex> lk(s->qmu);
+        s->residentBytes += s->ctx->in_use - used0;
         *mesh_id = (int)s->meshes.size();
         s->meshes.push_back(std::move(me));
     });
@@ -478,7 +491,7 @@ int fv_server_submit(fv_server* s, const fv_request* q, int64_t* id) {
         if (q->h_T0) r->T0.assign(q->h_T0, q->h_T0 + me.N);
         r->d.h_wall_values = nullptr;
         r->d.h_T0 = nullptr;
-        r->estimate = estimate_bytes(me, q->kind);
+        r->estimate = estimate_bytes(s, me, q->kind);
         r->tSubmit = Clock::now();
         r->id = s->nextId++;
         if (!s->haveFirst) {
@@ -486,7 +499,7 @@ int fv_server_submit(fv_server* s, const fv_request* q, int64_t* id) {
             s->tFirst = r->tSubmit;
         }
         ++s->submitted;
-        if (s->opts.hbm_budget && r->estimate > s->opts.hbm_budget) {
+        if (s->opts.hbm_budget && s->residentBytes + r->estimate > s->opts.hbm_budget) {
             ++s->rejected;
             delete r;
             throw Error{FV_ENOMEM_BUDGET, "request needs more device memory than the server budget"};
@@ -545,6 +558,15 @@ int fv_server_cancel(fv_server* s, int64_t id) {
     });
 }
 
+int fv_server_request_bytes(fv_server* s, int mesh_id, int kind, size_t* bytes) {
+    return guarded([&] {
+        FV_REQUIRE(s && bytes, "NULL argument");
+        FV_REQUIRE(kind == FV_REQ_HEAT || kind == FV_REQ_POISSON, "unknown request kind");
+        std::lock_guard<std::mutex> lk(s->qmu);
+        *bytes = estimate_bytes(s, mesh_of(s, mesh_id), kind);
+    });
+}
+
 int fv_server_get_stats(fv_server* s, fv_server_stats* o) {
     return guarded([&] {
         FV_REQUIRE(s && o, "NULL argument");
@@ -557,6 +579,7 @@ int fv_server_get_stats(fv_server* s, fv_server_stats* o) {
         o->cancelled = s->cancelled;
         o->peak_bytes = s->peakBytes;
         o->peak_active = s->peakActive;
+        o->resident_bytes = s->residentBytes;
         if (s->haveFirst && s->completed) o->wall_s = ms_between(s->tFirst, s->tLast) * 1e-3;
         if (o->wall_s > 0) o->solves_per_s = s->completed / o->wall_s;
         if (!s->latencies.empty()) {

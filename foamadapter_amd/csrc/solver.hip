@@ -3,17 +3,13 @@
 // criteria {iteration, relative_residual_norm}, src/compatibility/fvSolution.cpp:22-28,
 // 50-55,106-142), in FP64, and in FP32 inside FP64 iterative refinement.
 //
-// Per CG iteration three kernels run on the context stream:
-//   spmv   q = A p, partial sums of p.q          -> last block: alpha
-//   update x += alpha p, r -= alpha q, z = D^-1 r, partials of r.r, r.z
-//                                                -> last block: residual, beta, stop
-//   dir    p = z + beta p
-// Dot products are reduced deterministically: every block writes its partial to
-// a fixed slot, the last block to finish (integer ticket) sums the slots in
-// slot order.  All scalars live on the device, kernels turn into no-ops once
-// the stop flag is set, and the host polls the flag every `check` iterations,
-// so a solve needs no host round trip per iteration.
-#include "cgstate.hpp"
+// The iteration runs as the row-tile kernels of cgkernels.hpp over implicit
+// tiles of the one system (at most kCgMaxTiles tiles, rows in multiples of 64).
+// All scalars live on the device; kernels turn into no-ops once the stop flag is
+// set, and the host polls the flag every `check` iterations, so a solve needs
+// no host round trip per iteration.  The workspace stays with the linear
+// system, so repeated solves (time steps) allocate nothing.
+#include "cgkernels.hpp"
 
 #include <cmath>
 
@@ -21,269 +17,142 @@ using namespace fvhip;
 
 namespace {
 
-constexpr int kBlock = kCgBlock;
-constexpr int kMaxGrid = 1024;
+constexpr int kB = kCgBlock;
 
-// q = A p with A in sliced ELL (one 64-row slice per wavefront, entry k of all
-// 64 rows contiguous, so every value/column load is one coalesced 256/512-byte
-// access); rows summed in column order, as the CSR product
-struct Sell {
-    const int64_t* off;
-    const int32_t* col;
-};
-
+// Workspace of one CG in value type S on one linear system.
 template <typename S>
-__global__ __launch_bounds__(kBlock) void k_spmv_pq(int N, Sell A, const S* __restrict__ val,
-                                                    const S* __restrict__ p, S* __restrict__ q,
-                                                    double* __restrict__ partials,
-                                                    CgState* __restrict__ st) {
-    __shared__ double sh[kBlock / 64];
-    if (st->done) return;
-    double pq = 0.0;
-    const int npad = (N + 63) & ~63;
-    for (int r = blockIdx.x * blockDim.x + threadIdx.x; r < npad; r += gridDim.x * blockDim.x) {
-        const int sl = r >> 6, l = r & 63;
-        const int64_t b = A.off[sl];
-        const int w = (int)((A.off[sl + 1] - b) >> 6);
-        S s = 0;
-        for (int k = 0; k < w; ++k) {
-            const int64_t pos = b + 64 * (int64_t)k + l;
-            s += val[pos] * p[A.col[pos]];
-        }
-        if (r < N) {
-            q[r] = s;
-            pq += (double)p[r] * (double)s;
-        }
-    }
-    double v[1] = {pq}, tot[1];
-    if (publish<1>(v, partials, st, sh, tot) && threadIdx.x == 0)
-        st->alpha = st->rz[st->it & 1] / tot[0];
-}
+struct CgWork : fv_solver_cache {
+    DBuf<S> dbuf, r, z, p, q, dinv, sval;
+    DBuf<double> r64, pq, rz, rb;
+    DBuf<CgState> st, st2;
+    DBuf<CgSys<S>> sys;
+    CgSys<S> h{};
+    CgGrid<S> grid{};
+    CgPartials red{};
+    int G = 0;  // launch grid (multiple of 8)
 
-template <typename S>
-__global__ void k_sell_values(int64_t n, const int32_t* __restrict__ src,
-                              const double* __restrict__ values, S* __restrict__ out) {
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-         i += (int64_t)gridDim.x * blockDim.x) {
-        const int32_t j = src[i];
-        out[i] = j >= 0 ? (S)values[j] : (S)0;
-    }
-}
-
-template <typename S>
-__global__ __launch_bounds__(kBlock) void k_update(int N, S* __restrict__ x, S* __restrict__ r,
-                                                   S* __restrict__ z, const S* __restrict__ p,
-                                                   const S* __restrict__ q,
-                                                   const S* __restrict__ dinv,
-                                                   double* __restrict__ partials,
-                                                   CgState* __restrict__ st) {
-    __shared__ double sh[kBlock / 64];
-    if (st->done) return;
-    const S a = (S)st->alpha;
-    double rr = 0.0, rz = 0.0;
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += gridDim.x * blockDim.x) {
-        x[i] += a * p[i];
-        const S ri = r[i] - a * q[i];
-        r[i] = ri;
-        const S zi = dinv[i] * ri;
-        z[i] = zi;
-        rr += (double)ri * (double)ri;
-        rz += (double)ri * (double)zi;
-    }
-    double v[2] = {rr, rz}, tot[2];
-    if (publish<2>(v, partials, st, sh, tot) && threadIdx.x == 0) {
-        const int it = st->it + 1;
-        st->it = it;
-        st->rz[it & 1] = tot[1];
-        const double rn = sqrt(tot[0]);
-        st->res = rn / st->bnorm;
-        if (st->res <= st->tol || it >= st->maxit || !(st->res == st->res) ||
-            (st->delta > 0.0 && rn < st->delta * st->rrep))
-            st->done = 1;
-    }
-}
-
-template <typename S>
-__global__ __launch_bounds__(kBlock) void k_direction(int N, S* __restrict__ p,
-                                                      const S* __restrict__ z,
-                                                      const CgState* __restrict__ st) {
-    if (st->done) return;
-    const int it = st->it;
-    const S b = (S)(st->rz[it & 1] / st->rz[(it - 1) & 1]);
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += gridDim.x * blockDim.x)
-        p[i] = z[i] + b * p[i];
-}
-
-__global__ void k_set_delta(CgState* st, double delta) {
-    if (threadIdx.x == 0) st->delta = delta;
-}
-
-// r = b - A x ; z = D^-1 r ; p = z ; partials of b.b, r.r, r.z -> initial state
-template <typename S>
-__global__ __launch_bounds__(kBlock) void k_init(int N, const int32_t* __restrict__ rowOffs,
-                                                 const int32_t* __restrict__ col,
-                                                 const S* __restrict__ A,
-                                                 const S* __restrict__ b, const S* __restrict__ x,
-                                                 const S* __restrict__ dinv, S* __restrict__ r,
-                                                 S* __restrict__ z, S* __restrict__ p,
-                                                 double* __restrict__ partials,
-                                                 CgState* __restrict__ st, int x_is_zero) {
-    __shared__ double sh[kBlock / 64];
-    double bb = 0.0, rr = 0.0, rz = 0.0;
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += gridDim.x * blockDim.x) {
-        S s = 0;
-        if (!x_is_zero) {
-            const int e = rowOffs[i + 1];
-            for (int k = rowOffs[i]; k < e; ++k) s += A[k] * x[col[k]];
-        }
-        const S ri = b[i] - s;
-        const S zi = dinv[i] * ri;
-        r[i] = ri;
-        z[i] = zi;
-        p[i] = zi;
-        bb += (double)b[i] * (double)b[i];
-        rr += (double)ri * (double)ri;
-        rz += (double)ri * (double)zi;
-    }
-    double v[3] = {bb, rr, rz}, tot[3];
-    if (publish<3>(v, partials, st, sh, tot) && threadIdx.x == 0) {
-        double bn = sqrt(tot[0]);
-        st->bnorm = bn > 0.0 ? bn : 1.0;
-        st->res0 = sqrt(tot[1]) / st->bnorm;
-        st->res = st->res0;
-        st->rz[0] = tot[2];
-        st->it = 0;
-        st->done = (st->res <= st->tol || st->maxit <= 0) ? 1 : 0;
-    }
-}
-
-template <typename S>
-__global__ __launch_bounds__(kBlock) void k_dinv(int N, const int32_t* __restrict__ rowOffs,
-                                                 const int32_t* __restrict__ diag,
-                                                 const double* __restrict__ values,
-                                                 S* __restrict__ dinv) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < N) dinv[i] = (S)(1.0 / values[rowOffs[i] + diag[i]]);
-}
-
-template <typename D, typename S>
-__global__ void k_cast(int64_t n, const S* __restrict__ s, D* __restrict__ d) {
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-         i += (int64_t)gridDim.x * blockDim.x)
-        d[i] = (D)s[i];
-}
-
-__global__ void k_zero_state(CgState* st, double tol, int maxit) {
-    if (threadIdx.x == 0) {
-        CgState s{};
-        s.tol = tol;
-        s.maxit = maxit;
-        *st = s;
-    }
-}
-
-int grid_of(int N) { return grid_for(N, kBlock, kMaxGrid); }
-
-// Workspace for one CG in value type S.
-template <typename S>
-struct CgWork {
-    DBuf<S> r, z, p, q, dinv, sval;
-    DBuf<double> partials;
-    DBuf<CgState> st;
-    void alloc(fv_ctx* c, const fv_sparsity* sp) {
+    CgWork(fv_ctx* c, const fv_linsys* ls, bool separateCorrection) {
+        const fv_sparsity* sp = ls->sp;
         const int N = sp->nRows;
+        const int R = cg_tile_rows(N, kB);
+        const int tiles = (N + R - 1) / R;
+        G = cg_grid(tiles);
         r.alloc(c, N);
         z.alloc(c, N);
         p.alloc(c, N);
         q.alloc(c, N);
         dinv.alloc(c, N);
         sval.alloc(c, sp->sellTotal);
-        partials.alloc(c, 3 * kMaxGrid);
+        r64.alloc(c, N);
+        pq.alloc(c, tiles);
+        rz.alloc(c, 2 * (size_t)tiles);
+        rb.alloc(c, 2 * (size_t)tiles);
         st.alloc(c, 1);
+        st2.alloc(c, 1);
+        sys.alloc(c, 1);
+        if (separateCorrection) dbuf.alloc(c, N);
+        h.N = N;
+        h.tile0 = 0;
+        h.ntiles = tiles;
+        h.sellOff = sp->sellOff.p;
+        h.sellCol = sp->sellCol.p;
+        h.sval = sval.p;
+        h.rowOffs = sp->rowOffs.p;
+        h.colIdxs = sp->colIdxs.p;
+        h.vals = ls->values.p;
+        h.rhs = ls->rhs.p;
+        h.r = r.p;
+        h.z = z.p;
+        h.p = p.p;
+        h.q = q.p;
+        h.dinv = dinv.p;
+        h.r64 = r64.p;
+        h.st = st.p;
+        grid.tiles = nullptr;
+        grid.probs = sys.p;
+        grid.nTiles = tiles;
+        grid.rowsPerBlock = R;
+        red = CgPartials{pq.p, rz.p, rb.p};
     }
-    // matrix values (and Jacobi diagonal) of the assembled FP64 system in S
-    void load_matrix(fv_ctx* c, const fv_sparsity* sp, const double* values) {
+
+    // Binds the solution vector and loads the matrix (values may have changed
+    // since the last solve): sliced-ELL values and the Jacobi diagonal in S.
+    void bind(fv_ctx* c, const fv_linsys* ls, double* x) {
+        const fv_sparsity* sp = ls->sp;
         hipStream_t s = c->stream;
-        k_sell_values<S><<<grid_for(sp->sellTotal, kBlock, 8192), kBlock, 0, s>>>(
-            sp->sellTotal, sp->sellSrc.p, values, sval.p);
-        k_dinv<S><<<grid_for(sp->nRows, kBlock), kBlock, 0, s>>>(sp->nRows, sp->rowOffs.p,
-                                                                 sp->diagOffset.p, values, dinv.p);
+        h.x = x;
+        h.xold = nullptr;
+        h.d = dbuf.p ? dbuf.p : (S*)x;  // plain FP64 updates x in place
+        FV_HIP(hipMemcpyAsync(sys.p, &h, sizeof h, hipMemcpyHostToDevice, s));
+        kc_sell_values<S><<<grid_for(sp->sellTotal, kB, 8192), kB, 0, s>>>(
+            sp->sellTotal, sp->sellSrc.p, ls->values.p, sval.p);
+        kc_dinv<S><<<grid_for(sp->nRows, kB), kB, 0, s>>>(sp->nRows, sp->rowOffs.p,
+                                                          sp->diagOffset.p, ls->values.p, dinv.p);
+        if (dbuf.p) FV_HIP(hipMemsetAsync(dbuf.p, 0, dbuf.n * sizeof(S), s));
         FV_LAUNCH_CHECK();
+    }
+
+    // FP64 residual of the current x, r = (S) r64, z = D^-1 r, p = z, rz_0
+    void start(hipStream_t s, double tol, int maxit, double delta) {
+        kc_init_state<<<1, 64, 0, s>>>(st.p, 1, tol, maxit, delta);
+        kc_resid<S, 1><<<G, kB, 0, s>>>(grid, red);
+        kc_replace<S, 1><<<G, kB, 0, s>>>(grid, red);
+        kc_direction<S, 2><<<G, kB, 0, s>>>(grid, red);
+    }
+    void iteration(hipStream_t s, bool direction) {
+        kc_spmv<S><<<G, kB, 0, s>>>(grid, red);
+        kc_update<S><<<G, kB, 0, s>>>(grid, red);
+        if (direction) kc_direction<S, 0><<<G, kB, 0, s>>>(grid, red);
+    }
+    void replacement(hipStream_t s) {
+        kc_accum<S><<<G, kB, 0, s>>>(grid);
+        kc_resid<S, 0><<<G, kB, 0, s>>>(grid, red);
+        kc_replace<S, 0><<<G, kB, 0, s>>>(grid, red);
+        kc_direction<S, 1><<<G, kB, 0, s>>>(grid, red);
+    }
+    CgState state(hipStream_t s) {
+        CgState o{};
+        FV_HIP(hipMemcpyAsync(&o, st.p, sizeof o, hipMemcpyDeviceToHost, s));
+        FV_HIP(hipStreamSynchronize(s));
+        return o;
+    }
+    // explicit FP64 relative residual ||b - A x|| / ||b|| of the bound x
+    double true_residual(hipStream_t s) {
+        kc_resid<S, 1><<<G, kB, 0, s>>>(grid, red);
+        kc_resid_norm<S><<<1, kB, 0, s>>>(sys.p, red, st2.p);
+        FV_LAUNCH_CHECK();
+        CgState o{};
+        FV_HIP(hipMemcpyAsync(&o, st2.p, sizeof o, hipMemcpyDeviceToHost, s));
+        FV_HIP(hipStreamSynchronize(s));
+        return o.res;
     }
 };
 
-// Runs CG for x (S) on the matrix loaded into w; csrA (values in S, CSR) is
-// only read for the initial residual when x is not zero.
 template <typename S>
-CgState run_cg(fv_ctx* ctx, const fv_sparsity* sp, const S* csrA, const S* b, S* x, bool x_zero,
-               CgWork<S>& w, double tol, int maxit, int check) {
-    const int N = sp->nRows, G = grid_of(N);
+CgWork<S>& work_of(fv_linsys* ls, bool mixed) {
+    auto& slot = ls->cg[mixed ? 1 : 0];
+    if (!slot) slot.reset(new CgWork<S>(ls->sp->mesh->ctx, ls, mixed));
+    return *static_cast<CgWork<S>*>(slot.get());
+}
+
+// Plain FP64 CG from x; stops on the recurrence residual estimate.
+CgState run_cg(fv_ctx* ctx, CgWork<double>& w, double tol, int maxit, int check) {
     hipStream_t s = ctx->stream;
-    const Sell sell{sp->sellOff.p, sp->sellCol.p};
-    k_zero_state<<<1, 64, 0, s>>>(w.st.p, tol, maxit);
-    k_init<S><<<G, kBlock, 0, s>>>(N, sp->rowOffs.p, sp->colIdxs.p, csrA, b, x, w.dinv.p, w.r.p,
-                                   w.z.p, w.p.p, w.partials.p, w.st.p, x_zero ? 1 : 0);
+    w.start(s, tol, maxit, 0.0);
     FV_LAUNCH_CHECK();
     CgState h{};
     int launched = 0, batch = std::min(4, check);
     while (true) {
-        FV_HIP(hipMemcpyAsync(&h, w.st.p, sizeof(CgState), hipMemcpyDeviceToHost, s));
-        FV_HIP(hipStreamSynchronize(s));
-        if (h.done || launched >= maxit) break;
+        h = w.state(s);
+        if (h.stop || launched >= maxit) break;
         // poll early for short solves, then every `check` iterations
         const int n = std::min(batch, maxit - launched);
         batch = std::min(2 * batch, check);
-        for (int i = 0; i < n; ++i) {
-            k_spmv_pq<S><<<G, kBlock, 0, s>>>(N, sell, w.sval.p, w.p.p, w.q.p, w.partials.p,
-                                              w.st.p);
-            k_update<S><<<G, kBlock, 0, s>>>(N, x, w.r.p, w.z.p, w.p.p, w.q.p, w.dinv.p,
-                                             w.partials.p, w.st.p);
-            k_direction<S><<<G, kBlock, 0, s>>>(N, w.p.p, w.z.p, w.st.p);
-        }
+        for (int i = 0; i < n; ++i) w.iteration(s, true);
         FV_LAUNCH_CHECK();
         launched += n;
     }
     return h;
-}
-
-__global__ void k_residual_norms(int N, const int32_t* __restrict__ rowOffs,
-                                 const int32_t* __restrict__ col, const double* __restrict__ A,
-                                 const double* __restrict__ b, const double* __restrict__ x,
-                                 double* __restrict__ r, double* __restrict__ partials,
-                                 CgState* __restrict__ st) {
-    __shared__ double sh[kBlock / 64];
-    double bb = 0.0, rr = 0.0;
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += gridDim.x * blockDim.x) {
-        double s = 0.0;
-        const int e = rowOffs[i + 1];
-        for (int k = rowOffs[i]; k < e; ++k) s += A[k] * x[col[k]];
-        const double ri = b[i] - s;
-        r[i] = ri;
-        bb += b[i] * b[i];
-        rr += ri * ri;
-    }
-    double v[2] = {bb, rr}, tot[2];
-    if (publish<2>(v, partials, st, sh, tot) && threadIdx.x == 0) {
-        const double bn = sqrt(tot[0]);
-        st->bnorm = bn > 0.0 ? bn : 1.0;
-        st->res = sqrt(tot[1]) / st->bnorm;
-    }
-}
-
-// FP64 explicit residual: r = b - A x, returns ||r||/||b||
-double true_residual(fv_ctx* ctx, const fv_linsys* ls, const double* x, double* r,
-                     DBuf<double>& partials, DBuf<CgState>& st) {
-    const fv_sparsity* sp = ls->sp;
-    const int N = sp->nRows;
-    hipStream_t s = ctx->stream;
-    k_zero_state<<<1, 64, 0, s>>>(st.p, 0.0, 0);
-    k_residual_norms<<<grid_of(N), kBlock, 0, s>>>(N, sp->rowOffs.p, sp->colIdxs.p,
-                                                   ls->values.p, ls->rhs.p, x, r, partials.p, st.p);
-    FV_LAUNCH_CHECK();
-    CgState h{};
-    FV_HIP(hipMemcpyAsync(&h, st.p, sizeof(CgState), hipMemcpyDeviceToHost, s));
-    FV_HIP(hipStreamSynchronize(s));
-    return h.res;
 }
 
 // ---- mixed precision: FP32 CG with FP64 residual replacement ---------------
@@ -295,107 +164,30 @@ double true_residual(fv_ctx* ctx, const fv_linsys* ls, const double* x, double* 
 // tolerance; the FP32 residual is replaced by the true one and CG continues
 // with its search direction (beta from the replaced residual), so the Krylov
 // space is not restarted.
-
-__global__ void k_ir_accum(int N, float* __restrict__ d, double* __restrict__ x) {
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += gridDim.x * blockDim.x) {
-        x[i] += (double)d[i];
-        d[i] = 0.0f;
-    }
-}
-
-// r32 = r64; z = D^-1 r32; (start: p = z); partials of r.r and r.z
-__global__ __launch_bounds__(kBlock) void k_ir_replace(int N, const double* __restrict__ r64,
-                                                       const float* __restrict__ dinv,
-                                                       float* __restrict__ r, float* __restrict__ z,
-                                                       float* __restrict__ p,
-                                                       double* __restrict__ partials,
-                                                       CgState* __restrict__ st,
-                                                       const CgState* __restrict__ st64, int start) {
-    __shared__ double sh[kBlock / 64];
-    double rr = 0.0, rz = 0.0;
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += gridDim.x * blockDim.x) {
-        const float ri = (float)r64[i];
-        const float zi = dinv[i] * ri;
-        r[i] = ri;
-        z[i] = zi;
-        if (start) p[i] = zi;
-        rr += (double)ri * (double)ri;
-        rz += (double)ri * (double)zi;
-    }
-    double v[2] = {rr, rz}, tot[2];
-    if (publish<2>(v, partials, st, sh, tot) && threadIdx.x == 0) {
-        st->bnorm = st64->bnorm;
-        st->rrep = sqrt(tot[0]);
-        if (start) st->res0 = st64->res;
-        // the replaced residual takes the place of r_it (see CgState)
-        st->rz[st->it & 1] = tot[1];
-        st->res = st64->res;
-        // 2: converged in FP64; 0: continue (direction update follows)
-        st->done = st64->res <= st->tol ? 2 : 0;
-    }
-}
-
 struct IrOut {
     double init_res, res;
     int iters, outer;
 };
 
-IrOut run_mixed(fv_ctx* ctx, const fv_linsys* ls, double* x, double tol, int maxit, int chunk,
-                double delta, int max_outer) {
-    const fv_sparsity* sp = ls->sp;
-    const int N = sp->nRows, G = grid_of(N);
+IrOut run_mixed(fv_ctx* ctx, CgWork<float>& w, double tol, int maxit, int chunk, double delta,
+                int max_outer) {
     hipStream_t s = ctx->stream;
-    CgWork<float> w;
-    w.alloc(ctx, sp);
-    w.load_matrix(ctx, sp, ls->values.p);
-    DBuf<float> d32;
-    d32.alloc(ctx, N);
-    FV_HIP(hipMemsetAsync(d32.p, 0, N * sizeof(float), s));
-    DBuf<double> r64, partials64;
-    DBuf<CgState> st64;
-    r64.alloc(ctx, N);
-    partials64.alloc(ctx, 3 * kMaxGrid);
-    st64.alloc(ctx, 1);
-    const Sell sell{sp->sellOff.p, sp->sellCol.p};
-    // FP64 residual of the initial guess, then FP32 CG state from it
-    k_zero_state<<<1, 64, 0, s>>>(w.st.p, tol, maxit);
-    k_zero_state<<<1, 64, 0, s>>>(st64.p, 0.0, 0);
-    k_residual_norms<<<G, kBlock, 0, s>>>(N, sp->rowOffs.p, sp->colIdxs.p, ls->values.p,
-                                          ls->rhs.p, x, r64.p, partials64.p, st64.p);
-    k_ir_replace<<<G, kBlock, 0, s>>>(N, r64.p, w.dinv.p, w.r.p, w.z.p, w.p.p, w.partials.p,
-                                      w.st.p, st64.p, 1);
-    k_set_delta<<<1, 64, 0, s>>>(w.st.p, delta);
+    w.start(s, tol, maxit, delta);
     FV_LAUNCH_CHECK();
     IrOut out{0, 0, 0, 0};
-    CgState h{};
-    FV_HIP(hipMemcpyAsync(&h, w.st.p, sizeof(CgState), hipMemcpyDeviceToHost, s));
-    FV_HIP(hipStreamSynchronize(s));
+    CgState h = w.state(s);
     out.init_res = h.res0;
     out.res = h.res;
     int batch = std::min(8, chunk);
-    while (h.done != 2 && h.it < maxit && out.outer < max_outer) {
-        // short chunks first (short solves end early), then `chunk` iterations
+    while (h.stop < 2 && h.it < maxit && out.outer < max_outer) {
+        // short chunks first (short solves end early), then `chunk` iterations;
+        // the chunk ends after an update: its direction update uses the replaced residual
         const int n = std::min(batch, maxit - h.it);
         batch = std::min(2 * batch, chunk);
-        // the chunk ends after an update: its direction update uses the replaced residual
-        for (int i = 0; i < n; ++i) {
-            k_spmv_pq<float><<<G, kBlock, 0, s>>>(N, sell, w.sval.p, w.p.p, w.q.p, w.partials.p,
-                                                  w.st.p);
-            k_update<float><<<G, kBlock, 0, s>>>(N, d32.p, w.r.p, w.z.p, w.p.p, w.q.p, w.dinv.p,
-                                                 w.partials.p, w.st.p);
-            if (i + 1 < n) k_direction<float><<<G, kBlock, 0, s>>>(N, w.p.p, w.z.p, w.st.p);
-        }
-        // FP64 residual replacement and convergence test
-        k_ir_accum<<<G, kBlock, 0, s>>>(N, d32.p, x);
-        k_zero_state<<<1, 64, 0, s>>>(st64.p, 0.0, 0);
-        k_residual_norms<<<G, kBlock, 0, s>>>(N, sp->rowOffs.p, sp->colIdxs.p, ls->values.p,
-                                              ls->rhs.p, x, r64.p, partials64.p, st64.p);
-        k_ir_replace<<<G, kBlock, 0, s>>>(N, r64.p, w.dinv.p, w.r.p, w.z.p, w.p.p, w.partials.p,
-                                          w.st.p, st64.p, 0);
-        k_direction<float><<<G, kBlock, 0, s>>>(N, w.p.p, w.z.p, w.st.p);
+        for (int i = 0; i < n; ++i) w.iteration(s, i + 1 < n);
+        w.replacement(s);
         FV_LAUNCH_CHECK();
-        FV_HIP(hipMemcpyAsync(&h, w.st.p, sizeof(CgState), hipMemcpyDeviceToHost, s));
-        FV_HIP(hipStreamSynchronize(s));
+        h = w.state(s);
         ++out.outer;
         out.res = h.res;
     }
@@ -417,18 +209,18 @@ int fv_solve_cg(fv_linsys* ls, void* d_x, const fv_solve_opts* o, fv_solver_stat
         const int check = o->check_every > 0 ? o->check_every : 8;
         double* x = (double*)d_x;
         if (o->mode == FV_SOLVE_F64) {
-            CgWork<double> w;
-            w.alloc(ctx, sp);
-            w.load_matrix(ctx, sp, ls->values.p);
-            CgState h = run_cg<double>(ctx, sp, ls->values.p, ls->rhs.p, x, false, w, o->rel_tol,
-                                       o->max_iter, check);
+            CgWork<double>& w = work_of<double>(ls, false);
+            w.bind(ctx, ls, x);
+            CgState h = run_cg(ctx, w, o->rel_tol, o->max_iter, check);
             local.init_res = h.res0;
             local.final_res = h.res;
             local.iters = h.it;
             local.outer_iters = 1;
-            local.true_res = true_residual(ctx, ls, x, w.r.p, w.partials, w.st);
+            local.true_res = w.true_residual(ctx->stream);
         } else if (o->mode == FV_SOLVE_F32_IR) {
-            IrOut r = run_mixed(ctx, ls, x, o->rel_tol, o->max_iter,
+            CgWork<float>& w = work_of<float>(ls, true);
+            w.bind(ctx, ls, x);
+            IrOut r = run_mixed(ctx, w, o->rel_tol, o->max_iter,
                                 o->check_every > 0 ? o->check_every : 64,
                                 o->inner_rel_tol > 0.0 ? o->inner_rel_tol : 1e-3,
                                 o->max_outer > 0 ? o->max_outer : (1 << 30));

@@ -6,6 +6,7 @@ memory (NeoN::Vector storage on the GPU executor).
 """
 from __future__ import annotations
 
+import atexit
 import ctypes as C
 import weakref
 
@@ -20,6 +21,20 @@ def prec_of(dtype) -> int:
     return _DT[np.dtype(dtype)]
 
 
+_LIVE = weakref.WeakSet()
+
+
+@atexit.register
+def _close_all():
+    # at interpreter exit, release every context's handles while they are all
+    # still alive (finalizers would otherwise run in arbitrary order)
+    for ctx in list(_LIVE):
+        try:
+            ctx.close()
+        except Exception:
+            pass
+
+
 class Context:
     def __init__(self, device: int = 0, hbm_budget: int = 0):
         h = C.c_void_p()
@@ -27,6 +42,7 @@ class Context:
         self.handle = h
         self.device = device
         self._children = weakref.WeakSet()
+        _LIVE.add(self)
 
     def adopt(self, obj):
         self._children.add(obj)

@@ -45,7 +45,7 @@ class FvServerStats(C.Structure):
                 ("rejected", C.c_int64), ("cancelled", C.c_int64), ("wall_s", C.c_double),
                 ("solves_per_s", C.c_double), ("p50_ms", C.c_double), ("p99_ms", C.c_double),
                 ("mean_ms", C.c_double), ("max_ms", C.c_double), ("peak_bytes", C.c_size_t),
-                ("peak_active", C.c_int32)]
+                ("peak_active", C.c_int32), ("resident_bytes", C.c_size_t)]
 
 
 HEAT, POISSON = 0, 1
@@ -58,6 +58,7 @@ _SIGS = {
     "fv_server_wait": [C.c_void_p, C.c_int64, C.c_void_p, C.POINTER(FvRequestStats)],
     "fv_server_cancel": [C.c_void_p, C.c_int64],
     "fv_server_get_stats": [C.c_void_p, C.POINTER(FvServerStats)],
+    "fv_server_request_bytes": [C.c_void_p, C.c_int, C.c_int, C.POINTER(C.c_size_t)],
     "fv_server_destroy": [C.c_void_p],
 }
 for _n, _a in _SIGS.items():
@@ -136,6 +137,12 @@ class Server:
 
     def cancel(self, rid: int):
         call("fv_server_cancel", self.handle, rid)
+
+    def request_bytes(self, mesh_id: int, kind: int = HEAT) -> int:
+        """Device bytes a request of `kind` (HEAT or POISSON) is admitted with."""
+        n = C.c_size_t()
+        call("fv_server_request_bytes", self.handle, mesh_id, kind, C.byref(n))
+        return n.value
 
     def stats(self) -> dict:
         s = FvServerStats()

@@ -384,7 +384,8 @@ typedef struct fv_server fv_server;
 enum { FV_REQ_HEAT = 0, FV_REQ_POISSON = 1 };
 enum { FV_REQ_QUEUED = 0, FV_REQ_RUNNING = 1, FV_REQ_DONE = 2 };
 typedef struct {
-    size_t hbm_budget;   /* admission budget in bytes (0: unlimited) */
+    size_t hbm_budget;   /* admission budget in bytes: resident meshes + requests in
+                            flight (0: unlimited) */
     int32_t mode;        /* FV_SOLVE_F32_IR or FV_SOLVE_F64 */
     int32_t batched;     /* 1: continuous batching, 0: one request at a time via fv_solve_cg */
     int32_t max_active;  /* systems in flight (batched), 0: no limit */
@@ -419,11 +420,16 @@ typedef struct {
     double p50_ms, p99_ms, mean_ms, max_ms;
     size_t peak_bytes;   /* peak device bytes in use */
     int32_t peak_active; /* peak requests in flight */
+    size_t resident_bytes; /* device bytes of the resident meshes */
 } fv_server_stats;
 int fv_server_create(int device, const fv_server_opts* opts, fv_server** out);
 /* Adds a resident mesh (renumber != 0: Z-order renumbered internally). */
 int fv_server_add_mesh(fv_server* s, const fv_polymesh_desc* mesh, int renumber, int* mesh_id);
 int fv_server_mesh_cells(fv_server* s, int mesh_id, int32_t* n_cells);
+/* Device bytes one request of `kind` on the mesh is admitted with.  The budget
+ * holds the resident meshes plus these estimates of the requests in flight (at
+ * least one request runs whenever it fits beside the meshes). */
+int fv_server_request_bytes(fv_server* s, int mesh_id, int kind, size_t* bytes);
 /* Queues a request; FV_ENOMEM_BUDGET if it can never fit the budget. */
 int fv_server_submit(fv_server* s, const fv_request* req, int64_t* req_id);
 int fv_server_poll(fv_server* s, int64_t req_id, int* state);

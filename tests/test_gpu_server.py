@@ -116,16 +116,32 @@ def test_result_same_alone_and_in_mix(polys):
 
 def test_budget_admission_and_rejection(polys):
     from foamadapter_amd import FvError
-    # a budget that fits one request of the largest mesh at a time
-    srv, ids = _server(polys, batched=True, hbm_budget=3 << 20, tile_rows=256, chunk=8)
+    from foamadapter_amd.serve import HEAT
+    # size the budget from the server's own figures: the resident meshes plus
+    # one and a half heat requests on the largest mesh -> one request at a time
+    probe, ids = _server(polys, batched=True)
+    resident = probe.stats()["resident_bytes"]
+    est = probe.request_bytes(ids[2], HEAT)
+    probe.close()
+    assert resident > 0 and est > 8 * polys[2].n_cells
+    srv, ids = _server(polys, batched=True, hbm_budget=resident + est * 3 // 2, tile_rows=256,
+                       chunk=8)
     rids = [srv.submit_heat(ids[2], kappa=1e-2, steps=1, seed=i) for i in range(4)]
     for r in rids:
         _, st = srv.wait(r)
         assert st.status == 0
     s = srv.stats()
-    assert s["completed"] == 4 and s["peak_active"] <= 2
+    assert s["completed"] == 4 and s["peak_active"] == 1
     srv.close()
-    srv, ids = _server(polys, batched=True, hbm_budget=1 << 16)
+    # room for two at a time
+    srv, ids = _server(polys, batched=True, hbm_budget=resident + est * 2, tile_rows=256, chunk=8)
+    rids = [srv.submit_heat(ids[2], kappa=1e-2, steps=1, seed=i) for i in range(4)]
+    for r in rids:
+        assert srv.wait(r)[1].status == 0
+    assert srv.stats()["peak_active"] <= 2
+    srv.close()
+    # a request that can never fit is rejected at submission
+    srv, ids = _server(polys, batched=True, hbm_budget=resident + est // 2)
     with pytest.raises(FvError) as e:
         srv.submit_heat(ids[2], kappa=1e-2)
     assert e.value.code == 2   # FV_ENOMEM_BUDGET
