@@ -154,24 +154,27 @@ __global__ __launch_bounds__(kBlock) void k_laplacian(
 
 // ---- LDS-staged cell gathers (meshes in upper-triangular face order) --------
 //
-// A workgroup owns kCB consecutive cells [c0, c1).  In OpenFOAM face order the
+// A workgroup owns CELLS consecutive cells [c0, c1) (256, 512 or 1024).  In OpenFOAM face order the
 // internal faces those cells own are one contiguous range [fs, fe), so:
 //   phase 1: the workgroup streams that face range with coalesced loads,
 //            computes every face contribution once and keeps it in LDS;
-//   phase 2: one thread per cell accumulates, in ascending face order, its
+//   phase 2: each thread takes cells in turn and accumulates, in ascending face order, its
 //            lower faces (owned by lower-numbered cells: read from LDS when the
 //            owner is in this workgroup, recomputed otherwise), its own faces
 //            (from LDS), then its boundary faces, and divides by V.
 // The accumulation order is the serial face order of the other kernels, so the
 // result is the same bit for bit.  On a Z-order renumbered mesh a workgroup is a
-// compact brick and only the faces on its lower sides are recomputed.
+// compact brick (8x8x4, 8x8x8 or 16x8x8 cells) and only the faces on its lower
+// sides are recomputed; larger bricks recompute fewer faces per cell.
 // Workgroups are dispatched round-robin over the 8 XCDs; the block index is
 // remapped so each XCD walks one contiguous range of bricks and keeps the
 // faces it recomputes in its own L2.
 
-constexpr int kCB = 256;      // cells per workgroup
-constexpr int kMaxUp = 1024;  // LDS face slots per workgroup
 constexpr int kXcd = 8;
+// LDS face slots of a CELLS-cell workgroup: 3 owned faces per hex cell plus slack;
+// faces beyond it (unusual meshes) are recomputed in phase 2
+template <int CELLS>
+constexpr int max_up() { return CELLS * 3 + CELLS / 4; }
 
 struct Split {
     const int32_t *own, *nbr, *ownStart, *lowOff, *lowFace, *lowOwn, *bOff, *bFace;
@@ -235,25 +238,12 @@ struct LapOp {
 };
 
 template <typename S, typename Op>
-__global__ __launch_bounds__(kCB) void k_gather(int N, int nBlocks, int perXcd, Split t, Op op,
-                                                const S* __restrict__ V, S coeff,
-                                                S* __restrict__ out) {
+__device__ inline void gather_cell(int c, int fs, int nUp, const Split& t, const Op& op,
+                                   const S* __restrict__ V, S coeff, S* __restrict__ out,
+                                   const S* flux0, int stride) {
     constexpr int NC = Op::NC;
-    __shared__ S flux[NC][kMaxUp];
-    const int lb = (blockIdx.x % kXcd) * perXcd + blockIdx.x / kXcd;
-    if (lb >= nBlocks) return;
-    const int c0 = lb * kCB, c1 = min(c0 + kCB, N);
-    const int fs = t.ownStart[c0], fe = t.ownStart[c1];
-    const int nUp = min(fe - fs, kMaxUp);
-    for (int i = threadIdx.x; i < nUp; i += kCB) {
-        const int f = fs + i;
-        S v[NC];
-        op.face(f, t.own[f], t.nbr[f], v);
-        for (int j = 0; j < NC; ++j) flux[j][i] = v[j];
-    }
-    __syncthreads();
-    const int c = c0 + threadIdx.x;
-    if (c >= c1) return;
+    const S* flux[NC];
+    for (int j = 0; j < NC; ++j) flux[j] = flux0 + j * stride;
     S acc[NC];
     for (int j = 0; j < NC; ++j) acc[j] = S(0);
     const int l1 = t.lowOff[c + 1];
@@ -289,18 +279,50 @@ __global__ __launch_bounds__(kCB) void k_gather(int N, int nBlocks, int perXcd, 
     for (int j = 0; j < NC; ++j) out[(int64_t)NC * c + j] = coeff * (acc[j] / vol);
 }
 
+template <typename S, typename Op, int CELLS, int THREADS>
+__global__ __launch_bounds__(THREADS) void k_gather(int N, int nBlocks, int perXcd, Split t, Op op,
+                                                    const S* __restrict__ V, S coeff,
+                                                    S* __restrict__ out) {
+    constexpr int NC = Op::NC;
+    constexpr int kMaxUp = max_up<CELLS>();
+    __shared__ S flux[NC][kMaxUp];
+    const int lb = (blockIdx.x % kXcd) * perXcd + blockIdx.x / kXcd;
+    if (lb >= nBlocks) return;
+    const int c0 = lb * CELLS, c1 = min(c0 + CELLS, N);
+    const int fs = t.ownStart[c0], fe = t.ownStart[c1];
+    const int nUp = min(fe - fs, kMaxUp);
+    for (int i = threadIdx.x; i < nUp; i += THREADS) {
+        const int f = fs + i;
+        S v[NC];
+        op.face(f, t.own[f], t.nbr[f], v);
+        for (int j = 0; j < NC; ++j) flux[j][i] = v[j];
+    }
+    __syncthreads();
+    for (int c = c0 + threadIdx.x; c < c1; c += THREADS)
+        gather_cell<S, Op>(c, fs, nUp, t, op, V, coeff, out, flux[0], kMaxUp);
+}
+
 Split split_of(const fv_mesh* m) {
     return {m->own.p,     m->nbr.p,    m->ownStart.p, m->lowOff.p,
             m->lowFace.p, m->lowOwn.p, m->bOff.p,     m->bFace.p};
 }
 
+template <typename S, typename Op, int CELLS, int THREADS>
+void launch_gather_t(fv_ctx* ctx, const fv_mesh* m, const Op& op, const S* V, S coeff, S* out) {
+    const int nBlocks = (m->nCells + CELLS - 1) / CELLS;
+    const int perXcd = (nBlocks + kXcd - 1) / kXcd;
+    k_gather<S, Op, CELLS, THREADS><<<perXcd * kXcd, THREADS, 0, ctx->stream>>>(
+        m->nCells, nBlocks, perXcd, split_of(m), op, V, coeff, out);
+    FV_LAUNCH_CHECK();
+}
+
 template <typename S, typename Op>
 void launch_gather(fv_ctx* ctx, const fv_mesh* m, const Op& op, const S* V, S coeff, S* out) {
-    const int nBlocks = (m->nCells + kCB - 1) / kCB;
-    const int perXcd = (nBlocks + kXcd - 1) / kXcd;
-    k_gather<S, Op><<<perXcd * kXcd, kCB, 0, ctx->stream>>>(m->nCells, nBlocks, perXcd,
-                                                            split_of(m), op, V, coeff, out);
-    FV_LAUNCH_CHECK();
+    switch (gather_cells(m)) {
+        case 1024: launch_gather_t<S, Op, 1024, 512>(ctx, m, op, V, coeff, out); break;
+        case 512: launch_gather_t<S, Op, 512, 512>(ctx, m, op, V, coeff, out); break;
+        default: launch_gather_t<S, Op, 256, 256>(ctx, m, op, V, coeff, out); break;
+    }
 }
 
 // ---- face-result operators -------------------------------------------------
@@ -382,7 +404,7 @@ __global__ __launch_bounds__(kBlock) void k_correct_bc(int B, const int32_t* __r
 // stencil walk otherwise (same arithmetic, same order).
 template <typename S>
 void run_grad(fv_ctx* ctx, const fv_mesh* m, Geo<S> g, const S* T, const S* Tb, S coeff, S* out) {
-    if (m->upperTri && !m->forceWalk) {
+    if (gather_cells(m)) {
         launch_gather<S>(ctx, m, GradOp<S>{T, g.w, g.Sf, g.Sfb, Tb}, g.V, coeff, out);
         return;
     }
@@ -395,7 +417,7 @@ void run_grad(fv_ctx* ctx, const fv_mesh* m, Geo<S> g, const S* T, const S* Tb, 
 template <typename S>
 void run_div(fv_ctx* ctx, const fv_mesh* m, int scheme, Geo<S> g, const S* phi, const S* T,
              const S* Tb, S coeff, S* out) {
-    if (m->upperTri && !m->forceWalk) {
+    if (gather_cells(m)) {
         if (!T)
             launch_gather<S>(ctx, m, IntegrateOp<S>{phi, m->nInt}, g.V, coeff, out);
         else if (scheme == FV_LINEAR)
@@ -413,7 +435,7 @@ void run_div(fv_ctx* ctx, const fv_mesh* m, int scheme, Geo<S> g, const S* phi, 
 template <typename S>
 void run_lap(fv_ctx* ctx, const fv_mesh* m, Geo<S> g, const int8_t* kind, const S* gamma,
              const S* T, const S* Tb, const S* Tg, S coeff, S* out) {
-    if (m->upperTri && !m->forceWalk) {
+    if (gather_cells(m)) {
         launch_gather<S>(ctx, m,
                          LapOp<S>{T, gamma, g.magSf, g.nodc, g.magSfb, g.dcb, Tb, Tg, kind, m->nInt},
                          g.V, coeff, out);

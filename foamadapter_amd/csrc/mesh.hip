@@ -483,15 +483,15 @@ int fv_mesh_get(const fv_mesh* m, fv_geom which, void* h) {
 
 int fv_mesh_set_gather(fv_mesh* m, int mode) {
     return guarded([&] {
-        FV_REQUIRE(m && (mode == 0 || mode == 1), "bad argument");
-        m->forceWalk = mode == 1;
+        FV_REQUIRE(m && mode >= 0 && mode <= 4, "bad argument");
+        m->gatherMode = mode;
     });
 }
 
 int fv_mesh_get_gather(const fv_mesh* m, int* mode) {
     return guarded([&] {
         FV_REQUIRE(m && mode, "NULL argument");
-        *mode = (m->upperTri && !m->forceWalk) ? 2 : 1;
+        *mode = gather_cells(m) == 0 ? 1 : gather_cells(m) == 256 ? 2 : gather_cells(m) == 512 ? 3 : 4;
     });
 }
 

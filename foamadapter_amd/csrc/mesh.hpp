@@ -41,7 +41,7 @@ struct fv_mesh {
     //     ascending, with their owner
     //   bOff[N+1], bFace[B]: boundary faces of c, ascending
     bool upperTri = false;
-    bool forceWalk = false;  // run the stencil-walk kernels even when upperTri
+    int gatherMode = 0;  // cell-result kernels: 0 default, 1 stencil walk, 2/3/4 staged 256/512/1024
     fvhip::DBuf<int32_t> ownStart, lowOff, lowFace, lowOwn, bOff, bFace;
     MeshF32 f32;
 };
@@ -58,4 +58,18 @@ void build_stencil(fv_mesh* m, const int32_t* h_own, const int32_t* h_nbr,
                    const int32_t* h_faceCells);
 // FP32 copies of the geometry (lazy).
 void ensure_f32(fv_mesh* m);
+// Cells per workgroup of the LDS-staged gather for the cell-result operators,
+// 0 when the per-cell stencil walk runs instead (faces not in upper-triangular
+// order, or forced by fv_mesh_set_gather).
+constexpr int kDefaultGatherCells = 512;
+inline int gather_cells(const fv_mesh* m) {
+    if (!m->upperTri) return 0;
+    switch (m->gatherMode) {
+        case 1: return 0;
+        case 2: return 256;
+        case 3: return 512;
+        case 4: return 1024;
+        default: return kDefaultGatherCells;
+    }
+}
 }  // namespace fvhip

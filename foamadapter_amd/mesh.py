@@ -298,14 +298,15 @@ class Mesh:
         return dst
 
     def set_gather(self, mode: int):
-        """0: LDS-staged face blocks when possible (default); 1: per-cell stencil walk."""
+        """0: LDS-staged face blocks when possible (default); 1: per-cell stencil walk;
+        2/3/4: staged with 256/512 (the default)/1024 cells per workgroup."""
         call("fv_mesh_set_gather", self.handle, int(mode))
 
     @property
     def gather_kernel(self) -> str:
         m = C.c_int()
         call("fv_mesh_get_gather", self.handle, C.byref(m))
-        return {1: "stencil-walk", 2: "staged"}[m.value]
+        return {1: "stencil-walk", 2: "staged-256", 3: "staged", 4: "staged-1024"}[m.value]
 
     def to_device(self, src, kind: str = "cell", oriented: bool = False):
         """Caller-numbered device array -> this mesh's numbering (identity if not renumbered).

@@ -205,10 +205,12 @@ typedef enum {
 int fv_mesh_get(const fv_mesh* mesh, fv_geom which, void* h_out);
 
 /* Kernel choice for the cell-result operators (grad, div, laplacian): mode 0
- * picks the LDS-staged face-block gather when the faces are in upper-triangular
- * order, mode 1 forces the per-cell stencil walk (same arithmetic and order,
- * bit-identical results).  fv_mesh_get_gather reports the kernel in use:
- * 2 staged, 1 stencil walk.  No counterpart in the reference: a tuning knob. */
+ * picks the LDS-staged face-block gather (default workgroup size) when the
+ * faces are in upper-triangular order, mode 1 forces the per-cell stencil walk,
+ * modes 2/3/4 the staged gather with 256/512/1024 cells per workgroup (same
+ * arithmetic and order: bit-identical results).  fv_mesh_get_gather reports the
+ * kernel in use: 1 stencil walk, 2/3/4 staged with 256/512/1024 cells.  No
+ * counterpart in the reference: a tuning knob. */
 int fv_mesh_set_gather(fv_mesh* mesh, int mode);
 int fv_mesh_get_gather(const fv_mesh* mesh, int* mode);
 

@@ -287,10 +287,17 @@ def test_staged_and_walk_kernels_agree(ctx, cases, dt):
         m.set_gather(1)
         assert m.gather_kernel == "stencil-walk"
         walk, ref = _run_all(ctx, m, g, dt)
-        m.set_gather(0)
         for op in staged:
             assert np.array_equal(staged[op], walk[op]), (name, op)
             assert np.array_equal(walk[op], ref[op]), (name, op)
+        # larger workgroups (partial last blocks on these small meshes)
+        for mode, label in ((2, "staged-256"), (4, "staged-1024")):
+            m.set_gather(mode)
+            assert m.gather_kernel == label
+            big, _ = _run_all(ctx, m, g, dt)
+            for op in staged:
+                assert np.array_equal(big[op], walk[op]), (name, label, op)
+        m.set_gather(0)
 
 
 @pytest.mark.parametrize("method", ["morton", "rcm"])
