@@ -138,6 +138,12 @@ _SIGS = {
     "fv_assemble_ddt_laplacian": [_vp, _vp, _d, _vp, _vp, _vp, _vp, _d],
     "fv_set_reference": [_vp, C.c_int, _d],
     "fv_residual": [_vp, _vp, _vp],
+    "fv_compute_rau": [_vp, _vp, _vp],
+    "fv_compute_hbya": [_vp, _vp, _vp, _vp, _vp, _vp],
+    "fv_update_face_velocity": [_vp, _vp, _vp, _vp],
+    "fv_update_velocity": [_vp, _vp, _vp, _vp, _vp, _vp],
+    "fv_constrain_hbya": [_vp, _vp, _vp],
+    "fv_linsys_boundary_coeffs": [_vp, C.POINTER(_vp), C.POINTER(_vp)],
     "fv_solve_cg": [_vp, _vp, C.POINTER(FvSolveOpts), C.POINTER(FvSolverStats)],
     "fv_batch_create": [_vp, C.c_int, C.c_int, C.c_int, _pp],
     "fv_batch_destroy": [_vp],

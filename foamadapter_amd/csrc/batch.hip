@@ -11,10 +11,12 @@
 // own tiling, bit for bit the same alone or in any mix.
 //
 // Precision: the CG runs in S (float: mixed, double: FP64) on a sliced-ELL copy
-// of the matrix; every chunk of iterations ends with an FP64 residual
-// replacement (x += d, r = b - A x in FP64, convergence test in FP64, CG
-// continues from the replaced residual without restarting its direction), so
-// each system stops on its true FP64 relative residual.  A chunk is captured
+// of the matrix; a system whose CG stopped in a chunk (FP32 residual dropped by
+// the trigger factor, estimate at the tolerance, iteration limit) gets an FP64
+// residual replacement at the end of that chunk (x += d, r = b - A x in FP64,
+// convergence test in FP64, CG continues from the replaced residual without
+// restarting its direction), so each system stops on its true FP64 relative
+// residual while the others iterate on.  A chunk is captured
 // once as a hipGraph (per membership) and replayed until every system has
 // stopped.  Members can join and leave between chunks (continuous batching):
 // joining members are started by their own tiles, the others carry on.
@@ -144,9 +146,9 @@ void enqueue_chunk(fv_batch* b) {
     for (int i = 0; i < b->chunk; ++i) {
         kc_spmv<S><<<G, kB, 0, s>>>(g, red);
         kc_update<S><<<G, kB, 0, s>>>(g, red);
-        // the chunk ends after an update: its direction update uses the replaced residual
-        if (i + 1 < b->chunk) kc_direction<S, 0><<<G, kB, 0, s>>>(g, red);
+        kc_direction<S, 0><<<G, kB, 0, s>>>(g, red);
     }
+    // FP64 residual replacement of the systems that stopped in this chunk
     kc_accum<S><<<G, kB, 0, s>>>(g);
     kc_resid<S, 0><<<G, kB, 0, s>>>(g, red);
     kc_replace<S, 0><<<G, kB, 0, s>>>(g, red);

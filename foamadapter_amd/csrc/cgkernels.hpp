@@ -18,6 +18,9 @@
 //   resid      r64 = b - A x (FP64 CSR), partials of r.r and b.b
 //   replace    relative residual from those partials; r = r64, z = D^-1 r,
 //              partials of r.r and r.z (a direction kernel in mode 1 follows)
+// The replacement runs only for systems that stopped (stop == 1: the FP32
+// residual fell by the trigger factor, its estimate reached the tolerance, or
+// the iteration limit); the others go on iterating across chunks.
 // Every dot product is summed by each consuming block from the producer's
 // slots in a fixed order: deterministic, no atomics, no fences.
 #pragma once
@@ -148,8 +151,8 @@ __global__ __launch_bounds__(kCgBlock) void kc_update(CgGrid<S> g, CgPartials re
 
 // MODE 0: after an update (stop on the recurrence estimate, the iteration limit,
 //         NaN or the residual-drop trigger);
-// MODE 1: after a residual replacement (stop on the FP64 residual: 2 converged,
-//         3 non-finite, 1 iteration limit);
+// MODE 1: after a residual replacement, for the replaced systems only (stop on
+//         the FP64 residual: 2 converged, 3 non-finite, 1 iteration limit);
 // MODE 2: after the start (replacement of the initial guess; p = z already).
 template <typename S, int MODE>
 __global__ __launch_bounds__(kCgBlock) void kc_direction(CgGrid<S> g, CgPartials red) {
@@ -160,7 +163,7 @@ __global__ __launch_bounds__(kCgBlock) void kc_direction(CgGrid<S> g, CgPartials
     const CgSys<S>& P = *Pp;
     CgState* st = P.st;
     const int done = st->done;
-    if ((MODE == 0 && done) || (MODE == 1 && done >= 2)) return;
+    if ((MODE == 0 && done) || (MODE == 1 && done != 1)) return;
     double tot[2];
     sum_slots<2>(red.rz, P.tile0, P.ntiles, sh, tot);
     const int it = st->it;
@@ -196,7 +199,7 @@ __global__ __launch_bounds__(kCgBlock) void kc_accum(CgGrid<S> g) {
     int row0, row1, slot;
     if (!cg_locate(g, Pp, row0, row1, slot)) return;
     const CgSys<S>& P = *Pp;
-    if (P.st->stop >= 2) return;
+    if (P.st->stop != 1) return;
     const int r1 = min(row1, P.N);
     for (int i = row0 + threadIdx.x; i < r1; i += kCgBlock) {
         P.x[i] += (double)P.d[i];
@@ -212,7 +215,7 @@ __global__ __launch_bounds__(kCgBlock) void kc_resid(CgGrid<S> g, CgPartials red
     int row0, row1, slot;
     if (!cg_locate(g, Pp, row0, row1, slot)) return;
     const CgSys<S>& P = *Pp;
-    if (!START && P.st->stop >= 2) return;
+    if (!START && P.st->stop != 1) return;
     double rr = 0.0, bb = 0.0;
     const int r1 = min(row1, P.N);
     for (int i = row0 + threadIdx.x; i < r1; i += kCgBlock) {
@@ -238,7 +241,8 @@ __global__ __launch_bounds__(kCgBlock) void kc_replace(CgGrid<S> g, CgPartials r
     if (!cg_locate(g, Pp, row0, row1, slot)) return;
     const CgSys<S>& P = *Pp;
     CgState* st = P.st;
-    if (!START && st->stop >= 2) return;
+    const int stop = st->stop;
+    if (!START && stop != 1) return;
     double tot[2];
     sum_slots<2>(red.rb, P.tile0, P.ntiles, sh, tot);
     double bn;
@@ -251,6 +255,7 @@ __global__ __launch_bounds__(kCgBlock) void kc_replace(CgGrid<S> g, CgPartials r
     const double res = sqrt(tot[0]) / bn;
     if (row0 == 0 && threadIdx.x == 0) {
         st->res = res;
+        st->done = stop;  // tells the mode-1 direction kernel that this system was replaced
         if (START) {
             st->bnorm = bn;
             st->res0 = res;

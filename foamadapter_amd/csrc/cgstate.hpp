@@ -16,8 +16,9 @@ constexpr int kCgBlock = 256;
 //         (recurrence estimate at the tolerance, iteration limit, NaN, or the
 //         residual-drop trigger of the mixed solver), 2 converged in FP64,
 //         3 failed (non-finite FP64 residual)
-//   done  copy of stop taken by the SpMV kernel at the start of an iteration;
-//         the rest of the iteration reads it
+//   done  copy of stop taken by the SpMV kernel at the start of an iteration
+//         (the rest of the iteration reads it) and by the replacement kernel
+//         (the direction kernel after it reads it)
 //   it    completed iterations; the SpMV kernel of iteration k sets it = k + 1
 //   rz    rz[it & 1] holds r_it.z_it, so beta = rz[it&1] / rz[(it-1)&1]; a
 //         residual replacement overwrites rz[it & 1] with the replaced value

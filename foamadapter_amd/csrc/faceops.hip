@@ -180,28 +180,55 @@ struct Split {
     const int32_t *own, *nbr, *ownStart, *lowOff, *lowFace, *lowOwn, *bOff, *bFace;
 };
 
+// An operator gives each face a contribution of NC components.  The staged
+// gather keeps a face's NS-component state in LDS (state/bstate) and expands it
+// to the contribution when it accumulates (contrib/bcontrib): the gradient keeps
+// the face value (8 bytes instead of 24) and multiplies by Sf when it adds.
 template <typename S>
 struct GradOp {
-    static constexpr int NC = 3;
+    static constexpr int NC = 3, NS = 1;
     const S *T, *w, *Sf, *Sfb, *Tb;
-    __device__ void face(int f, int o, int n, S* v) const {
+    __device__ void state(int f, int o, int n, S* s) const {
         const S TP = T[o], TN = T[n];
-        const S tf = w[f] * (TP - TN) + TN;
-        v[0] = Sf[3 * (int64_t)f] * tf;
-        v[1] = Sf[3 * (int64_t)f + 1] * tf;
-        v[2] = Sf[3 * (int64_t)f + 2] * tf;
+        s[0] = w[f] * (TP - TN) + TN;
     }
-    __device__ void bface(int b, int, S* v) const {
-        const S tb = Tb[b];
-        v[0] = Sfb[3 * b] * tb;
-        v[1] = Sfb[3 * b + 1] * tb;
-        v[2] = Sfb[3 * b + 2] * tb;
+    __device__ void contrib(int f, const S* s, S* v) const {
+        v[0] = Sf[3 * (int64_t)f] * s[0];
+        v[1] = Sf[3 * (int64_t)f + 1] * s[0];
+        v[2] = Sf[3 * (int64_t)f + 2] * s[0];
+    }
+    __device__ void bstate(int b, int, S* s) const { s[0] = Tb[b]; }
+    __device__ void bcontrib(int b, const S* s, S* v) const {
+        v[0] = Sfb[3 * b] * s[0];
+        v[1] = Sfb[3 * b + 1] * s[0];
+        v[2] = Sfb[3 * b + 2] * s[0];
+    }
+    __device__ void face(int f, int o, int n, S* v) const {
+        S s[NS];
+        state(f, o, n, s);
+        contrib(f, s, v);
+    }
+    __device__ void bface(int b, int c, S* v) const {
+        S s[NS];
+        bstate(b, c, s);
+        bcontrib(b, s, v);
     }
 };
 
+// operators whose state is the contribution itself
+template <typename S, typename D>
+struct PlainOp {
+    static constexpr int NC = 1, NS = 1;
+    __device__ void state(int f, int o, int n, S* s) const {
+        static_cast<const D*>(this)->face(f, o, n, s);
+    }
+    __device__ void contrib(int, const S* s, S* v) const { v[0] = s[0]; }
+    __device__ void bstate(int b, int c, S* s) const { static_cast<const D*>(this)->bface(b, c, s); }
+    __device__ void bcontrib(int, const S* s, S* v) const { v[0] = s[0]; }
+};
+
 template <typename S, int SCHEME>  // 0 linear, 1 upwind
-struct DivOp {
-    static constexpr int NC = 1;
+struct DivOp : PlainOp<S, DivOp<S, SCHEME>> {
     const S *T, *w, *phi, *Tb;
     int F;
     __device__ void face(int f, int o, int n, S* v) const {
@@ -214,8 +241,7 @@ struct DivOp {
 };
 
 template <typename S>
-struct IntegrateOp {  // surfaceIntegrate(phi): div with a unit field
-    static constexpr int NC = 1;
+struct IntegrateOp : PlainOp<S, IntegrateOp<S>> {  // surfaceIntegrate(phi): div with a unit field
     const S* phi;
     int F;
     __device__ void face(int f, int, int, S* v) const { v[0] = phi[f]; }
@@ -223,8 +249,7 @@ struct IntegrateOp {  // surfaceIntegrate(phi): div with a unit field
 };
 
 template <typename S>
-struct LapOp {
-    static constexpr int NC = 1;
+struct LapOp : PlainOp<S, LapOp<S>> {
     const S *T, *gamma, *magSf, *nodc, *magSfb, *dcb, *Tb, *Tg;
     const int8_t* kind;
     int F;
@@ -240,10 +265,8 @@ struct LapOp {
 template <typename S, typename Op>
 __device__ inline void gather_cell(int c, int fs, int nUp, const Split& t, const Op& op,
                                    const S* __restrict__ V, S coeff, S* __restrict__ out,
-                                   const S* flux0, int stride) {
-    constexpr int NC = Op::NC;
-    const S* flux[NC];
-    for (int j = 0; j < NC; ++j) flux[j] = flux0 + j * stride;
+                                   const S* st0, int stride) {
+    constexpr int NC = Op::NC, NS = Op::NS;
     S acc[NC];
     for (int j = 0; j < NC; ++j) acc[j] = S(0);
     const int l1 = t.lowOff[c + 1];
@@ -252,7 +275,9 @@ __device__ inline void gather_cell(int c, int fs, int nUp, const Split& t, const
         S v[NC];
         const int i = f - fs;
         if (i >= 0 && i < nUp) {
-            for (int j = 0; j < NC; ++j) v[j] = flux[j][i];
+            S sv[NS];
+            for (int j = 0; j < NS; ++j) sv[j] = st0[j * stride + i];
+            op.contrib(f, sv, v);
         } else {
             op.face(f, t.lowOwn[k], c, v);
         }
@@ -263,7 +288,9 @@ __device__ inline void gather_cell(int c, int fs, int nUp, const Split& t, const
         S v[NC];
         const int i = f - fs;
         if (i < nUp) {
-            for (int j = 0; j < NC; ++j) v[j] = flux[j][i];
+            S sv[NS];
+            for (int j = 0; j < NS; ++j) sv[j] = st0[j * stride + i];
+            op.contrib(f, sv, v);
         } else {
             op.face(f, c, t.nbr[f], v);
         }
@@ -279,27 +306,138 @@ __device__ inline void gather_cell(int c, int fs, int nUp, const Split& t, const
     for (int j = 0; j < NC; ++j) out[(int64_t)NC * c + j] = coeff * (acc[j] / vol);
 }
 
+struct Lists {
+    const int32_t *extOff, *extFace, *extOwn, *bndOff, *bndFace;
+    const uint16_t *extCell, *bndCell;
+};
+
+constexpr int kMaxLow = 8;  // lower faces per cell from inside the workgroup, in LDS
+
+// A workgroup of CELLS cells.  Phase 1 computes, with coalesced loads, every
+// face contribution the workgroup needs into LDS -- its own faces (streamed),
+// its external lower faces and its boundary faces (from the per-workgroup
+// lists) -- and builds each cell's face lists in LDS: the own-face range from
+// the owner sequence, the lower faces from inside the workgroup by LDS atomics
+// (then sorted, so the order is fixed), the external and boundary ranges from
+// the lists (sorted by cell).  Phase 2 accumulates each cell in ascending face
+// order (external lower, inner lower, own, boundary) and divides by V.  A
+// workgroup whose faces do not fit the LDS arrays takes the per-cell path of
+// gather_cell instead; both give the same bits.
 template <typename S, typename Op, int CELLS, int THREADS>
-__global__ __launch_bounds__(THREADS) void k_gather(int N, int nBlocks, int perXcd, Split t, Op op,
-                                                    const S* __restrict__ V, S coeff,
-                                                    S* __restrict__ out) {
-    constexpr int NC = Op::NC;
+__global__ __launch_bounds__(THREADS) void k_gather(int N, int nBlocks, int perXcd, Split t,
+                                                    Lists L, Op op, const S* __restrict__ V,
+                                                    S coeff, S* __restrict__ out) {
+    constexpr int NC = Op::NC, NS = Op::NS;
     constexpr int kMaxUp = max_up<CELLS>();
-    __shared__ S flux[NC][kMaxUp];
+    constexpr int kMaxExt = CELLS / 2;
+    constexpr int kMaxBnd = CELLS / 4;
+    __shared__ S fluxU[NS][kMaxUp];
+    __shared__ S fluxE[NS][kMaxExt];
+    __shared__ S fluxB[NS][kMaxBnd];
+    __shared__ uint16_t upS[CELLS], upE[CELLS], exS[CELLS], exE[CELLS], bS[CELLS], bE[CELLS];
+    __shared__ uint16_t lowL[CELLS][kMaxLow];
+    __shared__ int lowN[CELLS];
+    __shared__ int overflow;
     const int lb = (blockIdx.x % kXcd) * perXcd + blockIdx.x / kXcd;
     if (lb >= nBlocks) return;
     const int c0 = lb * CELLS, c1 = min(c0 + CELLS, N);
     const int fs = t.ownStart[c0], fe = t.ownStart[c1];
-    const int nUp = min(fe - fs, kMaxUp);
+    const int nF = fe - fs;
+    const int e0 = L.extOff[lb], e1 = L.extOff[lb + 1];
+    const int b0 = L.bndOff[lb], b1 = L.bndOff[lb + 1];
+    const bool fits = nF <= kMaxUp && e1 - e0 <= kMaxExt && b1 - b0 <= kMaxBnd;
+    for (int l = threadIdx.x; l < CELLS; l += THREADS) {
+        upS[l] = upE[l] = exS[l] = exE[l] = bS[l] = bE[l] = 0;
+        lowN[l] = 0;
+    }
+    if (threadIdx.x == 0) overflow = fits ? 0 : 1;
+    __syncthreads();
+    const int nUp = min(nF, kMaxUp);
     for (int i = threadIdx.x; i < nUp; i += THREADS) {
         const int f = fs + i;
-        S v[NC];
-        op.face(f, t.own[f], t.nbr[f], v);
-        for (int j = 0; j < NC; ++j) flux[j][i] = v[j];
+        const int o = t.own[f], n = t.nbr[f];
+        S sv[NS];
+        op.state(f, o, n, sv);
+        for (int j = 0; j < NS; ++j) fluxU[j][i] = sv[j];
+        if (fits) {
+            const int ol = o - c0;
+            if (i == 0 || t.own[f - 1] != o) upS[ol] = (uint16_t)i;
+            if (i == nF - 1 || t.own[f + 1] != o) upE[ol] = (uint16_t)(i + 1);
+            if (n < c1) {
+                const int nl = n - c0;
+                const int slot = atomicAdd(&lowN[nl], 1);
+                if (slot < kMaxLow)
+                    lowL[nl][slot] = (uint16_t)i;
+                else
+                    overflow = 1;
+            }
+        }
+    }
+    if (fits) {
+        for (int k = e0 + threadIdx.x; k < e1; k += THREADS) {
+            const int cl = L.extCell[k];
+            S sv[NS];
+            op.state(L.extFace[k], L.extOwn[k], c0 + cl, sv);
+            for (int j = 0; j < NS; ++j) fluxE[j][k - e0] = sv[j];
+            if (k == e0 || L.extCell[k - 1] != cl) exS[cl] = (uint16_t)(k - e0);
+            if (k == e1 - 1 || L.extCell[k + 1] != cl) exE[cl] = (uint16_t)(k - e0 + 1);
+        }
+        for (int k = b0 + threadIdx.x; k < b1; k += THREADS) {
+            const int cl = L.bndCell[k];
+            S sv[NS];
+            op.bstate(L.bndFace[k], c0 + cl, sv);
+            for (int j = 0; j < NS; ++j) fluxB[j][k - b0] = sv[j];
+            if (k == b0 || L.bndCell[k - 1] != cl) bS[cl] = (uint16_t)(k - b0);
+            if (k == b1 - 1 || L.bndCell[k + 1] != cl) bE[cl] = (uint16_t)(k - b0 + 1);
+        }
     }
     __syncthreads();
-    for (int c = c0 + threadIdx.x; c < c1; c += THREADS)
-        gather_cell<S, Op>(c, fs, nUp, t, op, V, coeff, out, flux[0], kMaxUp);
+    if (overflow) {
+        for (int c = c0 + threadIdx.x; c < c1; c += THREADS)
+            gather_cell<S, Op>(c, fs, nUp, t, op, V, coeff, out, fluxU[0], kMaxUp);
+        return;
+    }
+    for (int c = c0 + threadIdx.x; c < c1; c += THREADS) {
+        const int l = c - c0;
+        S acc[NC];
+        for (int j = 0; j < NC; ++j) acc[j] = S(0);
+        S sv[NS], v[NC];
+        for (int k = exS[l]; k < exE[l]; ++k) {
+            for (int j = 0; j < NS; ++j) sv[j] = fluxE[j][k];
+            op.contrib(L.extFace[e0 + k], sv, v);
+            for (int j = 0; j < NC; ++j) acc[j] -= v[j];
+        }
+        // the inner lower faces arrived in any order: sort this cell's list
+        const int n = lowN[l];
+        uint16_t* lst = lowL[l];
+        for (int a = 1; a < n; ++a) {
+            const uint16_t v = lst[a];
+            int b = a - 1;
+            while (b >= 0 && lst[b] > v) {
+                lst[b + 1] = lst[b];
+                --b;
+            }
+            lst[b + 1] = v;
+        }
+        for (int q = 0; q < n; ++q) {
+            const int i = lst[q];
+            for (int j = 0; j < NS; ++j) sv[j] = fluxU[j][i];
+            op.contrib(fs + i, sv, v);
+            for (int j = 0; j < NC; ++j) acc[j] -= v[j];
+        }
+        for (int i = upS[l]; i < upE[l]; ++i) {
+            for (int j = 0; j < NS; ++j) sv[j] = fluxU[j][i];
+            op.contrib(fs + i, sv, v);
+            for (int j = 0; j < NC; ++j) acc[j] += v[j];
+        }
+        for (int k = bS[l]; k < bE[l]; ++k) {
+            for (int j = 0; j < NS; ++j) sv[j] = fluxB[j][k];
+            op.bcontrib(L.bndFace[b0 + k], sv, v);
+            for (int j = 0; j < NC; ++j) acc[j] += v[j];
+        }
+        const S vol = V[c];
+        for (int j = 0; j < NC; ++j) out[(int64_t)NC * c + j] = coeff * (acc[j] / vol);
+    }
 }
 
 Split split_of(const fv_mesh* m) {
@@ -309,10 +447,14 @@ Split split_of(const fv_mesh* m) {
 
 template <typename S, typename Op, int CELLS, int THREADS>
 void launch_gather_t(fv_ctx* ctx, const fv_mesh* m, const Op& op, const S* V, S coeff, S* out) {
+    // the lists are built once per mesh and workgroup size (a cache on the mesh)
+    const GatherLists& gl = gather_lists(const_cast<fv_mesh*>(m), CELLS);
+    const Lists L{gl.extOff.p, gl.extFace.p, gl.extOwn.p, gl.bndOff.p, gl.bndFace.p,
+                  gl.extCell.p, gl.bndCell.p};
     const int nBlocks = (m->nCells + CELLS - 1) / CELLS;
     const int perXcd = (nBlocks + kXcd - 1) / kXcd;
     k_gather<S, Op, CELLS, THREADS><<<perXcd * kXcd, THREADS, 0, ctx->stream>>>(
-        m->nCells, nBlocks, perXcd, split_of(m), op, V, coeff, out);
+        m->nCells, nBlocks, perXcd, split_of(m), L, op, V, coeff, out);
     FV_LAUNCH_CHECK();
 }
 
@@ -419,11 +561,11 @@ void run_div(fv_ctx* ctx, const fv_mesh* m, int scheme, Geo<S> g, const S* phi, 
              const S* Tb, S coeff, S* out) {
     if (gather_cells(m)) {
         if (!T)
-            launch_gather<S>(ctx, m, IntegrateOp<S>{phi, m->nInt}, g.V, coeff, out);
+            launch_gather<S>(ctx, m, IntegrateOp<S>{{}, phi, m->nInt}, g.V, coeff, out);
         else if (scheme == FV_LINEAR)
-            launch_gather<S>(ctx, m, DivOp<S, 0>{T, g.w, phi, Tb, m->nInt}, g.V, coeff, out);
+            launch_gather<S>(ctx, m, DivOp<S, 0>{{}, T, g.w, phi, Tb, m->nInt}, g.V, coeff, out);
         else
-            launch_gather<S>(ctx, m, DivOp<S, 1>{T, g.w, phi, Tb, m->nInt}, g.V, coeff, out);
+            launch_gather<S>(ctx, m, DivOp<S, 1>{{}, T, g.w, phi, Tb, m->nInt}, g.V, coeff, out);
         return;
     }
     const int N = m->nCells;
@@ -437,7 +579,7 @@ void run_lap(fv_ctx* ctx, const fv_mesh* m, Geo<S> g, const int8_t* kind, const 
              const S* T, const S* Tb, const S* Tg, S coeff, S* out) {
     if (gather_cells(m)) {
         launch_gather<S>(ctx, m,
-                         LapOp<S>{T, gamma, g.magSf, g.nodc, g.magSfb, g.dcb, Tb, Tg, kind, m->nInt},
+                         LapOp<S>{{}, T, gamma, g.magSf, g.nodc, g.magSfb, g.dcb, Tb, Tg, kind, m->nInt},
                          g.V, coeff, out);
         return;
     }

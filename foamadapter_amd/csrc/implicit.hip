@@ -60,7 +60,7 @@ __global__ __launch_bounds__(kBlock) void k_laplacian(
     const double* __restrict__ dcb, const int8_t* __restrict__ kind,
     const double* __restrict__ gamma, const double* __restrict__ Tb,
     const double* __restrict__ Tg, double coeff, double* __restrict__ values,
-    double* __restrict__ rhs) {
+    double* __restrict__ rhs, double* __restrict__ bMat, double* __restrict__ bRhs) {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= N) return;
     const int row = rowOffs[c];
@@ -69,14 +69,21 @@ __global__ __launch_bounds__(kBlock) void k_laplacian(
     for (int k = st.off[c]; k < e1; ++k) {
         const int e = st.idx[k];
         if (e >= F) {
+            // boundary coefficients: the cell is the only writer of its faces
             const int b = e - F;
             const double gm = gamma[e] * magSfb[b];
             if (kind[b] == FV_BC_FIXED_GRADIENT) {
-                bsum -= gm * (Tg ? Tg[b] : 0.0);
+                const double g = gm * (Tg ? Tg[b] : 0.0);
+                bsum -= g;
+                bMat[b] += 0.0;
+                bRhs[b] += coeff * g;
             } else {
                 const double ic = gm * dcb[b];
+                const double it = ic * Tb[b];
                 dsum -= ic;
-                bsum -= ic * Tb[b];
+                bsum -= it;
+                bMat[b] += coeff * ic;
+                bRhs[b] += coeff * it;
             }
         } else {
             const int f = e >= 0 ? e : ~e;
@@ -100,7 +107,7 @@ __global__ __launch_bounds__(kBlock) void k_ddt_laplacian(
     const double* __restrict__ dcb, const int8_t* __restrict__ kind,
     const double* __restrict__ gamma, const double* __restrict__ Tb,
     const double* __restrict__ Tg, double coeff, double* __restrict__ values,
-    double* __restrict__ rhs) {
+    double* __restrict__ rhs, double* __restrict__ bMat, double* __restrict__ bRhs) {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= N) return;
     const int row = rowOffs[c];
@@ -112,11 +119,17 @@ __global__ __launch_bounds__(kBlock) void k_ddt_laplacian(
             const int b = e - F;
             const double gm = gamma[e] * magSfb[b];
             if (kind[b] == FV_BC_FIXED_GRADIENT) {
-                bsum -= gm * (Tg ? Tg[b] : 0.0);
+                const double g = gm * (Tg ? Tg[b] : 0.0);
+                bsum -= g;
+                bMat[b] = 0.0;
+                bRhs[b] = coeff * g;
             } else {
                 const double ic = gm * dcb[b];
+                const double it = ic * Tb[b];
                 dsum -= ic;
-                bsum -= ic * Tb[b];
+                bsum -= it;
+                bMat[b] = coeff * ic;
+                bRhs[b] = coeff * it;
             }
         } else {
             const int f = e >= 0 ? e : ~e;
@@ -346,8 +359,12 @@ int fv_linsys_create(fv_sparsity* sp, fv_linsys** out) {
         ls->sp = sp;
         ls->values.alloc(ctx, sp->nnz);
         ls->rhs.alloc(ctx, sp->nRows);
+        ls->bMat.alloc(ctx, std::max<int32_t>(1, sp->mesh->nB));
+        ls->bRhs.alloc(ctx, std::max<int32_t>(1, sp->mesh->nB));
         FV_HIP(hipMemsetAsync(ls->values.p, 0, sp->nnz * sizeof(double), ctx->stream));
         FV_HIP(hipMemsetAsync(ls->rhs.p, 0, sp->nRows * sizeof(double), ctx->stream));
+        FV_HIP(hipMemsetAsync(ls->bMat.p, 0, ls->bMat.n * sizeof(double), ctx->stream));
+        FV_HIP(hipMemsetAsync(ls->bRhs.p, 0, ls->bRhs.n * sizeof(double), ctx->stream));
         *out = ls;
     });
     if (rc != FV_OK) delete ls;
@@ -367,6 +384,8 @@ int fv_linsys_zero(fv_linsys* ls) {
         hipStream_t s = mesh_of(ls)->ctx->stream;
         FV_HIP(hipMemsetAsync(ls->values.p, 0, ls->values.n * sizeof(double), s));
         FV_HIP(hipMemsetAsync(ls->rhs.p, 0, ls->rhs.n * sizeof(double), s));
+        FV_HIP(hipMemsetAsync(ls->bMat.p, 0, ls->bMat.n * sizeof(double), s));
+        FV_HIP(hipMemsetAsync(ls->bRhs.p, 0, ls->bRhs.n * sizeof(double), s));
     });
 }
 
@@ -419,7 +438,7 @@ int fv_assemble_laplacian(fv_linsys* ls, const void* gamma, const fv_bc* bc, con
         k_laplacian<<<grid_for(m->nCells, kBlock), kBlock, 0, m->ctx->stream>>>(
             m->nCells, m->nInt, st, sp->rowOffs.p, sp->diagOffset.p, m->magSf.p, m->nodc.p,
             m->magSfb.p, m->dcb.p, bc->faceKind.p, (const double*)gamma, (const double*)Tb,
-            (const double*)Tg, coeff, ls->values.p, ls->rhs.p);
+            (const double*)Tg, coeff, ls->values.p, ls->rhs.p, ls->bMat.p, ls->bRhs.p);
         FV_LAUNCH_CHECK();
     });
 }
@@ -437,7 +456,7 @@ int fv_assemble_ddt_laplacian(fv_linsys* ls, const void* Told, double dt, const 
             m->nCells, m->nInt, st, sp->rowOffs.p, sp->diagOffset.p, m->V.p, (const double*)Told,
             1.0 / dt, m->magSf.p, m->nodc.p, m->magSfb.p, m->dcb.p, bc->faceKind.p,
             (const double*)gamma, (const double*)Tb, (const double*)Tg, coeff, ls->values.p,
-            ls->rhs.p);
+            ls->rhs.p, ls->bMat.p, ls->bRhs.p);
         FV_LAUNCH_CHECK();
     });
 }

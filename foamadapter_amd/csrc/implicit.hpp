@@ -35,5 +35,9 @@ struct fv_solver_cache {
 struct fv_linsys {
     fv_sparsity* sp = nullptr;
     fvhip::DBuf<double> values, rhs;
+    // boundary coefficients of the assembled laplacian terms, per boundary face:
+    // the face flux is bRhs - bMat * x_P (NeoN's la::BoundaryCoefficients,
+    // read by updateFaceVelocity, pressureVelocityCoupling.cpp:180-192)
+    fvhip::DBuf<double> bMat, bRhs;
     std::unique_ptr<fv_solver_cache> cg[2];  // FP64 CG, FP32 CG in FP64 refinement
 };

@@ -250,6 +250,51 @@ void build_stencil(fv_mesh* m, const int32_t* h_own, const int32_t* h_nbr,
     FV_HIP(hipStreamSynchronize(m->ctx->stream));
 }
 
+const GatherLists& gather_lists(fv_mesh* m, int cells) {
+    const int slot = cells == 256 ? 0 : cells == 512 ? 1 : 2;
+    GatherLists& L = m->glists[slot];
+    if (L.cells == cells) return L;
+    FV_REQUIRE(m->upperTri, "staged gather lists need upper-triangular face order");
+    const int64_t N = m->nCells, F = m->nInt, B = m->nB;
+    std::vector<int32_t> lowOff(N + 1), lowFace(F), lowOwn(F), bOff(N + 1), bFace(B);
+    m->lowOff.download(lowOff.data());
+    m->lowFace.download(lowFace.data());
+    m->lowOwn.download(lowOwn.data());
+    m->bOff.download(bOff.data());
+    m->bFace.download(bFace.data());
+    const int64_t nBlocks = (N + cells - 1) / cells;
+    std::vector<int32_t> eo(nBlocks + 1, 0), ef, ew, bo(nBlocks + 1, 0), bf;
+    std::vector<uint16_t> ec, bc;
+    for (int64_t blk = 0; blk < nBlocks; ++blk) {
+        const int64_t c0 = blk * cells, c1 = std::min<int64_t>(c0 + cells, N);
+        for (int64_t c = c0; c < c1; ++c) {
+            // a cell's lower faces ascend; those owned before the block come first
+            for (int32_t k = lowOff[c]; k < lowOff[c + 1] && lowOwn[k] < c0; ++k) {
+                ef.push_back(lowFace[k]);
+                ew.push_back(lowOwn[k]);
+                ec.push_back((uint16_t)(c - c0));
+            }
+            for (int32_t k = bOff[c]; k < bOff[c + 1]; ++k) {
+                bf.push_back(bFace[k]);
+                bc.push_back((uint16_t)(c - c0));
+            }
+        }
+        eo[blk + 1] = (int32_t)ef.size();
+        bo[blk + 1] = (int32_t)bf.size();
+    }
+    fv_ctx* ctx = m->ctx;
+    L.extOff.upload(ctx, eo.data(), eo.size());
+    L.extFace.upload(ctx, ef.data(), ef.size());
+    L.extOwn.upload(ctx, ew.data(), ew.size());
+    L.extCell.upload(ctx, ec.data(), ec.size());
+    L.bndOff.upload(ctx, bo.data(), bo.size());
+    L.bndFace.upload(ctx, bf.data(), bf.size());
+    L.bndCell.upload(ctx, bc.data(), bc.size());
+    FV_HIP(hipStreamSynchronize(ctx->stream));
+    L.cells = cells;
+    return L;
+}
+
 void ensure_f32(fv_mesh* m) {
     if (m->f32.ready) return;
     fv_ctx* c = m->ctx;

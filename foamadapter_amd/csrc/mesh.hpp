@@ -23,6 +23,18 @@ struct MeshF32 {
     bool ready = false;
 };
 
+// Per-workgroup face lists of the LDS-staged gather for one workgroup size
+// (`cells` consecutive cells per workgroup, block b = cells [b*cells, ...)):
+//   extOff[nBlocks+1]; extFace, extOwn, extCell (cell - c0): the block's lower
+//     faces whose owner lies in an earlier block, sorted by (cell, face)
+//   bndOff[nBlocks+1]; bndFace (boundary face b), bndCell: the block's boundary
+//     faces sorted by (cell, b)
+struct GatherLists {
+    int cells = 0;  // 0: not built
+    fvhip::DBuf<int32_t> extOff, extFace, extOwn, bndOff, bndFace;
+    fvhip::DBuf<uint16_t> extCell, bndCell;
+};
+
 struct fv_mesh {
     fv_ctx* ctx = nullptr;
     int32_t nCells = 0, nInt = 0, nB = 0, nPatches = 0;
@@ -43,6 +55,7 @@ struct fv_mesh {
     bool upperTri = false;
     int gatherMode = 0;  // cell-result kernels: 0 default, 1 stencil walk, 2/3/4 staged 256/512/1024
     fvhip::DBuf<int32_t> ownStart, lowOff, lowFace, lowOwn, bOff, bFace;
+    GatherLists glists[3];  // workgroups of 256, 512, 1024 cells (built on first use)
     MeshF32 f32;
 };
 
@@ -58,6 +71,8 @@ void build_stencil(fv_mesh* m, const int32_t* h_own, const int32_t* h_nbr,
                    const int32_t* h_faceCells);
 // FP32 copies of the geometry (lazy).
 void ensure_f32(fv_mesh* m);
+// The staged gather's per-workgroup lists for `cells` cells per workgroup.
+const GatherLists& gather_lists(fv_mesh* m, int cells);
 // Cells per workgroup of the LDS-staged gather for the cell-result operators,
 // 0 when the per-cell stencil walk runs instead (faces not in upper-triangular
 // order, or forced by fv_mesh_set_gather).

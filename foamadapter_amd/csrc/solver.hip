@@ -140,18 +140,21 @@ CgState run_cg(fv_ctx* ctx, CgWork<double>& w, double tol, int maxit, int check)
     hipStream_t s = ctx->stream;
     w.start(s, tol, maxit, 0.0);
     FV_LAUNCH_CHECK();
+    // iterations follow the start without a round trip (a system converged at
+    // the start turns them into no-ops); poll early for short solves, then
+    // every `check` iterations
     CgState h{};
     int launched = 0, batch = std::min(4, check);
-    while (true) {
-        h = w.state(s);
-        if (h.stop || launched >= maxit) break;
-        // poll early for short solves, then every `check` iterations
+    while (launched < maxit) {
         const int n = std::min(batch, maxit - launched);
         batch = std::min(2 * batch, check);
         for (int i = 0; i < n; ++i) w.iteration(s, true);
         FV_LAUNCH_CHECK();
         launched += n;
+        h = w.state(s);
+        if (h.stop) break;
     }
+    if (launched == 0) h = w.state(s);
     return h;
 }
 
@@ -175,22 +178,21 @@ IrOut run_mixed(fv_ctx* ctx, CgWork<float>& w, double tol, int maxit, int chunk,
     w.start(s, tol, maxit, delta);
     FV_LAUNCH_CHECK();
     IrOut out{0, 0, 0, 0};
-    CgState h = w.state(s);
-    out.init_res = h.res0;
-    out.res = h.res;
+    CgState h{};  // the first chunk follows the start without a round trip
     int batch = std::min(8, chunk);
     while (h.stop < 2 && h.it < maxit && out.outer < max_outer) {
-        // short chunks first (short solves end early), then `chunk` iterations;
-        // the chunk ends after an update: its direction update uses the replaced residual
+        // short chunks first (short solves end early), then `chunk` iterations
         const int n = std::min(batch, maxit - h.it);
         batch = std::min(2 * batch, chunk);
-        for (int i = 0; i < n; ++i) w.iteration(s, i + 1 < n);
-        w.replacement(s);
+        for (int i = 0; i < n; ++i) w.iteration(s, true);
+        w.replacement(s);  // only once the FP32 CG stopped (trigger, estimate, limit)
         FV_LAUNCH_CHECK();
         h = w.state(s);
         ++out.outer;
-        out.res = h.res;
     }
+    if (out.outer == 0) h = w.state(s);
+    out.init_res = h.res0;
+    out.res = h.res;
     out.iters = h.it;
     return out;
 }

@@ -342,6 +342,37 @@ typedef struct {
  * the tolerance is not reached. */
 int fv_solve_cg(fv_linsys* ls, void* d_x, const fv_solve_opts* opts, fv_solver_stats* stats);
 
+/* ---- pressure-velocity coupling (src/algorithms/pressureVelocityCoupling.cpp) ---
+ * FP64.  The momentum matrix is the scalar system ls (the reference reads
+ * component 0 of its Vec3 coefficients, :54-57); Vec3 fields are xyz
+ * interleaved [3N] / [3B]. */
+/* computeRAU (:38-61): rAU = V / diag(A); boundary values extrapolated into
+ * d_rau_b [B] (may be NULL). */
+int fv_compute_rau(fv_linsys* ls, void* d_rau, void* d_rau_b);
+/* computeRAUandHByA (:63-119): HbyA = (b3 - sum_nb A(c,nb) U_nb) * rAU / V with
+ * the sum in ascending face order (a deterministic gather in place of the
+ * reference's atomics); d_b3 is the momentum right-hand side [3N]; boundary
+ * values extrapolated into d_hbya_b [3B] (may be NULL). */
+int fv_compute_hbya(fv_linsys* ls, const void* d_b3, const void* d_U, const void* d_rau,
+                    void* d_hbya, void* d_hbya_b);
+/* updateFaceVelocity (:122-195) on the assembled pressure system ls:
+ * phi_f = pred_f - (A(N,P) p_N - A(P,N) p_P) on internal faces and
+ * phi_b = pred_b - (bRhs_b - bMat_b p_P) on boundary faces, where bMat/bRhs are
+ * the boundary coefficients the laplacian assembly of ls recorded (the face
+ * flux of the laplacian term is bRhs - bMat p_P).  Surface arrays [F+B]. */
+int fv_update_face_velocity(fv_linsys* ls, const void* d_phi_pred, const void* d_p, void* d_phi);
+/* updateVelocity (:197-211): U = HbyA - rAU * gradP (gradP from fv_grad_gauss). */
+int fv_update_velocity(fv_ctx* ctx, const fv_mesh* mesh, const void* d_hbya, const void* d_rau,
+                       const void* d_gradp, void* d_U);
+/* constrainHbyA (:14-36): HbyA_b = U_b on the faces of patches whose velocity
+ * condition is not assignable (fixedValue); other faces keep their value. */
+int fv_constrain_hbya(const fv_bc* u_bc, const void* d_Ub, void* d_hbya_b);
+/* The boundary coefficients [B] the laplacian assembly of ls recorded
+ * (la::BoundaryCoefficients{matrixValues, rhsValues}, read at :180-192);
+ * zeroed by fv_linsys_zero, accumulated by fv_assemble_laplacian, overwritten
+ * by fv_assemble_ddt_laplacian. */
+int fv_linsys_boundary_coeffs(fv_linsys* ls, void** d_bmat, void** d_brhs);
+
 /* ---- batched solves (many independent systems in one launch sequence) ----
  * No counterpart in the reference (each NeoN solve is a separate Ginkgo call);
  * this is the batching layer the served and batched workloads run on.  A batch

@@ -27,6 +27,14 @@
 //                             254-262), symmetry and row sums.
 //   Jacobi-PCG .............. scipy.sparse.linalg.spsolve on small meshes and the
 //                             explicit residual ||b-Ax||/||b|| <= tol.
+//   rAU / HbyA .............. the matrix form rAU = V/diag(A), HbyA = (b - (A -
+//                             diag A) U) rAU/V evaluated with scipy.
+//   face/cell velocity update the corrected flux of a solved pressure equation
+//                             is divergence-free (surfaceIntegrate(phi) ~ 0);
+//                             U = HbyA - rAU grad(p) against the grad oracle.
+//                             The boundary coefficients (NeoN's
+//                             BoundaryCoefficients, NeoN absent) are "parity
+//                             unpinned" beyond that conservation identity.
 //
 // Reference files followed:
 //   src/datastructures/meshAdapter.cpp:59-136   mesh arrays, flattened boundary
@@ -36,6 +44,9 @@
 //   include/FoamAdapter/datastructures/expression.hpp:95-111,181-220
 //                                                SetReference, diag, A.x-b
 //   src/compatibility/fvSolution.cpp:19-159     PCG + point Jacobi, relTol, maxIter
+//   src/algorithms/pressureVelocityCoupling.cpp:14-211  constrainHbyA, computeRAU,
+//                                                computeRAUandHByA, updateFaceVelocity,
+//                                                updateVelocity
 #include <cmath>
 #include <cstdint>
 #include <cstring>
@@ -550,6 +561,108 @@ int orc_cg(int nCells, const int32_t* rowOffs, const int32_t* colIdxs, const dou
     stats[1] = res;
     stats[2] = it;
     return res <= relTol ? 0 : 1;
+}
+
+// Boundary coefficients of coeff * laplacian(gamma, T) per boundary face, the
+// face flux being bRhs - bMat * T_P (NeoN's la::BoundaryCoefficients as read
+// by pressureVelocityCoupling.cpp:180-192): fixedValue bMat = coeff*gamma|Sf|dc,
+// bRhs = coeff*gamma|Sf|dc*Tb; fixedGradient bMat = 0, bRhs = coeff*gamma|Sf|*g.
+int orc_laplacian_bcoeffs(int nInt, int nB, int nPatches, const int32_t* poff,
+                          const int32_t* pkind, const double* magSfb, const double* dcb,
+                          const double* gamma, const double* Tb, const double* Tg, double coeff,
+                          double* bMat, double* bRhs) {
+    for (int b = 0; b < nB; ++b) {
+        int k = pkind[patch_of(b, nPatches, poff)];
+        double gm = gamma[nInt + b] * magSfb[b];
+        if (k == BC_FIXED_GRADIENT) {
+            bMat[b] += 0.0;
+            bRhs[b] += coeff * (gm * (Tg ? Tg[b] : 0.0));
+        } else {
+            double ic = gm * dcb[b];
+            bMat[b] += coeff * ic;
+            bRhs[b] += coeff * (ic * Tb[b]);
+        }
+    }
+    return 0;
+}
+
+// computeRAU (pressureVelocityCoupling.cpp:38-61): rAU = V / diag; boundary
+// values extrapolated (rAU_b = rAU_P)
+int orc_rau(int nCells, int nB, const int32_t* rowOffs, const int32_t* diagOffset,
+            const double* values, const double* V, const int32_t* faceCells, double* rau,
+            double* raub) {
+    for (int c = 0; c < nCells; ++c) rau[c] = V[c] / values[rowOffs[c] + diagOffset[c]];
+    for (int b = 0; b < nB; ++b) raub[b] = rau[faceCells[b]];
+    return 0;
+}
+
+// computeRAUandHByA (pressureVelocityCoupling.cpp:63-119), with OpenFOAM's serial
+// fvMatrix::H face loop in place of the atomics: H = 0; per face
+// H[nbr] -= A(nbr,own) U[own], H[own] -= A(own,nbr) U[nbr]; then
+// HbyA = (H + b) * (rAU / V); boundary values extrapolated.
+int orc_hbya(int nCells, int nInt, int nB, const int32_t* own, const int32_t* nbr,
+             const int32_t* rowOffs, const int32_t* ownerOffset, const int32_t* neighbourOffset,
+             const double* values, const double* b3, const double* U, const double* rau,
+             const double* V, const int32_t* faceCells, double* hbya, double* hbyab) {
+    for (int i = 0; i < 3 * nCells; ++i) hbya[i] = 0.0;
+    for (int f = 0; f < nInt; ++f) {
+        const int o = own[f], n = nbr[f];
+        const double lower = values[rowOffs[n] + neighbourOffset[f]];
+        const double upper = values[rowOffs[o] + ownerOffset[f]];
+        for (int j = 0; j < 3; ++j) {
+            hbya[3 * n + j] -= lower * U[3 * o + j];
+            hbya[3 * o + j] -= upper * U[3 * n + j];
+        }
+    }
+    for (int c = 0; c < nCells; ++c) {
+        const double s = rau[c] / V[c];
+        for (int j = 0; j < 3; ++j) {
+            hbya[3 * c + j] += b3[3 * c + j];
+            hbya[3 * c + j] *= s;
+        }
+    }
+    for (int b = 0; b < nB; ++b)
+        for (int j = 0; j < 3; ++j) hbyab[3 * b + j] = hbya[3 * faceCells[b] + j];
+    return 0;
+}
+
+// updateFaceVelocity (pressureVelocityCoupling.cpp:122-195):
+// phi_f = pred_f - (A(nbr,own) p_N - A(own,nbr) p_P); boundary
+// phi_b = pred_b - (bRhs - bMat p_P)
+int orc_update_face_velocity(int nInt, int nB, const int32_t* own, const int32_t* nbr,
+                             const int32_t* rowOffs, const int32_t* ownerOffset,
+                             const int32_t* neighbourOffset, const double* values,
+                             const int32_t* faceCells, const double* bMat, const double* bRhs,
+                             const double* pred, const double* p, double* phi) {
+    for (int f = 0; f < nInt; ++f) {
+        const int o = own[f], n = nbr[f];
+        const double upper = values[rowOffs[n] + neighbourOffset[f]];
+        const double lower = values[rowOffs[o] + ownerOffset[f]];
+        phi[f] = pred[f] - (upper * p[n] - lower * p[o]);
+    }
+    for (int b = 0; b < nB; ++b) {
+        const double bflux = bRhs[b] - bMat[b] * p[faceCells[b]];
+        phi[nInt + b] = pred[nInt + b] - bflux;
+    }
+    return 0;
+}
+
+// updateVelocity (pressureVelocityCoupling.cpp:197-211): U = HbyA - rAU grad(p)
+int orc_update_velocity(int nCells, const double* hbya, const double* rau, const double* gradp,
+                        double* U) {
+    for (int c = 0; c < nCells; ++c)
+        for (int j = 0; j < 3; ++j) U[3 * c + j] = hbya[3 * c + j] - rau[c] * gradp[3 * c + j];
+    return 0;
+}
+
+// constrainHbyA (pressureVelocityCoupling.cpp:14-36): on patches whose velocity
+// condition is not assignable (fixedValue) the boundary HbyA is the boundary U
+int orc_constrain_hbya(int nB, int nPatches, const int32_t* poff, const int32_t* ukind,
+                       const double* Ub, double* hbyab) {
+    for (int b = 0; b < nB; ++b)
+        if (ukind[patch_of(b, nPatches, poff)] == BC_FIXED_VALUE)
+            for (int j = 0; j < 3; ++j) hbyab[3 * b + j] = Ub[3 * b + j];
+    return 0;
 }
 
 }  // extern "C"

@@ -56,6 +56,8 @@ lib.orc_assemble_source.argtypes = [C.c_int, C.c_void_p, C.c_void_p, C.c_void_p,
                                     C.c_double, C.c_void_p]
 lib.orc_assemble_laplacian.argtypes = [C.c_int] * 4 + [C.c_void_p] * 16 + [C.c_double, C.c_void_p, C.c_void_p]
 lib.orc_assemble_div.argtypes = [C.c_int] * 5 + [C.c_void_p] * 14 + [C.c_double, C.c_void_p, C.c_void_p]
+lib.orc_laplacian_bcoeffs.argtypes = [C.c_int] * 3 + [C.c_void_p] * 7 + [C.c_double, C.c_void_p,
+                                                                         C.c_void_p]
 lib.orc_set_reference.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_double, C.c_void_p, C.c_void_p]
 lib.orc_cg.argtypes = [C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                        C.c_double, C.c_int, C.c_void_p]
@@ -209,6 +211,9 @@ class LinSys:
         N = len(g["V"])
         self.values = np.zeros(len(sp["colIdxs"]))
         self.rhs = np.zeros(N)
+        # boundary coefficients of the laplacian terms (face flux = bRhs - bMat T_P)
+        self.bMat = np.zeros(len(g["faceCells"]))
+        self.bRhs = np.zeros(len(g["faceCells"]))
 
     def ddt(self, Told, dt, coeff=1.0):
         sp = self.sp
@@ -231,6 +236,10 @@ class LinSys:
                                    P(_f(Tg)) if Tg is not None else None, P(sp["rowOffs"]),
                                    P(sp["diagOffset"]), P(sp["ownerOffset"]),
                                    P(sp["neighbourOffset"]), coeff, P(self.values), P(self.rhs))
+        lib.orc_laplacian_bcoeffs(F, B, len(poff) - 1, P(poff), P(k), P(g["magSf_b"]),
+                                  P(g["deltaCoeffs_b"]), P(_f(gamma)), P(_f(Tb)),
+                                  P(_f(Tg)) if Tg is not None else None, coeff, P(self.bMat),
+                                  P(self.bRhs))
 
     def div(self, phi, Tb, kinds, Tg=None, scheme=0, coeff=1.0):
         g, sp = self.g, self.sp
@@ -267,6 +276,49 @@ class LinSys:
         rc = lib.orc_cg(len(self.rhs), P(sp["rowOffs"]), P(sp["colIdxs"]), P(self.values),
                         P(self.rhs), P(x), rel_tol, max_iter, P(stats))
         return x, {"init_res": stats[0], "final_res": stats[1], "iters": int(stats[2]), "rc": rc}
+
+
+# ---- pressure-velocity coupling helpers (pressureVelocityCoupling.cpp:14-211) -------
+
+def rau(g, sp, values):
+    N, F, B = _sizes(g)
+    out, outb = np.zeros(N), np.zeros(B)
+    lib.orc_rau(N, B, P(sp["rowOffs"]), P(sp["diagOffset"]), P(_f(values)), P(g["V"]),
+                P(g["faceCells"]), P(out), P(outb))
+    return out, outb
+
+
+def hbya(g, sp, values, b3, U, rAU):
+    N, F, B = _sizes(g)
+    out, outb = np.zeros((N, 3)), np.zeros((B, 3))
+    lib.orc_hbya(N, F, B, P(g["owner"]), P(g["neighbour"]), P(sp["rowOffs"]),
+                 P(sp["ownerOffset"]), P(sp["neighbourOffset"]), P(_f(values)), P(_f(b3)),
+                 P(_f(U)), P(_f(rAU)), P(g["V"]), P(g["faceCells"]), P(out), P(outb))
+    return out, outb
+
+
+def update_face_velocity(g, sp, values, bMat, bRhs, pred, p):
+    N, F, B = _sizes(g)
+    out = np.zeros(F + B)
+    lib.orc_update_face_velocity(F, B, P(g["owner"]), P(g["neighbour"]), P(sp["rowOffs"]),
+                                 P(sp["ownerOffset"]), P(sp["neighbourOffset"]), P(_f(values)),
+                                 P(g["faceCells"]), P(_f(bMat)), P(_f(bRhs)), P(_f(pred)),
+                                 P(_f(p)), P(out))
+    return out
+
+
+def update_velocity(hbya_, rAU, gradp):
+    N = len(rAU)
+    out = np.zeros((N, 3))
+    lib.orc_update_velocity(N, P(_f(hbya_)), P(_f(rAU)), P(_f(gradp)), P(out))
+    return out
+
+
+def constrain_hbya(g, ukinds, Ub, hbyab):
+    k, poff = _kinds(g, ukinds)
+    out = np.array(hbyab, dtype=np.float64, copy=True)
+    lib.orc_constrain_hbya(len(g["faceCells"]), len(poff) - 1, P(poff), P(k), P(_f(Ub)), P(out))
+    return out
 
 
 # ---- counter-based RNG (same hash as the library's fv_fill_uniform) ----------------

@@ -269,3 +269,112 @@ def test_set_reference():
 def test_uniform_rng_matches_library_formula():
     u = orc.uniform(5, 0.0, 1.0, 1, 2)
     assert np.all((u >= 0) & (u < 1)) and len(set(u)) == 5
+
+
+# ---- pressure-velocity coupling helpers -------------------------------------------
+
+def _momentum(n=6, seed=11):
+    """ddt + upwind div (asymmetric) - laplacian: a momentum-like matrix."""
+    poly = PolyMesh.hex(n, jitter=0.2, jitter_seed=seed, perm_seed=seed)
+    g = orc.geometry(poly)
+    sp = orc.sparsity(g)
+    N, F, B = len(g["V"]), len(g["owner"]), len(g["faceCells"])
+    ls = orc.LinSys(g, sp)
+    ls.ddt(orc.uniform(N, -1, 1, seed, 0), 1e-2)
+    kinds = [0] * 6
+    Tb = orc.uniform(B, -1, 1, seed, 1)
+    ls.div(orc.uniform(F + B, -1, 1, seed, 2), Tb, kinds, scheme=1)
+    ls.laplacian(orc.uniform(F + B, 0.5, 2.0, seed, 3), Tb, kinds, coeff=-1.0)
+    return g, sp, ls
+
+
+def test_rau_hbya_matrix_form():
+    """rAU = V/diag(A) and HbyA = (b - (A - diag A) U) rAU/V, evaluated as a
+    scipy matrix expression; the matrix is asymmetric (upwind div), so a swapped
+    lower/upper coefficient fails."""
+    import scipy.sparse as sps
+    g, sp, ls = _momentum()
+    N, B = len(g["V"]), len(g["faceCells"])
+    A = ls.scipy()
+    assert abs(A - A.T).max() > 1e-3                      # really asymmetric
+    rau, raub = orc.rau(g, sp, ls.values)
+    assert np.array_equal(rau, g["V"] / A.diagonal())
+    assert np.array_equal(raub, rau[g["faceCells"]])
+    U = orc.uniform(3 * N, -1, 1, 5, 0).reshape(N, 3)
+    b3 = orc.uniform(3 * N, -1, 1, 5, 1).reshape(N, 3)
+    h, hb = orc.hbya(g, sp, ls.values, b3, U, rau)
+    off = A - sps.diags(A.diagonal())
+    ref = (b3 - off @ U) * (rau / g["V"])[:, None]
+    assert np.max(np.abs(h - ref)) <= 1e-12 * np.max(np.abs(ref))
+    assert np.array_equal(hb, h[g["faceCells"]])
+    # a transposed matrix (lower/upper swapped) is detected
+    ref_t = (b3 - off.T @ U) * (rau / g["V"])[:, None]
+    assert np.max(np.abs(h - ref_t)) > 1e-6
+
+
+def test_corrected_flux_is_divergence_free():
+    """Solve laplacian(rAU_f, p) = div(phiHbyA) (p = 0 on patch 0, zeroGradient
+    elsewhere); the flux updated with the pressure system's coefficients is
+    divergence-free to the solver tolerance, inside and through the boundary."""
+    poly = PolyMesh.hex(7, jitter=0.2, jitter_seed=3, perm_seed=4)
+    g = orc.geometry(poly)
+    sp = orc.sparsity(g)
+    N, F, B = len(g["V"]), len(g["owner"]), len(g["faceCells"])
+    fc = g["faceCells"]
+    rau = orc.uniform(N, 0.5, 2.0, 7, 0)
+    rauf = orc.interpolate(g, rau, rau[fc])
+    U = orc.uniform(3 * N, -1, 1, 7, 1).reshape(N, 3)
+    Ub = orc.uniform(3 * B, -1, 1, 7, 2).reshape(B, 3)
+    phiH = orc.flux(g, U, Ub)
+    ones_c, ones_b = np.ones(N), np.ones(B)
+    d = orc.div(g, phiH, ones_c, ones_b)
+    kinds = [0] + [1] * 5
+    ls = orc.LinSys(g, sp)
+    ls.laplacian(rauf, np.zeros(B), kinds)
+    ls.rhs += d * g["V"]
+    p, st = ls.cg(np.zeros(N), 1e-13, 20000)
+    assert st["rc"] == 0
+    phi = orc.update_face_velocity(g, sp, ls.values, ls.bMat, ls.bRhs, phiH, p)
+    cont = orc.div(g, phi, ones_c, ones_b)
+    assert np.max(np.abs(cont)) <= 1e-9 * np.max(np.abs(d))
+    # fixedValue faces carry the pressure correction, zeroGradient faces none
+    po = g["patchOffsets"]
+    assert np.array_equal(phi[F + po[1]:], phiH[F + po[1]:])
+    assert np.max(np.abs(phi[F:F + po[1]] - phiH[F:F + po[1]])) > 0
+    # the opposite sign of the correction is not divergence-free
+    wrong = 2 * phiH - phi
+    assert np.max(np.abs(orc.div(g, wrong, ones_c, ones_b))) > 0.5 * np.max(np.abs(d))
+
+
+def test_update_velocity_closed_forms():
+    """U = HbyA - rAU grad(p): a uniform pressure leaves HbyA (up to the round-off
+    of grad of a constant), a linear pressure on a uniform mesh subtracts rAU
+    times its exact gradient."""
+    poly = PolyMesh.hex(5)
+    g = orc.geometry(poly)
+    N, B = len(g["V"]), len(g["faceCells"])
+    h = orc.uniform(3 * N, -1, 1, 9, 0).reshape(N, 3)
+    rau = orc.uniform(N, 0.5, 2.0, 9, 1)
+    gp = orc.grad(g, np.full(N, 3.7), np.full(B, 3.7))
+    assert np.max(np.abs(orc.update_velocity(h, rau, gp) - h)) < 1e-12
+    a = np.array([0.3, -1.2, 2.5])
+    p = g["C"] @ a
+    pb = g["Cf_b"].reshape(B, 3) @ a
+    gp = orc.grad(g, p, pb)
+    assert np.max(np.abs(gp - a)) < 1e-12
+    U = orc.update_velocity(h, rau, gp)
+    assert np.max(np.abs(U - (h - rau[:, None] * a))) < 1e-12
+
+
+def test_constrain_hbya_fixed_value_patches_only():
+    poly = PolyMesh.hex(4)
+    g = orc.geometry(poly)
+    B = len(g["faceCells"])
+    po = g["patchOffsets"]
+    Ub = orc.uniform(3 * B, -1, 1, 2, 0).reshape(B, 3)
+    hb = orc.uniform(3 * B, -1, 1, 2, 1).reshape(B, 3)
+    kinds = [0, 1, 2, 0, 1, 2]
+    out = orc.constrain_hbya(g, kinds, Ub, hb)
+    for p in range(6):
+        sl = slice(po[p], po[p + 1])
+        assert np.array_equal(out[sl], Ub[sl] if kinds[p] == 0 else hb[sl])
