@@ -52,6 +52,37 @@ BYTES = {
 OPS = ["interp_linear", "grad", "div_linear", "div_upwind", "laplacian", "sngrad"]
 
 
+# kernel that runs each (op, precision) of the sweep on a renumbered mesh, as rocprofv3 names it
+KERNEL_OF = {
+    ("interp_linear", "f64"): "k_interp<double>", ("interp_linear", "f32"): "k_interp<float>",
+    ("sngrad", "f64"): "k_sngrad<double>", ("sngrad", "f32"): "k_sngrad<float>",
+    ("grad", "f64"): "k_gather_stencil<double, GradOp<double>, 512, 512>",
+    ("grad", "f32"): "k_gather_stencil<float, GradOp<float>, 512, 512>",
+    ("div_linear", "f64"): "k_gather<double, DivOp<double, 0>, 512, 512>",
+    ("div_linear", "f32"): "k_gather<float, DivOp<float, 0>, 512, 512>",
+    ("div_upwind", "f64"): "k_gather<double, DivOp<double, 1>, 512, 512>",
+    ("div_upwind", "f32"): "k_gather<float, DivOp<float, 1>, 512, 512>",
+    ("laplacian", "f64"): "k_gather<double, LapOp<double>, 512, 512>",
+    ("laplacian", "f32"): "k_gather<float, LapOp<float>, 512, 512>",
+}
+TRAFFIC_PROFILE = ROOT / "profiles" / "faceops_sweep_256cube.md"
+
+
+def measured_traffic(kernel, n):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 FETCH_SIZE /
+    WRITE_SIZE summary of this sweep (256^3 only), or None."""
+    if n != 256 or not TRAFFIC_PROFILE.exists():
+        return None
+    for line in TRAFFIC_PROFILE.read_text().splitlines():
+        cells = [c.strip() for c in line.strip().strip("|").split("|")]
+        if len(cells) == 4 and cells[0] == f"`{kernel}`":
+            try:
+                return int(float(cells[3]) * 1e9)
+            except ValueError:
+                return None
+    return None
+
+
 def alg_bytes(op, prec, N, F, B):
     f, c, b = BYTES[(op, prec)]
     return f * F + c * N + b * B
@@ -561,7 +592,9 @@ def main():
         "per_op": sweep["per_op"],
         "roofline": {"bound": "hbm", "kernel": f"{dop}_{dprec}", "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": None, "bytes_per_launch": dom_bytes, "avg_launch_ms": round(dom_ms, 4)},
+                     "traffic": measured_traffic(KERNEL_OF[(dop, dprec)], args.n),
+                     "bytes_per_launch": dom_bytes, "avg_launch_ms": round(dom_ms, 4),
+                     "kernel_name": KERNEL_OF[(dop, dprec)]},
         "boundary_mapping": {"ms": round(case.mapping_ms, 3),
                              "what": "permuting the caller's T, U (cells) and gamma (faces) into "
                                      "the renumbered order once at upload"},

@@ -84,8 +84,16 @@ void member_tile_list(const fv_batch* b, int i, int slot0, std::vector<CgTile>& 
 
 template <typename S>
 CgPartials partials_of(fv_batch* b) {
-    const int64_t n = std::max(1, b->nTiles);
+    const int64_t n = std::max(1, b->tileCap);
     return CgPartials{b->partials.p, b->partials.p + n, b->partials.p + 3 * n};
+}
+
+constexpr int kXcdGroups = 8;
+
+int next_pow2(int v) {
+    int c = 64;
+    while (c < v) c *= 2;
+    return c;
 }
 
 template <typename S>
@@ -122,27 +130,63 @@ void build_tables(fv_batch* b) {
         p.r64 = (double*)(m->vec.p + 6 * (int64_t)N * sizeof(S));
         p.st = b->states.p + m->slot;
     }
-    b->nTiles = (int)ht.size();
-    b->probs.alloc(ctx, std::max<size_t>(1, P * sizeof(CgSys<S>)));
+    // Launch table: logical blocks are numbered XCD-contiguously (block i runs on
+    // XCD i % 8), so the tiles are split into 8 contiguous groups of about equal
+    // rows -- one per XCD, padded to equal length with empty entries -- so that
+    // members with large tiles do not load one XCD more than the others.
+    int64_t total = 0;
+    for (const CgTile& t : ht) total += t.row1 - t.row0;
+    int gs[kXcdGroups + 1];
+    {
+        int k = 0;
+        int64_t acc = 0;
+        gs[0] = 0;
+        for (int i = 0; i < (int)ht.size() && k < kXcdGroups - 1; ++i) {
+            if (acc >= total * (k + 1) / kXcdGroups) gs[++k] = i;
+            acc += ht[i].row1 - ht[i].row0;
+        }
+        while (k < kXcdGroups) gs[++k] = (int)ht.size();
+    }
+    int per = 1;
+    for (int k = 0; k < kXcdGroups; ++k) per = std::max(per, gs[k + 1] - gs[k]);
+    // the table holds kXcdGroups groups of perCap entries (the graph's grid);
+    // perCap follows per within [per, 1.25 per]: growing past it or shrinking
+    // below half of it reallocates and invalidates the captured graph
+    const int perCap = b->tileCap / kXcdGroups;
+    const bool regrow = per > perCap || per < perCap / 2 || P > b->probCap || !b->gridDev.p;
+    if (regrow) {
+        const int pc = (per + per / 4 + 7) / 8 * 8;
+        b->tileCap = kXcdGroups * pc;
+        b->probCap = std::max(b->probCap, next_pow2(P));
+        b->probs.alloc(ctx, b->probCap * sizeof(CgSys<S>));
+        b->tiles.alloc(ctx, b->tileCap * sizeof(CgTile));
+        b->partials.alloc(ctx, 5 * (size_t)b->tileCap);
+        b->gridDev.alloc(ctx, sizeof(CgGrid<S>));
+        drop_graph(b);
+    }
+    const int stride = b->tileCap / kXcdGroups;
+    std::vector<CgTile> lt((size_t)b->tileCap, CgTile{-1, 0, 0, 0});
+    for (int k = 0; k < kXcdGroups; ++k)
+        for (int j = gs[k]; j < gs[k + 1]; ++j) lt[(size_t)k * stride + (j - gs[k])] = ht[j];
+    b->nTiles = (int)lt.size();
     if (P)
         FV_HIP(hipMemcpyAsync(b->probs.p, hp.data(), P * sizeof(CgSys<S>), hipMemcpyHostToDevice,
                               ctx->stream));
-    b->tiles.alloc(ctx, std::max<size_t>(1, ht.size() * sizeof(CgTile)));
-    if (!ht.empty())
-        FV_HIP(hipMemcpyAsync(b->tiles.p, ht.data(), ht.size() * sizeof(CgTile),
+    if (!lt.empty())
+        FV_HIP(hipMemcpyAsync(b->tiles.p, lt.data(), lt.size() * sizeof(CgTile),
                               hipMemcpyHostToDevice, ctx->stream));
-    b->partials.alloc(ctx, 5 * std::max<size_t>(1, ht.size()));
+    const CgGrid<S> g{(const CgTile*)b->tiles.p, (const CgSys<S>*)b->probs.p, b->nTiles, 0};
+    FV_HIP(hipMemcpyAsync(b->gridDev.p, &g, sizeof g, hipMemcpyHostToDevice, ctx->stream));
     FV_HIP(hipStreamSynchronize(ctx->stream));
     b->dirty = false;
-    drop_graph(b);
 }
 
 template <typename S>
 void enqueue_chunk(fv_batch* b) {
     hipStream_t s = b->ctx->stream;
-    const CgGrid<S> g{(const CgTile*)b->tiles.p, (const CgSys<S>*)b->probs.p, b->nTiles, 0};
+    const auto* g = (const CgGrid<S>*)b->gridDev.p;
     const CgPartials red = partials_of<S>(b);
-    const int G = cg_grid(b->nTiles);
+    const int G = cg_grid(b->tileCap);
     for (int i = 0; i < b->chunk; ++i) {
         kc_spmv<S><<<G, kB, 0, s>>>(g, red);
         kc_update<S><<<G, kB, 0, s>>>(g, red);
@@ -204,11 +248,15 @@ void start_members(fv_batch* b) {
         slot += member_tiles(b, m);
     }
     if (st.empty()) return;
-    b->startTiles.alloc(ctx, st.size() * sizeof(CgTile));
+    if (b->startTiles.n < st.size() * sizeof(CgTile))
+        b->startTiles.alloc(ctx, next_pow2((int)st.size()) * sizeof(CgTile));
+    if (!b->startGridDev.p) b->startGridDev.alloc(ctx, sizeof(CgGrid<S>));
     FV_HIP(hipMemcpyAsync(b->startTiles.p, st.data(), st.size() * sizeof(CgTile),
                           hipMemcpyHostToDevice, s));
-    const CgGrid<S> g{(const CgTile*)b->startTiles.p, (const CgSys<S>*)b->probs.p,
-                      (int)st.size(), 0};
+    const CgGrid<S> hg{(const CgTile*)b->startTiles.p, (const CgSys<S>*)b->probs.p,
+                       (int)st.size(), 0};
+    FV_HIP(hipMemcpyAsync(b->startGridDev.p, &hg, sizeof hg, hipMemcpyHostToDevice, s));
+    const auto* g = (const CgGrid<S>*)b->startGridDev.p;
     const CgPartials red = partials_of<S>(b);
     const int G = cg_grid((int)st.size());
     kc_resid<S, 1><<<G, kB, 0, s>>>(g, red);

@@ -33,13 +33,16 @@ struct fv_batch {
     std::vector<BMember*> members;
     std::vector<int> freeSlots;
     int slotCap = 0;
-    // device tables, rebuilt when membership changes
+    // device tables, rewritten when membership changes; their capacities only
+    // grow, and the captured chunk graph stays valid until they do
     fvhip::DBuf<uint8_t> probs;
     fvhip::DBuf<uint8_t> tiles;
     fvhip::DBuf<uint8_t> startTiles;
+    fvhip::DBuf<uint8_t> gridDev, startGridDev;  // launch descriptions (CgGrid)
     fvhip::DBuf<fvhip::CgState> states;
     fvhip::DBuf<double> partials;
     int nTiles = 0, nStartTiles = 0;
+    int tileCap = 0, probCap = 0;
     bool dirty = true;
     hipGraph_t graph = nullptr;
     hipGraphExec_t graphExec = nullptr;

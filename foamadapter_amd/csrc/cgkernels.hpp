@@ -49,7 +49,7 @@ struct CgSys {
 };
 
 struct CgTile {
-    int32_t prob, row0, row1, slot;
+    int32_t prob, row0, row1, slot;  // prob < 0: empty entry
 };
 
 template <typename S>
@@ -65,14 +65,19 @@ struct CgPartials {
     double *pq, *rz, *rb;
 };
 
+// The launch description lives in device memory, so a captured hipGraph stays
+// valid when the batch's tile table changes (only the table and the count are
+// rewritten); a grid larger than the tile count leaves the extra blocks idle.
 template <typename S>
-__device__ inline bool cg_locate(const CgGrid<S>& g, const CgSys<S>*& P, int& row0, int& row1,
-                                 int& slot) {
+__device__ inline bool cg_locate(const CgGrid<S>* __restrict__ gp, const CgSys<S>*& P, int& row0,
+                                 int& row1, int& slot) {
+    const CgGrid<S> g = *gp;
     const int per = gridDim.x >> 3;  // the grid is a multiple of 8 blocks
     const int lb = (int)(blockIdx.x & 7) * per + (int)(blockIdx.x >> 3);
     if (lb >= g.nTiles) return false;
     if (g.tiles) {
         const CgTile t = g.tiles[lb];
+        if (t.prob < 0) return false;  // padding entry of the launch table
         P = g.probs + t.prob;
         row0 = t.row0;
         row1 = t.row1;
@@ -87,7 +92,7 @@ __device__ inline bool cg_locate(const CgGrid<S>& g, const CgSys<S>*& P, int& ro
 }
 
 template <typename S>
-__global__ __launch_bounds__(kCgBlock) void kc_spmv(CgGrid<S> g, CgPartials red) {
+__global__ __launch_bounds__(kCgBlock) void kc_spmv(const CgGrid<S>* __restrict__ g, CgPartials red) {
     __shared__ double sh[kCgBlock / 64];
     const CgSys<S>* Pp;
     int row0, row1, slot;
@@ -120,7 +125,7 @@ __global__ __launch_bounds__(kCgBlock) void kc_spmv(CgGrid<S> g, CgPartials red)
 }
 
 template <typename S>
-__global__ __launch_bounds__(kCgBlock) void kc_update(CgGrid<S> g, CgPartials red) {
+__global__ __launch_bounds__(kCgBlock) void kc_update(const CgGrid<S>* __restrict__ g, CgPartials red) {
     __shared__ double sh[kCgBlock / 64];
     const CgSys<S>* Pp;
     int row0, row1, slot;
@@ -155,7 +160,7 @@ __global__ __launch_bounds__(kCgBlock) void kc_update(CgGrid<S> g, CgPartials re
 //         the FP64 residual: 2 converged, 3 non-finite, 1 iteration limit);
 // MODE 2: after the start (replacement of the initial guess; p = z already).
 template <typename S, int MODE>
-__global__ __launch_bounds__(kCgBlock) void kc_direction(CgGrid<S> g, CgPartials red) {
+__global__ __launch_bounds__(kCgBlock) void kc_direction(const CgGrid<S>* __restrict__ g, CgPartials red) {
     __shared__ double sh[kCgBlock / 64];
     const CgSys<S>* Pp;
     int row0, row1, slot;
@@ -194,7 +199,7 @@ __global__ __launch_bounds__(kCgBlock) void kc_direction(CgGrid<S> g, CgPartials
 }
 
 template <typename S>
-__global__ __launch_bounds__(kCgBlock) void kc_accum(CgGrid<S> g) {
+__global__ __launch_bounds__(kCgBlock) void kc_accum(const CgGrid<S>* __restrict__ g) {
     const CgSys<S>* Pp;
     int row0, row1, slot;
     if (!cg_locate(g, Pp, row0, row1, slot)) return;
@@ -209,7 +214,7 @@ __global__ __launch_bounds__(kCgBlock) void kc_accum(CgGrid<S> g) {
 
 // r64 = b - A x in FP64 (CSR rows in column order), partials of r.r and b.b
 template <typename S, int START>
-__global__ __launch_bounds__(kCgBlock) void kc_resid(CgGrid<S> g, CgPartials red) {
+__global__ __launch_bounds__(kCgBlock) void kc_resid(const CgGrid<S>* __restrict__ g, CgPartials red) {
     __shared__ double sh[kCgBlock / 64];
     const CgSys<S>* Pp;
     int row0, row1, slot;
@@ -234,7 +239,7 @@ __global__ __launch_bounds__(kCgBlock) void kc_resid(CgGrid<S> g, CgPartials red
 // relative FP64 residual from the resid partials; r = (S) r64, z = D^-1 r,
 // START: p = z and ||b||; partials of r.r and r.z
 template <typename S, int START>
-__global__ __launch_bounds__(kCgBlock) void kc_replace(CgGrid<S> g, CgPartials red) {
+__global__ __launch_bounds__(kCgBlock) void kc_replace(const CgGrid<S>* __restrict__ g, CgPartials red) {
     __shared__ double sh[kCgBlock / 64];
     const CgSys<S>* Pp;
     int row0, row1, slot;

@@ -187,6 +187,7 @@ struct Split {
 template <typename S>
 struct GradOp {
     static constexpr int NC = 3, NS = 1;
+    static constexpr bool kLists = false;  // runs k_gather_stencil
     const S *T, *w, *Sf, *Sfb, *Tb;
     __device__ void state(int f, int o, int n, S* s) const {
         const S TP = T[o], TN = T[n];
@@ -219,6 +220,7 @@ struct GradOp {
 template <typename S, typename D>
 struct PlainOp {
     static constexpr int NC = 1, NS = 1;
+    static constexpr bool kLists = true;  // runs k_gather
     __device__ void state(int f, int o, int n, S* s) const {
         static_cast<const D*>(this)->face(f, o, n, s);
     }
@@ -262,11 +264,13 @@ struct LapOp : PlainOp<S, LapOp<S>> {
     }
 };
 
-template <typename S, typename Op>
+// One cell of the per-cell path: LDS holds, for the workgroup's first nUp own
+// faces, the face state (kState, NS components) or the contribution (NC).
+template <typename S, typename Op, bool kState>
 __device__ inline void gather_cell(int c, int fs, int nUp, const Split& t, const Op& op,
                                    const S* __restrict__ V, S coeff, S* __restrict__ out,
                                    const S* st0, int stride) {
-    constexpr int NC = Op::NC, NS = Op::NS;
+    constexpr int NC = Op::NC, NS = kState ? Op::NS : Op::NC;
     S acc[NC];
     for (int j = 0; j < NC; ++j) acc[j] = S(0);
     const int l1 = t.lowOff[c + 1];
@@ -277,7 +281,10 @@ __device__ inline void gather_cell(int c, int fs, int nUp, const Split& t, const
         if (i >= 0 && i < nUp) {
             S sv[NS];
             for (int j = 0; j < NS; ++j) sv[j] = st0[j * stride + i];
-            op.contrib(f, sv, v);
+            if (kState)
+                op.contrib(f, sv, v);
+            else
+                for (int j = 0; j < NC; ++j) v[j] = sv[j];
         } else {
             op.face(f, t.lowOwn[k], c, v);
         }
@@ -290,7 +297,10 @@ __device__ inline void gather_cell(int c, int fs, int nUp, const Split& t, const
         if (i < nUp) {
             S sv[NS];
             for (int j = 0; j < NS; ++j) sv[j] = st0[j * stride + i];
-            op.contrib(f, sv, v);
+            if (kState)
+                op.contrib(f, sv, v);
+            else
+                for (int j = 0; j < NC; ++j) v[j] = sv[j];
         } else {
             op.face(f, c, t.nbr[f], v);
         }
@@ -304,6 +314,35 @@ __device__ inline void gather_cell(int c, int fs, int nUp, const Split& t, const
     }
     const S vol = V[c];
     for (int j = 0; j < NC; ++j) out[(int64_t)NC * c + j] = coeff * (acc[j] / vol);
+}
+
+// Staged gather with the per-cell stencil lists read from HBM: phase 1 keeps the
+// contribution (NC components) of every own face in LDS, phase 2 runs
+// gather_cell per cell.  Lighter in LDS and in phase-1 work than k_gather, it
+// is the faster variant for the gradient (three components per face), whose
+// phase 2 would otherwise re-read Sf per face.
+template <typename S, typename Op, int CELLS, int THREADS>
+__global__ __launch_bounds__(THREADS) void k_gather_stencil(int N, int nBlocks, int perXcd,
+                                                            Split t, Op op,
+                                                            const S* __restrict__ V, S coeff,
+                                                            S* __restrict__ out) {
+    constexpr int NC = Op::NC;
+    constexpr int kMaxUp = max_up<CELLS>();
+    __shared__ S flux[NC][kMaxUp];
+    const int lb = (blockIdx.x % kXcd) * perXcd + blockIdx.x / kXcd;
+    if (lb >= nBlocks) return;
+    const int c0 = lb * CELLS, c1 = min(c0 + CELLS, N);
+    const int fs = t.ownStart[c0], fe = t.ownStart[c1];
+    const int nUp = min(fe - fs, kMaxUp);
+    for (int i = threadIdx.x; i < nUp; i += THREADS) {
+        const int f = fs + i;
+        S v[NC];
+        op.face(f, t.own[f], t.nbr[f], v);
+        for (int j = 0; j < NC; ++j) flux[j][i] = v[j];
+    }
+    __syncthreads();
+    for (int c = c0 + threadIdx.x; c < c1; c += THREADS)
+        gather_cell<S, Op, false>(c, fs, nUp, t, op, V, coeff, out, flux[0], kMaxUp);
 }
 
 struct Lists {
@@ -394,7 +433,7 @@ __global__ __launch_bounds__(THREADS) void k_gather(int N, int nBlocks, int perX
     __syncthreads();
     if (overflow) {
         for (int c = c0 + threadIdx.x; c < c1; c += THREADS)
-            gather_cell<S, Op>(c, fs, nUp, t, op, V, coeff, out, fluxU[0], kMaxUp);
+            gather_cell<S, Op, true>(c, fs, nUp, t, op, V, coeff, out, fluxU[0], kMaxUp);
         return;
     }
     for (int c = c0 + threadIdx.x; c < c1; c += THREADS) {
@@ -447,12 +486,18 @@ Split split_of(const fv_mesh* m) {
 
 template <typename S, typename Op, int CELLS, int THREADS>
 void launch_gather_t(fv_ctx* ctx, const fv_mesh* m, const Op& op, const S* V, S coeff, S* out) {
+    const int nBlocks = (m->nCells + CELLS - 1) / CELLS;
+    const int perXcd = (nBlocks + kXcd - 1) / kXcd;
+    if (!Op::kLists) {
+        k_gather_stencil<S, Op, CELLS, THREADS><<<perXcd * kXcd, THREADS, 0, ctx->stream>>>(
+            m->nCells, nBlocks, perXcd, split_of(m), op, V, coeff, out);
+        FV_LAUNCH_CHECK();
+        return;
+    }
     // the lists are built once per mesh and workgroup size (a cache on the mesh)
     const GatherLists& gl = gather_lists(const_cast<fv_mesh*>(m), CELLS);
     const Lists L{gl.extOff.p, gl.extFace.p, gl.extOwn.p, gl.bndOff.p, gl.bndFace.p,
                   gl.extCell.p, gl.bndCell.p};
-    const int nBlocks = (m->nCells + CELLS - 1) / CELLS;
-    const int perXcd = (nBlocks + kXcd - 1) / kXcd;
     k_gather<S, Op, CELLS, THREADS><<<perXcd * kXcd, THREADS, 0, ctx->stream>>>(
         m->nCells, nBlocks, perXcd, split_of(m), L, op, V, coeff, out);
     FV_LAUNCH_CHECK();

@@ -28,6 +28,7 @@ struct CgWork : fv_solver_cache {
     DBuf<CgSys<S>> sys;
     CgSys<S> h{};
     CgGrid<S> grid{};
+    DBuf<CgGrid<S>> gdev;  // device copy of grid (the kernels' launch description)
     CgPartials red{};
     int G = 0;  // launch grid (multiple of 8)
 
@@ -72,6 +73,8 @@ struct CgWork : fv_solver_cache {
         grid.probs = sys.p;
         grid.nTiles = tiles;
         grid.rowsPerBlock = R;
+        gdev.upload(c, &grid, 1);
+        FV_HIP(hipStreamSynchronize(c->stream));
         red = CgPartials{pq.p, rz.p, rb.p};
     }
 
@@ -95,20 +98,20 @@ struct CgWork : fv_solver_cache {
     // FP64 residual of the current x, r = (S) r64, z = D^-1 r, p = z, rz_0
     void start(hipStream_t s, double tol, int maxit, double delta) {
         kc_init_state<<<1, 64, 0, s>>>(st.p, 1, tol, maxit, delta);
-        kc_resid<S, 1><<<G, kB, 0, s>>>(grid, red);
-        kc_replace<S, 1><<<G, kB, 0, s>>>(grid, red);
-        kc_direction<S, 2><<<G, kB, 0, s>>>(grid, red);
+        kc_resid<S, 1><<<G, kB, 0, s>>>(gdev.p, red);
+        kc_replace<S, 1><<<G, kB, 0, s>>>(gdev.p, red);
+        kc_direction<S, 2><<<G, kB, 0, s>>>(gdev.p, red);
     }
     void iteration(hipStream_t s, bool direction) {
-        kc_spmv<S><<<G, kB, 0, s>>>(grid, red);
-        kc_update<S><<<G, kB, 0, s>>>(grid, red);
-        if (direction) kc_direction<S, 0><<<G, kB, 0, s>>>(grid, red);
+        kc_spmv<S><<<G, kB, 0, s>>>(gdev.p, red);
+        kc_update<S><<<G, kB, 0, s>>>(gdev.p, red);
+        if (direction) kc_direction<S, 0><<<G, kB, 0, s>>>(gdev.p, red);
     }
     void replacement(hipStream_t s) {
-        kc_accum<S><<<G, kB, 0, s>>>(grid);
-        kc_resid<S, 0><<<G, kB, 0, s>>>(grid, red);
-        kc_replace<S, 0><<<G, kB, 0, s>>>(grid, red);
-        kc_direction<S, 1><<<G, kB, 0, s>>>(grid, red);
+        kc_accum<S><<<G, kB, 0, s>>>(gdev.p);
+        kc_resid<S, 0><<<G, kB, 0, s>>>(gdev.p, red);
+        kc_replace<S, 0><<<G, kB, 0, s>>>(gdev.p, red);
+        kc_direction<S, 1><<<G, kB, 0, s>>>(gdev.p, red);
     }
     CgState state(hipStream_t s) {
         CgState o{};
@@ -118,7 +121,7 @@ struct CgWork : fv_solver_cache {
     }
     // explicit FP64 relative residual ||b - A x|| / ||b|| of the bound x
     double true_residual(hipStream_t s) {
-        kc_resid<S, 1><<<G, kB, 0, s>>>(grid, red);
+        kc_resid<S, 1><<<G, kB, 0, s>>>(gdev.p, red);
         kc_resid_norm<S><<<1, kB, 0, s>>>(sys.p, red, st2.p);
         FV_LAUNCH_CHECK();
         CgState o{};
