@@ -58,12 +58,12 @@ KERNEL_OF = {
     ("sngrad", "f64"): "k_sngrad<double>", ("sngrad", "f32"): "k_sngrad<float>",
     ("grad", "f64"): "k_gather_stencil<double, GradOp<double>, 512, 512>",
     ("grad", "f32"): "k_gather_stencil<float, GradOp<float>, 512, 512>",
-    ("div_linear", "f64"): "k_gather<double, DivOp<double, 0>, 512, 512>",
-    ("div_linear", "f32"): "k_gather<float, DivOp<float, 0>, 512, 512>",
-    ("div_upwind", "f64"): "k_gather<double, DivOp<double, 1>, 512, 512>",
-    ("div_upwind", "f32"): "k_gather<float, DivOp<float, 1>, 512, 512>",
-    ("laplacian", "f64"): "k_gather<double, LapOp<double>, 512, 512>",
-    ("laplacian", "f32"): "k_gather<float, LapOp<float>, 512, 512>",
+    ("div_linear", "f64"): "k_gather<double, DivOp<double, 0>, 256, 256>",
+    ("div_linear", "f32"): "k_gather<float, DivOp<float, 0>, 256, 256>",
+    ("div_upwind", "f64"): "k_gather<double, DivOp<double, 1>, 256, 256>",
+    ("div_upwind", "f32"): "k_gather<float, DivOp<float, 1>, 256, 256>",
+    ("laplacian", "f64"): "k_gather<double, LapOp<double>, 256, 256>",
+    ("laplacian", "f32"): "k_gather<float, LapOp<float>, 256, 256>",
 }
 TRAFFIC_PROFILE = ROOT / "profiles" / "faceops_sweep_256cube.md"
 

@@ -171,6 +171,7 @@ __global__ __launch_bounds__(kBlock) void k_laplacian(
 // faces it recomputes in its own L2.
 
 constexpr int kXcd = 8;
+constexpr int kU = 4;  // faces per thread whose loads a staged-gather phase 1 keeps in flight
 // LDS face slots of a CELLS-cell workgroup: 3 owned faces per hex cell plus slack;
 // faces beyond it (unusual meshes) are recomputed in phase 2
 template <int CELLS>
@@ -188,6 +189,7 @@ template <typename S>
 struct GradOp {
     static constexpr int NC = 3, NS = 1;
     static constexpr bool kLists = false;  // runs k_gather_stencil
+    static constexpr int kDefaultCells = 512;
     const S *T, *w, *Sf, *Sfb, *Tb;
     __device__ void state(int f, int o, int n, S* s) const {
         const S TP = T[o], TN = T[n];
@@ -221,6 +223,7 @@ template <typename S, typename D>
 struct PlainOp {
     static constexpr int NC = 1, NS = 1;
     static constexpr bool kLists = true;  // runs k_gather
+    static constexpr int kDefaultCells = 256;
     __device__ void state(int f, int o, int n, S* s) const {
         static_cast<const D*>(this)->face(f, o, n, s);
     }
@@ -334,11 +337,25 @@ __global__ __launch_bounds__(THREADS) void k_gather_stencil(int N, int nBlocks, 
     const int c0 = lb * CELLS, c1 = min(c0 + CELLS, N);
     const int fs = t.ownStart[c0], fe = t.ownStart[c1];
     const int nUp = min(fe - fs, kMaxUp);
-    for (int i = threadIdx.x; i < nUp; i += THREADS) {
-        const int f = fs + i;
-        S v[NC];
-        op.face(f, t.own[f], t.nbr[f], v);
-        for (int j = 0; j < NC; ++j) flux[j][i] = v[j];
+    // kU faces per thread in flight: their loads are issued together (lanes past
+    // the range work on the last face and store nothing)
+    for (int i0 = threadIdx.x; i0 < nUp; i0 += THREADS * kU) {
+        int oo[kU], nn[kU];
+#pragma unroll
+        for (int u = 0; u < kU; ++u) {
+            const int f = fs + min(i0 + u * THREADS, nUp - 1);
+            oo[u] = t.own[f];
+            nn[u] = t.nbr[f];
+        }
+        S v[kU][NC];
+#pragma unroll
+        for (int u = 0; u < kU; ++u) op.face(fs + min(i0 + u * THREADS, nUp - 1), oo[u], nn[u], v[u]);
+#pragma unroll
+        for (int u = 0; u < kU; ++u) {
+            const int i = i0 + u * THREADS;
+            if (i < nUp)
+                for (int j = 0; j < NC; ++j) flux[j][i] = v[u][j];
+        }
     }
     __syncthreads();
     for (int c = c0 + threadIdx.x; c < c1; c += THREADS)
@@ -392,12 +409,25 @@ __global__ __launch_bounds__(THREADS) void k_gather(int N, int nBlocks, int perX
     if (threadIdx.x == 0) overflow = fits ? 0 : 1;
     __syncthreads();
     const int nUp = min(nF, kMaxUp);
-    for (int i = threadIdx.x; i < nUp; i += THREADS) {
+    for (int i0 = threadIdx.x; i0 < nUp; i0 += THREADS * kU) {
+      // kU faces per thread in flight (see k_gather_stencil)
+      int oo[kU], nn[kU];
+      S svs[kU][NS];
+#pragma unroll
+      for (int u = 0; u < kU; ++u) {
+          const int f = fs + min(i0 + u * THREADS, nUp - 1);
+          oo[u] = t.own[f];
+          nn[u] = t.nbr[f];
+      }
+#pragma unroll
+      for (int u = 0; u < kU; ++u) op.state(fs + min(i0 + u * THREADS, nUp - 1), oo[u], nn[u], svs[u]);
+#pragma unroll
+      for (int u = 0; u < kU; ++u) {
+        const int i = i0 + u * THREADS;
+        if (i >= nUp) break;
         const int f = fs + i;
-        const int o = t.own[f], n = t.nbr[f];
-        S sv[NS];
-        op.state(f, o, n, sv);
-        for (int j = 0; j < NS; ++j) fluxU[j][i] = sv[j];
+        const int o = oo[u], n = nn[u];
+        for (int j = 0; j < NS; ++j) fluxU[j][i] = svs[u][j];
         if (fits) {
             const int ol = o - c0;
             if (i == 0 || t.own[f - 1] != o) upS[ol] = (uint16_t)i;
@@ -411,6 +441,7 @@ __global__ __launch_bounds__(THREADS) void k_gather(int N, int nBlocks, int perX
                     overflow = 1;
             }
         }
+      }
     }
     if (fits) {
         for (int k = e0 + threadIdx.x; k < e1; k += THREADS) {
@@ -505,7 +536,10 @@ void launch_gather_t(fv_ctx* ctx, const fv_mesh* m, const Op& op, const S* V, S 
 
 template <typename S, typename Op>
 void launch_gather(fv_ctx* ctx, const fv_mesh* m, const Op& op, const S* V, S coeff, S* out) {
-    switch (gather_cells(m)) {
+    // default workgroup size per operator (measured at 256^3: the gradient's
+    // stencil-list kernel is fastest with 512 cells, the LDS-list kernel with 256)
+    const int cells = m->gatherMode == 0 ? Op::kDefaultCells : gather_cells(m);
+    switch (cells) {
         case 1024: launch_gather_t<S, Op, 1024, 512>(ctx, m, op, V, coeff, out); break;
         case 512: launch_gather_t<S, Op, 512, 512>(ctx, m, op, V, coeff, out); break;
         default: launch_gather_t<S, Op, 256, 256>(ctx, m, op, V, coeff, out); break;

@@ -75,7 +75,8 @@ void ensure_f32(fv_mesh* m);
 const GatherLists& gather_lists(fv_mesh* m, int cells);
 // Cells per workgroup of the LDS-staged gather for the cell-result operators,
 // 0 when the per-cell stencil walk runs instead (faces not in upper-triangular
-// order, or forced by fv_mesh_set_gather).
+// order, or forced by fv_mesh_set_gather); kDefaultGatherCells stands for the
+// per-operator default (the gradient 512, divergence and laplacian 256).
 constexpr int kDefaultGatherCells = 512;
 inline int gather_cells(const fv_mesh* m) {
     if (!m->upperTri) return 0;
