@@ -70,7 +70,7 @@ FV_SOLVE_F64, FV_SOLVE_F32_IR = 0, 1
 class FvSolveOpts(C.Structure):
     _fields_ = [("rel_tol", C.c_double), ("max_iter", C.c_int32), ("mode", C.c_int32),
                 ("inner_rel_tol", C.c_double), ("max_outer", C.c_int32),
-                ("check_every", C.c_int32)]
+                ("check_every", C.c_int32), ("abs_tol", C.c_double)]
 
 
 class FvSolverStats(C.Structure):
@@ -145,6 +145,7 @@ _SIGS = {
     "fv_constrain_hbya": [_vp, _vp, _vp],
     "fv_linsys_boundary_coeffs": [_vp, C.POINTER(_vp), C.POINTER(_vp)],
     "fv_solve_cg": [_vp, _vp, C.POINTER(FvSolveOpts), C.POINTER(FvSolverStats)],
+    "fv_solve_bicgstab": [_vp, _vp, C.POINTER(FvSolveOpts), C.POINTER(FvSolverStats)],
     "fv_batch_create": [_vp, C.c_int, C.c_int, C.c_int, _pp],
     "fv_batch_destroy": [_vp],
     "fv_batch_add": [_vp, _vp, _vp, _vp, C.POINTER(C.c_int)],

@@ -176,13 +176,14 @@ __global__ __launch_bounds__(kCgBlock) void kc_direction(const CgGrid<S>* __rest
     int stop;
     if (MODE == 0) {
         const double est = rn / st->bnorm;
-        stop = (est <= st->tol || it >= st->maxit || !(est == est) ||
+        stop = (est <= st->tol || rn <= st->atol || it >= st->maxit || !(est == est) ||
                 (st->delta > 0.0 && rn < st->delta * st->rrep))
                    ? 1
                    : 0;
     } else {
         const double res = st->res;
-        stop = res <= st->tol ? 2 : !(res <= 1.7976931348623157e308) ? 3 : it >= st->maxit ? 1 : 0;
+        const bool conv = res <= st->tol || res * st->bnorm <= st->atol;
+        stop = conv ? 2 : !(res <= 1.7976931348623157e308) ? 3 : it >= st->maxit ? 1 : 0;
     }
     if (row0 == 0 && threadIdx.x == 0) {
         st->rz[it & 1] = tot[1];
@@ -297,11 +298,13 @@ __global__ __launch_bounds__(kCgBlock) void kc_resid_norm(const CgSys<S>* __rest
 }
 
 template <int = 0>
-__global__ void kc_init_state(CgState* st, int n, double tol, int maxit, double delta) {
+__global__ void kc_init_state(CgState* st, int n, double tol, int maxit, double delta,
+                              double atol = 0.0) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) {
         CgState s{};
         s.tol = tol;
+        s.atol = atol;
         s.maxit = maxit;
         s.delta = delta;
         st[i] = s;

@@ -25,7 +25,7 @@ constexpr int kCgBlock = 256;
 struct CgState {
     double rz[2], alpha, bnorm, res, res0, rrep, delta;
     int32_t it, done, maxit, stop;
-    double tol;
+    double tol, atol;  // relative and absolute (0: off) residual targets
 };
 
 __device__ inline double block_sum(double v, double* sh) {

@@ -39,5 +39,5 @@ struct fv_linsys {
     // the face flux is bRhs - bMat * x_P (NeoN's la::BoundaryCoefficients,
     // read by updateFaceVelocity, pressureVelocityCoupling.cpp:180-192)
     fvhip::DBuf<double> bMat, bRhs;
-    std::unique_ptr<fv_solver_cache> cg[2];  // FP64 CG, FP32 CG in FP64 refinement
+    std::unique_ptr<fv_solver_cache> cg[3];  // FP64 CG, FP32 CG in FP64 refinement, BiCGStab
 };

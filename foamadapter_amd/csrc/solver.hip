@@ -96,8 +96,8 @@ struct CgWork : fv_solver_cache {
     }
 
     // FP64 residual of the current x, r = (S) r64, z = D^-1 r, p = z, rz_0
-    void start(hipStream_t s, double tol, int maxit, double delta) {
-        kc_init_state<<<1, 64, 0, s>>>(st.p, 1, tol, maxit, delta);
+    void start(hipStream_t s, double tol, int maxit, double delta, double atol) {
+        kc_init_state<<<1, 64, 0, s>>>(st.p, 1, tol, maxit, delta, atol);
         kc_resid<S, 1><<<G, kB, 0, s>>>(gdev.p, red);
         kc_replace<S, 1><<<G, kB, 0, s>>>(gdev.p, red);
         kc_direction<S, 2><<<G, kB, 0, s>>>(gdev.p, red);
@@ -139,9 +139,9 @@ CgWork<S>& work_of(fv_linsys* ls, bool mixed) {
 }
 
 // Plain FP64 CG from x; stops on the recurrence residual estimate.
-CgState run_cg(fv_ctx* ctx, CgWork<double>& w, double tol, int maxit, int check) {
+CgState run_cg(fv_ctx* ctx, CgWork<double>& w, double tol, int maxit, int check, double atol) {
     hipStream_t s = ctx->stream;
-    w.start(s, tol, maxit, 0.0);
+    w.start(s, tol, maxit, 0.0, atol);
     FV_LAUNCH_CHECK();
     // iterations follow the start without a round trip (a system converged at
     // the start turns them into no-ops); poll early for short solves, then
@@ -171,16 +171,16 @@ CgState run_cg(fv_ctx* ctx, CgWork<double>& w, double tol, int maxit, int check)
 // with its search direction (beta from the replaced residual), so the Krylov
 // space is not restarted.
 struct IrOut {
-    double init_res, res;
+    double init_res, res, bnorm;
     int iters, outer;
 };
 
 IrOut run_mixed(fv_ctx* ctx, CgWork<float>& w, double tol, int maxit, int chunk, double delta,
-                int max_outer) {
+                int max_outer, double atol) {
     hipStream_t s = ctx->stream;
-    w.start(s, tol, maxit, delta);
+    w.start(s, tol, maxit, delta, atol);
     FV_LAUNCH_CHECK();
-    IrOut out{0, 0, 0, 0};
+    IrOut out{0, 0, 0, 0, 0};
     CgState h{};  // the first chunk follows the start without a round trip
     int batch = std::min(8, chunk);
     while (h.stop < 2 && h.it < maxit && out.outer < max_outer) {
@@ -196,6 +196,7 @@ IrOut run_mixed(fv_ctx* ctx, CgWork<float>& w, double tol, int maxit, int chunk,
     if (out.outer == 0) h = w.state(s);
     out.init_res = h.res0;
     out.res = h.res;
+    out.bnorm = h.bnorm;
     out.iters = h.it;
     return out;
 }
@@ -208,17 +209,20 @@ int fv_solve_cg(fv_linsys* ls, void* d_x, const fv_solve_opts* o, fv_solver_stat
     fv_solver_stats local{};
     int rc = guarded([&] {
         FV_REQUIRE(ls && ls->sp && d_x && o, "NULL argument");
-        FV_REQUIRE(o->rel_tol >= 0.0 && o->max_iter >= 0, "bad solver options");
+        FV_REQUIRE(o->rel_tol >= 0.0 && o->max_iter >= 0 && o->abs_tol >= 0.0,
+                   "bad solver options");
         fv_sparsity* sp = ls->sp;
         fv_ctx* ctx = sp->mesh->ctx;
         const int check = o->check_every > 0 ? o->check_every : 8;
         double* x = (double*)d_x;
+        double bnorm = 0.0;
         if (o->mode == FV_SOLVE_F64) {
             CgWork<double>& w = work_of<double>(ls, false);
             w.bind(ctx, ls, x);
-            CgState h = run_cg(ctx, w, o->rel_tol, o->max_iter, check);
+            CgState h = run_cg(ctx, w, o->rel_tol, o->max_iter, check, o->abs_tol);
             local.init_res = h.res0;
             local.final_res = h.res;
+            bnorm = h.bnorm;
             local.iters = h.it;
             local.outer_iters = 1;
             local.true_res = w.true_residual(ctx->stream);
@@ -228,9 +232,10 @@ int fv_solve_cg(fv_linsys* ls, void* d_x, const fv_solve_opts* o, fv_solver_stat
             IrOut r = run_mixed(ctx, w, o->rel_tol, o->max_iter,
                                 o->check_every > 0 ? o->check_every : 64,
                                 o->inner_rel_tol > 0.0 ? o->inner_rel_tol : 1e-3,
-                                o->max_outer > 0 ? o->max_outer : (1 << 30));
+                                o->max_outer > 0 ? o->max_outer : (1 << 30), o->abs_tol);
             local.init_res = r.init_res;
             local.final_res = r.res;
+            bnorm = r.bnorm;
             local.true_res = r.res;
             local.iters = r.iters;
             local.outer_iters = r.outer;
@@ -240,7 +245,8 @@ int fv_solve_cg(fv_linsys* ls, void* d_x, const fv_solve_opts* o, fv_solver_stat
         // convergence is judged on the solver's residual: the CG recurrence for
         // FP64 (as Ginkgo's residual-norm criterion), the explicit FP64
         // residual b - A x for iterative refinement
-        if (!(local.final_res <= o->rel_tol))
+        if (!(local.final_res <= o->rel_tol) &&
+            !(o->abs_tol > 0.0 && local.final_res * bnorm <= o->abs_tol))
             throw Error{FV_ENOCONVERGE, "CG stopped at relative residual " +
                                             std::to_string(local.final_res) + " after " +
                                             std::to_string(local.iters) + " iterations"};

@@ -1,6 +1,6 @@
 """Linear algebra handles with NeoN's la names: SparsityPattern, LinearSystem,
-computeResidual and the Jacobi-PCG solve (libfvhip fv_sparsity_*, fv_linsys_*,
-fv_solve_cg)."""
+computeResidual, the fvSolution mapping and the Jacobi-PCG / Jacobi-BiCGStab
+solves (libfvhip fv_sparsity_*, fv_linsys_*, fv_solve_cg, fv_solve_bicgstab)."""
 from __future__ import annotations
 
 import ctypes as C
@@ -133,14 +133,81 @@ class SolverStats:
     status: int
 
 
+# OpenFOAM solver / preconditioner names -> (Ginkgo type, solver run here)
+# (src/compatibility/fvSolution.cpp:19-48: updateSolver)
+SOLVER_MAP = {
+    "PCG": ("solver::Cg", "cg"),
+    "PBiCG": ("solver::Bicg", None),
+    "PBiCGStab": ("solver::Bicgstab", "bicgstab"),
+    "smoothSolver": ("solver::Bicgstab", "bicgstab"),
+    "GAMG": ("solver::Multigrid", None),
+}
+# fvSolution.cpp:50-104: updatePreconditioner
+PRECONDITIONER_MAP = {
+    "DIC": {"type": "preconditioner::Jacobi", "max_block_size": 1},
+    "DILU": {"type": "preconditioner::Ilu", "reverse_apply": False,
+             "factorization": {"type": "factorization::ParIlu"}},
+}
+
+
+def mapFvSolution(solver: dict) -> dict:
+    """FoamAdapter::mapFvSolution (src/compatibility/fvSolution.cpp:148-159): map an
+    OpenFOAM fvSolution solver dictionary to the NeoN/Ginkgo form (solver "Ginkgo",
+    type, preconditioner dictionary, criteria {iteration, relative_residual_norm,
+    absolute_residual_norm}); a dictionary with configFile is returned unchanged."""
+    d = dict(solver)
+    if "configFile" in d:
+        return d
+    name = d.get("solver")
+    if name in SOLVER_MAP:
+        d["solver"] = "Ginkgo"
+        d["type"] = SOLVER_MAP[name][0]
+    if "preconditioner" not in d:
+        d["preconditioner"] = dict(PRECONDITIONER_MAP["DIC"])
+    d.pop("smoother", None)
+    pre = d["preconditioner"]
+    if isinstance(pre, dict):
+        if isinstance(pre.get("type"), dict):
+            raise ValueError("GAMG is not supported in FoamAdapter, please use a different "
+                             "preconditioner.")
+    elif pre in PRECONDITIONER_MAP:
+        d["preconditioner"] = dict(PRECONDITIONER_MAP[pre])
+    crit = dict(d.get("criteria", {}))
+    crit["iteration"] = 1000
+    if "relTol" in d:
+        crit["relative_residual_norm"] = d.pop("relTol")
+    if "maxIter" in d:
+        crit["iteration"] = d.pop("maxIter")
+    if "tolerance" in d:
+        crit["absolute_residual_norm"] = d.pop("tolerance")
+    d["criteria"] = crit
+    return d
+
+
+def _method(d: dict) -> str:
+    """The solver that runs for a (mapped or plain) dictionary: 'cg' or 'bicgstab'."""
+    t = d.get("type")
+    name = d.get("solver", "PCG")
+    if t is None and name in SOLVER_MAP:
+        t = SOLVER_MAP[name][0]
+    if t in (None, "solver::Cg") or name == "CG":
+        return "cg"
+    if t == "solver::Bicgstab" or name in ("BiCGStab", "Bicgstab"):
+        return "bicgstab"
+    raise ValueError(f"solver {t or name!r} is not supported (CG and BiCGStab only)")
+
+
 def solver_opts(solver: dict | None) -> _lib.FvSolveOpts:
     """Map an fvSolution-style solver dictionary (src/compatibility/fvSolution.cpp:106-159:
-    relTol -> relative_residual_norm, maxIter -> iteration, default 1000) to solver options.
+    relTol -> relative_residual_norm, maxIter -> iteration, default 1000, tolerance ->
+    absolute_residual_norm), or its mapped Ginkgo form, to solver options.
     'precision': 'mixed' selects FP32 CG inside FP64 iterative refinement."""
     d = dict(solver or {})
+    crit = d.get("criteria", {})
     o = _lib.FvSolveOpts()
-    o.rel_tol = float(d.get("relTol", d.get("rel_tol", 1e-6)))
-    o.max_iter = int(d.get("maxIter", d.get("maxIters", d.get("max_iter", 1000))))
+    o.rel_tol = float(d.get("relTol", crit.get("relative_residual_norm", d.get("rel_tol", 1e-6))))
+    o.max_iter = int(d.get("maxIter", crit.get("iteration", d.get("maxIters", d.get("max_iter", 1000)))))
+    o.abs_tol = float(d.get("tolerance", crit.get("absolute_residual_norm", 0.0)))
     mode = d.get("precision", "double")
     if mode not in ("double", "mixed"):
         raise ValueError(f"unknown precision {mode!r}")
@@ -151,11 +218,11 @@ def solver_opts(solver: dict | None) -> _lib.FvSolveOpts:
     o.inner_rel_tol = float(d.get("innerRelTol", 1e-3))
     o.max_outer = int(d.get("maxOuter", 0))
     o.check_every = int(d.get("checkEvery", 32))
-    solver_name = d.get("solver", "PCG")
-    if solver_name not in ("PCG", "CG"):
-        raise ValueError(f"solver {solver_name!r} is not supported (PCG only)")
+    _method(d)
     pre = d.get("preconditioner", "DIC")
-    if pre not in ("DIC", "Jacobi", "diagonal"):
+    if isinstance(pre, dict):
+        pre = pre.get("type")
+    if pre not in ("DIC", "Jacobi", "diagonal", "preconditioner::Jacobi"):
         raise ValueError(f"preconditioner {pre!r} is not supported (point Jacobi only)")
     return o
 
@@ -219,9 +286,12 @@ class Batch:
 
 def solve_cg(ls: LinearSystem, x: DeviceArray, solver: dict | None = None,
              raise_on_noconverge: bool = True) -> SolverStats:
+    """Solve with the method the dictionary names (Jacobi-PCG by default, Jacobi-
+    BiCGStab for PBiCGStab / smoothSolver / solver::Bicgstab)."""
     o = solver_opts(solver)
     st = _lib.FvSolverStats()
-    rc = lib.fv_solve_cg(ls.handle, x.ptr, C.byref(o), C.byref(st))
+    fn = lib.fv_solve_bicgstab if _method(dict(solver or {})) == "bicgstab" else lib.fv_solve_cg
+    rc = fn(ls.handle, x.ptr, C.byref(o), C.byref(st))
     if rc not in (_lib.FV_OK, _lib.FV_ENOCONVERGE) or (rc != _lib.FV_OK and raise_on_noconverge):
         raise _lib.FvError(rc, _lib.last_error())
     return SolverStats(st.init_res, st.final_res, st.iters, st.outer_iters, st.true_res, rc)

@@ -326,6 +326,8 @@ typedef struct {
     double inner_rel_tol; /* FP32 inner CG tolerance per refinement step (F32_IR) */
     int32_t max_outer;    /* refinement steps (F32_IR) */
     int32_t check_every;  /* host polls the stop flag every this many iterations */
+    double abs_tol;       /* also stop when ||r|| <= abs_tol (criteria absolute_residual_norm,
+                             fvSolution.cpp:139-144); 0: off */
 } fv_solve_opts;
 typedef struct {
     double init_res;      /* SolverStats::initResNorm, relative */
@@ -341,6 +343,14 @@ typedef struct {
  * the convergence test in FP64.  Returns FV_ENOCONVERGE (stats filled) when
  * the tolerance is not reached. */
 int fv_solve_cg(fv_linsys* ls, void* d_x, const fv_solve_opts* opts, fv_solver_stats* stats);
+/* Jacobi-preconditioned BiCGStab on the assembled (non-symmetric) system, FP64
+ * (solver::Bicgstab + preconditioner::Jacobi, the mapping of PBiCGStab and
+ * smoothSolver in src/compatibility/fvSolution.cpp:22-27; used for implicit
+ * advection, test/test_advection.cpp:171-178).  Stops on the recurrence residual
+ * (relative or absolute); stats.true_res is the explicit FP64 ||b - A x|| / ||b||.
+ * FV_ENOCONVERGE on the iteration limit or a breakdown (rho or omega zero). */
+int fv_solve_bicgstab(fv_linsys* ls, void* d_x, const fv_solve_opts* opts,
+                      fv_solver_stats* stats);
 
 /* ---- pressure-velocity coupling (src/algorithms/pressureVelocityCoupling.cpp) ---
  * FP64.  The momentum matrix is the scalar system ls (the reference reads
