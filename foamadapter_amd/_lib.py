@@ -144,6 +144,8 @@ _SIGS = {
     "fv_update_velocity": [_vp, _vp, _vp, _vp, _vp, _vp],
     "fv_constrain_hbya": [_vp, _vp, _vp],
     "fv_linsys_boundary_coeffs": [_vp, C.POINTER(_vp), C.POINTER(_vp)],
+    "fv_courant_number": [_vp, _vp, _vp, _d, C.POINTER(_d)],
+    "fv_set_delta_t": [_d, _d, _d, _d, C.POINTER(_d)],
     "fv_solve_cg": [_vp, _vp, C.POINTER(FvSolveOpts), C.POINTER(FvSolverStats)],
     "fv_solve_bicgstab": [_vp, _vp, C.POINTER(FvSolveOpts), C.POINTER(FvSolverStats)],
     "fv_batch_create": [_vp, C.c_int, C.c_int, C.c_int, _pp],

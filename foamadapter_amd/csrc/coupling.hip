@@ -1,6 +1,8 @@
 // coupling.hip -- pressure-velocity coupling helpers of the PISO/SIMPLE loop
 // (src/algorithms/pressureVelocityCoupling.cpp:14-211): computeRAU,
-// computeRAUandHByA, updateFaceVelocity, updateVelocity, constrainHbyA.
+// computeRAUandHByA, updateFaceVelocity, updateVelocity, constrainHbyA; and the
+// time-step control of the transient solvers: the Courant number and setDeltaT
+// (src/auxiliary/setup.cpp:13-22).
 //
 // The reference scatters the off-diagonal products into HbyA with float
 // atomics (pressureVelocityCoupling.cpp:109-110), so its result depends on the
@@ -11,6 +13,10 @@
 // coefficients, "all the diagonal coefficients are the same"), the right-hand
 // side and the fields are Vec3 (xyz interleaved).
 #include "implicit.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <vector>
 
 using namespace fvhip;
 
@@ -118,6 +124,37 @@ __global__ __launch_bounds__(kBlock) void k_constrain(int B, const int8_t* __res
     for (int j = 0; j < 3; ++j) hbyab[3 * (int64_t)b + j] = Ub[3 * (int64_t)b + j];
 }
 
+// Courant number (OpenFOAM CourantNo.H, NeoN computeCoNum used at
+// examples/scalarAdvection/scalarAdvection.cpp:69): per cell the sum of |phi|
+// over all its faces (internal and boundary, fvc::surfaceSum(mag(phi))) over V;
+// each block writes the maximum of its cells (max is exact, so any order gives
+// the same result)
+__global__ __launch_bounds__(kBlock) void k_conum(int N, int F, const int32_t* __restrict__ off,
+                                                  const int32_t* __restrict__ idx,
+                                                  const double* __restrict__ phi,
+                                                  const double* __restrict__ V,
+                                                  double* __restrict__ blockMax) {
+    __shared__ double sh[kBlock];
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    double m = 0.0;
+    if (c < N) {
+        double s = 0.0;
+        const int e1 = off[c + 1];
+        for (int k = off[c]; k < e1; ++k) {
+            const int e = idx[k];
+            s += fabs(phi[e >= 0 ? e : ~e]);  // e >= F: boundary face e (phi is [F+B])
+        }
+        m = s / V[c];
+    }
+    sh[threadIdx.x] = m;
+    __syncthreads();
+    for (int w = kBlock / 2; w > 0; w >>= 1) {
+        if (threadIdx.x < w) sh[threadIdx.x] = fmax(sh[threadIdx.x], sh[threadIdx.x + w]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) blockMax[blockIdx.x] = sh[0];
+}
+
 void check_ls(const fv_linsys* ls) { FV_REQUIRE(ls && ls->sp && ls->sp->mesh, "NULL linear system"); }
 
 }  // namespace
@@ -191,6 +228,33 @@ int fv_constrain_hbya(const fv_bc* ubc, const void* d_Ub, void* d_hbya_b) {
         k_constrain<<<grid_for(m->nB, kBlock), kBlock, 0, m->ctx->stream>>>(
             m->nB, ubc->faceKind.p, (const double*)d_Ub, (double*)d_hbya_b);
         FV_LAUNCH_CHECK();
+    });
+}
+
+int fv_courant_number(fv_ctx* ctx, const fv_mesh* m, const void* d_phi, double dt, double* co) {
+    return guarded([&] {
+        FV_REQUIRE(ctx && m && m->ctx == ctx && d_phi && co, "bad arguments");
+        const int G = grid_for(m->nCells, kBlock);
+        DBuf<double> bm;
+        bm.alloc(ctx, G);
+        k_conum<<<G, kBlock, 0, ctx->stream>>>(m->nCells, m->nInt, m->cfOff.p, m->cfIdx.p,
+                                                (const double*)d_phi, m->V.p, bm.p);
+        FV_LAUNCH_CHECK();
+        std::vector<double> h(G);
+        bm.download(h.data());
+        double mx = 0.0;
+        for (double v : h) mx = std::max(mx, v);
+        *co = 0.5 * mx * dt;
+    });
+}
+
+int fv_set_delta_t(double dt, double co, double max_co, double max_dt, double* new_dt) {
+    return guarded([&] {
+        FV_REQUIRE(new_dt && dt > 0.0 && max_co > 0.0 && max_dt > 0.0, "bad arguments");
+        const double small = 1e-15;  // Foam::SMALL
+        const double maxFact = max_co / (co + small);
+        const double fact = std::min(std::min(maxFact, 1.0 + 0.1 * maxFact), 1.2);
+        *new_dt = std::min(fact * dt, max_dt);
     });
 }
 

@@ -192,3 +192,38 @@ def flux(exec: Context, mesh: Mesh, U: VolumeField, out: SurfaceField | None = N
     call("fv_flux", exec.handle, mesh.handle, prec_of(U.dtype), U.internal.ptr, U.boundary.ptr,
          out.values.ptr)
     return out
+
+
+def computeCoNum(phi: SurfaceField, dt: float) -> float:
+    """Courant number 0.5 * max_c(sum_faces |phi| / V) * dt (NeoN computeCoNum,
+    examples/scalarAdvection/scalarAdvection.cpp:69; OpenFOAM CourantNo.H)."""
+    import ctypes as C
+    m = phi.mesh
+    co = C.c_double()
+    call("fv_courant_number", m.ctx.handle, m.handle, phi.values.ptr, float(dt), C.byref(co))
+    return co.value
+
+
+def setDeltaT(dt: float, coNum: float, maxCo: float, maxDeltaT: float) -> float:
+    """New time step after setDeltaT (src/auxiliary/setup.cpp:13-22)."""
+    import ctypes as C
+    out = C.c_double()
+    call("fv_set_delta_t", float(dt), float(coNum), float(maxCo), float(maxDeltaT), C.byref(out))
+    return out.value
+
+
+def write(field: VolumeField, case_dir, time: str, name: str | None = None):
+    """FoamAdapter::write(volField, mesh, fieldName) (src/auxiliary/writers.cpp:66-135):
+    the field as an OpenFOAM ASCII file <case_dir>/<time>/<name>, internal values in
+    the caller's cell numbering (a renumbered mesh is mapped back; boundary faces
+    keep their positions under renumbering)."""
+    from .foamio import write_field
+    m = field.mesh
+    internal = field.internal
+    mapped = m.to_caller(internal, "cell") if m.cell_map is not None else internal
+    vals = mapped.numpy()
+    if mapped is not internal:
+        mapped.close()
+    b = field.boundary.numpy() if field.boundary is not None else None
+    patches = getattr(m, "all_patches", None) or m.patches
+    return write_field(case_dir, time, name or field.name, vals, patches, b)

@@ -279,6 +279,7 @@ class Mesh:
         call("fv_mesh_create_poly", ctx.handle, C.byref(d), C.byref(h))
         del keep
         m = cls(ctx, h, poly.fv_patches() if poly.patches else None)
+        m.all_patches = list(poly.patches)   # including empty ones (field files list them)
         if renumber:
             F = m.n_internal_faces
             bidx = np.arange(F, F + m.n_boundary_faces, dtype=np.int32)

@@ -377,6 +377,13 @@ int fv_update_velocity(fv_ctx* ctx, const fv_mesh* mesh, const void* d_hbya, con
 /* constrainHbyA (:14-36): HbyA_b = U_b on the faces of patches whose velocity
  * condition is not assignable (fixedValue); other faces keep their value. */
 int fv_constrain_hbya(const fv_bc* u_bc, const void* d_Ub, void* d_hbya_b);
+/* Courant number (NeoN computeCoNum, examples/scalarAdvection/scalarAdvection.cpp:69;
+ * OpenFOAM CourantNo.H): co = 0.5 * max_c(sum_faces |phi| / V_c) * dt over the
+ * internal and boundary faces of each cell, d_phi [F+B] FP64. */
+int fv_courant_number(fv_ctx* ctx, const fv_mesh* mesh, const void* d_phi, double dt, double* co);
+/* setDeltaT (src/auxiliary/setup.cpp:13-22): new_dt = min(f * dt, max_dt) with
+ * f = min(min(maxCo/(co + SMALL), 1 + 0.1 maxCo/(co + SMALL)), 1.2).  Host only. */
+int fv_set_delta_t(double dt, double co, double max_co, double max_dt, double* new_dt);
 /* The boundary coefficients [B] the laplacian assembly of ls recorded
  * (la::BoundaryCoefficients{matrixValues, rhsValues}, read at :180-192);
  * zeroed by fv_linsys_zero, accumulated by fv_assemble_laplacian, overwritten

@@ -29,6 +29,7 @@
 //                             explicit residual ||b-Ax||/||b|| <= tol.
 //   rAU / HbyA .............. the matrix form rAU = V/diag(A), HbyA = (b - (A -
 //                             diag A) U) rAU/V evaluated with scipy.
+//   Courant number .......... closed form u*dt/h for uniform flow on a uniform mesh
 //   face/cell velocity update the corrected flux of a solved pressure equation
 //                             is divergence-free (surfaceIntegrate(phi) ~ 0);
 //                             U = HbyA - rAU grad(p) against the grad oracle.
@@ -663,6 +664,21 @@ int orc_constrain_hbya(int nB, int nPatches, const int32_t* poff, const int32_t*
         if (ukind[patch_of(b, nPatches, poff)] == BC_FIXED_VALUE)
             for (int j = 0; j < 3; ++j) hbyab[3 * b + j] = Ub[3 * b + j];
     return 0;
+}
+
+// Courant number (OpenFOAM CourantNo.H): 0.5 * max_c(sum over the cell's faces
+// of |phi| / V_c) * dt, internal and boundary faces, summed in face order
+double orc_courant(int nCells, int nInt, int nB, const int32_t* own, const int32_t* nbr,
+                   const int32_t* faceCells, const double* phi, const double* V, double dt) {
+    std::vector<double> s(nCells, 0.0);
+    for (int f = 0; f < nInt; ++f) {
+        s[own[f]] += std::fabs(phi[f]);
+        s[nbr[f]] += std::fabs(phi[f]);
+    }
+    for (int b = 0; b < nB; ++b) s[faceCells[b]] += std::fabs(phi[nInt + b]);
+    double mx = 0.0;
+    for (int c = 0; c < nCells; ++c) mx = std::max(mx, s[c] / V[c]);
+    return 0.5 * mx * dt;
 }
 
 }  // extern "C"

@@ -314,6 +314,16 @@ def update_velocity(hbya_, rAU, gradp):
     return out
 
 
+lib.orc_courant.restype = C.c_double
+lib.orc_courant.argtypes = [C.c_int] * 3 + [C.c_void_p] * 5 + [C.c_double]
+
+
+def courant(g, phi, dt):
+    N, F, B = _sizes(g)
+    return lib.orc_courant(N, F, B, P(g["owner"]), P(g["neighbour"]), P(g["faceCells"]),
+                           P(_f(phi)), P(g["V"]), float(dt))
+
+
 def constrain_hbya(g, ukinds, Ub, hbyab):
     k, poff = _kinds(g, ukinds)
     out = np.array(hbyab, dtype=np.float64, copy=True)

@@ -63,3 +63,16 @@ def test_foam_dict_parser():
     assert d["solvers"]["p"] == {"solver": "PCG", "relTol": 0.01}
     assert d["solvers"]["U|k"]["solver"] == "smoothSolver"
     assert d["PISO"]["nCorrectors"] == 2 and "FoamFile" not in d
+
+
+def test_set_delta_t_formula():
+    """setDeltaT (src/auxiliary/setup.cpp:13-22), host arithmetic of the library."""
+    from foamadapter_amd.fvcc import setDeltaT
+    small = 1e-15
+    for dt, co, maxco, maxdt in [(1e-3, 0.25, 0.5, 1.0), (1e-3, 1.0, 0.5, 1.0),
+                                 (1e-3, 0.0, 0.5, 1.0), (1e-3, 0.49, 0.5, 1.1e-3)]:
+        f = maxco / (co + small)
+        want = min(min(min(f, 1.0 + 0.1 * f), 1.2) * dt, maxdt)
+        assert setDeltaT(dt, co, maxco, maxdt) == want
+    assert setDeltaT(1e-3, 0.25, 0.5, 1.0) == pytest.approx(1.2e-3, rel=1e-14)   # growth cap 1.2
+    assert setDeltaT(1e-3, 1.0, 0.5, 1.0) == pytest.approx(0.5e-3, rel=1e-14)   # maxCo/Co

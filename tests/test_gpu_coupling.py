@@ -130,3 +130,12 @@ def test_corrected_flux_divergence_free_on_gpu(ctx, case):
     ref = orc.update_face_velocity(g, orc.sparsity(g), ls.values(), bm, br,
                                    phiH.values.numpy(), p.internal.numpy())
     assert np.array_equal(ph, ref)
+
+
+def test_courant_number_matches_oracle(ctx, case):
+    from foamadapter_amd import fvcc
+    poly, m, g = case
+    N, F, B = len(g["V"]), len(g["owner"]), len(g["faceCells"])
+    phi = orc.uniform(F + B, -1, 1, 12, 0)
+    co = fvcc.computeCoNum(fvcc.SurfaceField(m, ctx.upload(phi)), 1e-3)
+    assert co == orc.courant(g, phi, 1e-3)

@@ -378,3 +378,20 @@ def test_constrain_hbya_fixed_value_patches_only():
     for p in range(6):
         sl = slice(po[p], po[p + 1])
         assert np.array_equal(out[sl], Ub[sl] if kinds[p] == 0 else hb[sl])
+
+
+def test_courant_number_closed_form():
+    """Uniform flow u along x on a uniform mesh of spacing h: every interior cell
+    has |phi| = u h^2 on its two x faces, so Co = u dt / h; a transverse flow adds."""
+    n = 8
+    poly = PolyMesh.hex(n)
+    g = orc.geometry(poly)
+    N, B = len(g["V"]), len(g["faceCells"])
+    h = 1.0 / n
+    u, dt = 2.5, 1e-2
+    U = np.tile([u, 0.0, 0.0], (N, 1))
+    phi = orc.flux(g, U, np.tile([u, 0.0, 0.0], (B, 1)))
+    assert abs(orc.courant(g, phi, dt) - u * dt / h) <= 1e-12 * (u * dt / h)
+    U2 = np.tile([u, -1.5, 0.5], (N, 1))
+    phi2 = orc.flux(g, U2, np.tile([u, -1.5, 0.5], (B, 1)))
+    assert abs(orc.courant(g, phi2, dt) - (u + 1.5 + 0.5) * dt / h) <= 1e-12
