@@ -292,8 +292,7 @@ struct LapOp : PlainOp<S, LapOp<S>> {
 // faces, the face state (kState, NS components) or the contribution (NC).
 template <typename S, typename Op, bool kState>
 __device__ inline void gather_cell(int c, int fs, int nUp, const Split& t, const Op& op,
-                                   const S* __restrict__ V, S coeff, S* __restrict__ out,
-                                   const S* st0, int stride) {
+                                   S vol, S coeff, S* __restrict__ out, const S* st0, int stride) {
     constexpr int NC = Op::NC, NS = kState ? Op::NS : Op::NC;
     S acc[NC];
     for (int j = 0; j < NC; ++j) acc[j] = S(0);
@@ -336,7 +335,6 @@ __device__ inline void gather_cell(int c, int fs, int nUp, const Split& t, const
         op.bface(t.bFace[k], c, v);
         for (int j = 0; j < NC; ++j) acc[j] += v[j];
     }
-    const S vol = V[c];
     for (int j = 0; j < NC; ++j) out[(int64_t)NC * c + j] = coeff * (acc[j] / vol);
 }
 
@@ -358,6 +356,14 @@ __global__ __launch_bounds__(THREADS) void k_gather_stencil(int N, int nBlocks, 
     const int c0 = lb * CELLS, c1 = min(c0 + CELLS, N);
     const int fs = t.ownStart[c0], fe = t.ownStart[c1];
     const int nUp = min(fe - fs, kMaxUp);
+    // the volumes phase 2 divides by, loaded now so their latency hides behind phase 1
+    constexpr int kPer = CELLS / THREADS;
+    S vols[kPer];
+#pragma unroll
+    for (int q = 0; q < kPer; ++q) {
+        const int c = c0 + (int)threadIdx.x + q * THREADS;
+        vols[q] = c < c1 ? V[c] : S(1);
+    }
     // kU faces per thread in flight: their loads are issued together (lanes past
     // the range work on the last face and store nothing)
     for (int i0 = threadIdx.x; i0 < nUp; i0 += THREADS * kU) {
@@ -379,8 +385,11 @@ __global__ __launch_bounds__(THREADS) void k_gather_stencil(int N, int nBlocks, 
         }
     }
     __syncthreads();
-    for (int c = c0 + threadIdx.x; c < c1; c += THREADS)
-        gather_cell<S, Op, false>(c, fs, nUp, t, op, V, coeff, out, flux[0], kMaxUp);
+#pragma unroll
+    for (int q = 0; q < kPer; ++q) {
+        const int c = c0 + (int)threadIdx.x + q * THREADS;
+        if (c < c1) gather_cell<S, Op, false>(c, fs, nUp, t, op, vols[q], coeff, out, flux[0], kMaxUp);
+    }
 }
 
 struct Lists {
@@ -423,6 +432,14 @@ __global__ __launch_bounds__(THREADS) void k_gather(int N, int nBlocks, int perX
     const int e0 = L.extOff[lb], e1 = L.extOff[lb + 1];
     const int b0 = L.bndOff[lb], b1 = L.bndOff[lb + 1];
     const bool fits = nF <= kMaxUp && e1 - e0 <= kMaxExt && b1 - b0 <= kMaxBnd;
+    // the volumes phase 2 divides by, loaded now so their latency hides behind phase 1
+    constexpr int kPer = CELLS / THREADS;
+    S vols[kPer];
+#pragma unroll
+    for (int q = 0; q < kPer; ++q) {
+        const int c = c0 + (int)threadIdx.x + q * THREADS;
+        vols[q] = c < c1 ? V[c] : S(1);
+    }
     for (int l = threadIdx.x; l < CELLS; l += THREADS) {
         upS[l] = upE[l] = exS[l] = exE[l] = bS[l] = bE[l] = 0;
         lowN[l] = 0;
@@ -484,11 +501,17 @@ __global__ __launch_bounds__(THREADS) void k_gather(int N, int nBlocks, int perX
     }
     __syncthreads();
     if (overflow) {
-        for (int c = c0 + threadIdx.x; c < c1; c += THREADS)
-            gather_cell<S, Op, true>(c, fs, nUp, t, op, V, coeff, out, fluxU[0], kMaxUp);
+#pragma unroll
+        for (int q = 0; q < kPer; ++q) {
+            const int c = c0 + (int)threadIdx.x + q * THREADS;
+            if (c < c1) gather_cell<S, Op, true>(c, fs, nUp, t, op, vols[q], coeff, out, fluxU[0], kMaxUp);
+        }
         return;
     }
-    for (int c = c0 + threadIdx.x; c < c1; c += THREADS) {
+#pragma unroll
+    for (int q = 0; q < kPer; ++q) {
+        const int c = c0 + (int)threadIdx.x + q * THREADS;
+        if (c >= c1) break;
         const int l = c - c0;
         S acc[NC];
         for (int j = 0; j < NC; ++j) acc[j] = S(0);
@@ -526,7 +549,7 @@ __global__ __launch_bounds__(THREADS) void k_gather(int N, int nBlocks, int perX
             op.bcontrib(L.bndFace[b0 + k], sv, v);
             for (int j = 0; j < NC; ++j) acc[j] += v[j];
         }
-        const S vol = V[c];
+        const S vol = vols[q];
         for (int j = 0; j < NC; ++j) out[(int64_t)NC * c + j] = coeff * (acc[j] / vol);
     }
 }
