@@ -35,12 +35,6 @@ namespace {
 
 constexpr int kB = kCgBlock;
 
-__global__ void kb_zero(int64_t n, uint8_t* __restrict__ p) {
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-         i += (int64_t)gridDim.x * blockDim.x)
-        p[i] = 0;
-}
-
 }  // namespace
 // ---- host side ----------------------------------------------------------------
 
@@ -113,6 +107,8 @@ void build_tables(fv_batch* b) {
         p.ntiles = (int)ht.size() - p.tile0;
         p.sellOff = sp->sellOff.p;
         p.sellCol = sp->sellCol.p;
+        p.sellSrc = sp->sellSrc.p;
+        p.diag = sp->diagOffset.p;
         p.sval = (const S*)m->sval.p;
         p.rowOffs = sp->rowOffs.p;
         p.colIdxs = sp->colIdxs.p;
@@ -226,23 +222,16 @@ void start_members(fv_batch* b) {
     fv_ctx* ctx = b->ctx;
     hipStream_t s = ctx->stream;
     std::vector<CgTile> st;
-    // tile list of the members that (re)start, with their global slots
+    std::vector<CgStartParam> prm(b->members.size());
+    // tile list of the members that (re)start, with their global slots, and
+    // their start parameters; one batched start launch covers all of them
     int slot = 0;
     for (int i = 0; i < (int)b->members.size(); ++i) {
         BMember* m = b->members[i];
         if (m->needStart) {
             member_tile_list<S>(b, i, slot, st);
-            const fv_sparsity* sp = m->ls->sp;
-            kc_sell_values<S><<<grid_for(sp->sellTotal, kB, 8192), kB, 0, s>>>(
-                sp->sellTotal, sp->sellSrc.p, m->ls->values.p, (S*)m->sval.p);
-            kc_dinv<S><<<grid_for(m->N, kB), kB, 0, s>>>(m->N, sp->rowOffs.p, sp->diagOffset.p,
-                                                         m->ls->values.p,
-                                                         (S*)m->vec.p + 5 * (int64_t)m->N);
-            kb_zero<<<grid_for(m->N * (int64_t)sizeof(S), kB, 4096), kB, 0, s>>>(
-                m->N * (int64_t)sizeof(S), m->vec.p);  // d = 0
-            kc_init_state<<<1, 64, 0, s>>>(b->states.p + m->slot, 1,
-                                           m->tol > 0.0 ? m->tol : b->tol,
-                                           m->maxit > 0 ? m->maxit : b->maxit, b->delta);
+            prm[i] = CgStartParam{m->tol > 0.0 ? m->tol : b->tol, b->delta, 0.0,
+                                  m->maxit > 0 ? m->maxit : b->maxit, 0};
             m->needStart = false;
         }
         slot += member_tiles(b, m);
@@ -250,8 +239,12 @@ void start_members(fv_batch* b) {
     if (st.empty()) return;
     if (b->startTiles.n < st.size() * sizeof(CgTile))
         b->startTiles.alloc(ctx, next_pow2((int)st.size()) * sizeof(CgTile));
+    if (b->startParams.n < prm.size() * sizeof(CgStartParam))
+        b->startParams.alloc(ctx, next_pow2((int)prm.size()) * sizeof(CgStartParam));
     if (!b->startGridDev.p) b->startGridDev.alloc(ctx, sizeof(CgGrid<S>));
     FV_HIP(hipMemcpyAsync(b->startTiles.p, st.data(), st.size() * sizeof(CgTile),
+                          hipMemcpyHostToDevice, s));
+    FV_HIP(hipMemcpyAsync(b->startParams.p, prm.data(), prm.size() * sizeof(CgStartParam),
                           hipMemcpyHostToDevice, s));
     const CgGrid<S> hg{(const CgTile*)b->startTiles.p, (const CgSys<S>*)b->probs.p,
                        (int)st.size(), 0};
@@ -259,6 +252,7 @@ void start_members(fv_batch* b) {
     const auto* g = (const CgGrid<S>*)b->startGridDev.p;
     const CgPartials red = partials_of<S>(b);
     const int G = cg_grid((int)st.size());
+    kc_start_tile<S><<<G, kB, 0, s>>>(g, (const CgStartParam*)b->startParams.p);
     kc_resid<S, 1><<<G, kB, 0, s>>>(g, red);
     kc_replace<S, 1><<<G, kB, 0, s>>>(g, red);
     kc_direction<S, 2><<<G, kB, 0, s>>>(g, red);

@@ -37,7 +37,7 @@ struct fv_batch {
     // grow, and the captured chunk graph stays valid until they do
     fvhip::DBuf<uint8_t> probs;
     fvhip::DBuf<uint8_t> tiles;
-    fvhip::DBuf<uint8_t> startTiles;
+    fvhip::DBuf<uint8_t> startTiles, startParams;
     fvhip::DBuf<uint8_t> gridDev, startGridDev;  // launch descriptions (CgGrid)
     fvhip::DBuf<fvhip::CgState> states;
     fvhip::DBuf<double> partials;

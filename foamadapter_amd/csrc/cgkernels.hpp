@@ -37,6 +37,8 @@ struct CgSys {
     const int64_t* sellOff;         // sliced ELL, 64-row slices, entries column-major
     const int32_t* sellCol;
     const S* sval;
+    const int32_t* sellSrc;         // CSR index of each sliced-ELL entry (-1: padding)
+    const int32_t* diag;            // CSR slot of the diagonal within each row
     const int32_t* rowOffs;         // FP64 CSR (residual replacement)
     const int32_t* colIdxs;
     const double* vals;
@@ -294,6 +296,45 @@ __global__ __launch_bounds__(kCgBlock) void kc_resid_norm(const CgSys<S>* __rest
         if (!(bn > 0.0)) bn = 1.0;
         out->bnorm = bn;
         out->res = sqrt(tot[0]) / bn;
+    }
+}
+
+// Start parameters of one system (batched start)
+struct CgStartParam {
+    double tol, delta, atol;
+    int32_t maxit, pad;
+};
+
+// Batched start, one workgroup per tile of the (re)starting systems: the
+// tile's sliced-ELL values and Jacobi diagonal in S from the FP64 matrix, d = 0,
+// and (first tile) the system's CG state.  Tile rows are multiples of 64, so a
+// tile owns whole slices.
+template <typename S>
+__global__ __launch_bounds__(kCgBlock) void kc_start_tile(const CgGrid<S>* __restrict__ g,
+                                                          const CgStartParam* __restrict__ prm) {
+    const CgSys<S>* Pp;
+    int row0, row1, slot;
+    if (!cg_locate(g, Pp, row0, row1, slot)) return;
+    const CgSys<S>& P = *Pp;
+    S* sval = const_cast<S*>(P.sval);
+    const int64_t e0 = P.sellOff[row0 >> 6], e1 = P.sellOff[row1 >> 6];
+    for (int64_t e = e0 + threadIdx.x; e < e1; e += kCgBlock) {
+        const int32_t j = P.sellSrc[e];
+        sval[e] = j >= 0 ? (S)P.vals[j] : S(0);
+    }
+    const int r1 = min(row1, P.N);
+    for (int i = row0 + threadIdx.x; i < r1; i += kCgBlock) {
+        P.dinv[i] = (S)(1.0 / P.vals[P.rowOffs[i] + P.diag[i]]);
+        P.d[i] = S(0);
+    }
+    if (row0 == 0 && threadIdx.x == 0) {
+        const CgStartParam q = prm[Pp - g->probs];
+        CgState s{};
+        s.tol = q.tol;
+        s.atol = q.atol;
+        s.maxit = q.maxit;
+        s.delta = q.delta;
+        *P.st = s;
     }
 }
 

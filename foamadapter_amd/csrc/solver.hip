@@ -57,6 +57,8 @@ struct CgWork : fv_solver_cache {
         h.ntiles = tiles;
         h.sellOff = sp->sellOff.p;
         h.sellCol = sp->sellCol.p;
+        h.sellSrc = sp->sellSrc.p;
+        h.diag = sp->diagOffset.p;
         h.sval = sval.p;
         h.rowOffs = sp->rowOffs.p;
         h.colIdxs = sp->colIdxs.p;
