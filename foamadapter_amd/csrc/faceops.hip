@@ -166,11 +166,10 @@ __global__ __launch_bounds__(kBlock) void k_laplacian(
 // result is the same bit for bit.  On a Z-order renumbered mesh a workgroup is a
 // compact brick (8x8x4, 8x8x8 or 16x8x8 cells) and only the faces on its lower
 // sides are recomputed; larger bricks recompute fewer faces per cell.
-// Workgroups are dispatched round-robin over the 8 XCDs; the block index is
-// remapped so each XCD walks one contiguous range of bricks and keeps the
-// faces it recomputes in its own L2.
+// Workgroups run in their natural order, which the hardware deals round-robin
+// over the 8 XCDs: measured at 256^3 this is as fast as or faster (FP64
+// gradient +7%) than remapping each XCD onto one contiguous range of bricks.
 
-constexpr int kXcd = 8;
 constexpr int kU = 4;  // faces per thread whose loads a staged-gather phase 1 keeps in flight
 // LDS face slots of a CELLS-cell workgroup: 3 owned faces per hex cell plus slack;
 // faces beyond it (unusual meshes) are recomputed in phase 2
@@ -344,14 +343,14 @@ __device__ inline void gather_cell(int c, int fs, int nUp, const Split& t, const
 // is the faster variant for the gradient (three components per face), whose
 // phase 2 would otherwise re-read Sf per face.
 template <typename S, typename Op, int CELLS, int THREADS>
-__global__ __launch_bounds__(THREADS) void k_gather_stencil(int N, int nBlocks, int perXcd,
+__global__ __launch_bounds__(THREADS) void k_gather_stencil(int N, int nBlocks,
                                                             Split t, Op op,
                                                             const S* __restrict__ V, S coeff,
                                                             S* __restrict__ out) {
     constexpr int NC = Op::NC;
     constexpr int kMaxUp = max_up<CELLS>();
     __shared__ S flux[NC][kMaxUp];
-    const int lb = (blockIdx.x % kXcd) * perXcd + blockIdx.x / kXcd;
+    const int lb = (int)blockIdx.x;
     if (lb >= nBlocks) return;
     const int c0 = lb * CELLS, c1 = min(c0 + CELLS, N);
     const int fs = t.ownStart[c0], fe = t.ownStart[c1];
@@ -410,7 +409,7 @@ constexpr int kMaxLow = 8;  // lower faces per cell from inside the workgroup, i
 // workgroup whose faces do not fit the LDS arrays takes the per-cell path of
 // gather_cell instead; both give the same bits.
 template <typename S, typename Op, int CELLS, int THREADS>
-__global__ __launch_bounds__(THREADS) void k_gather(int N, int nBlocks, int perXcd, Split t,
+__global__ __launch_bounds__(THREADS) void k_gather(int N, int nBlocks, Split t,
                                                     Lists L, Op op, const S* __restrict__ V,
                                                     S coeff, S* __restrict__ out) {
     constexpr int NC = Op::NC, NS = Op::NS;
@@ -424,7 +423,7 @@ __global__ __launch_bounds__(THREADS) void k_gather(int N, int nBlocks, int perX
     __shared__ uint16_t lowL[CELLS][kMaxLow];
     __shared__ int lowN[CELLS];
     __shared__ int overflow;
-    const int lb = (blockIdx.x % kXcd) * perXcd + blockIdx.x / kXcd;
+    const int lb = (int)blockIdx.x;
     if (lb >= nBlocks) return;
     const int c0 = lb * CELLS, c1 = min(c0 + CELLS, N);
     const int fs = t.ownStart[c0], fe = t.ownStart[c1];
@@ -562,10 +561,9 @@ Split split_of(const fv_mesh* m) {
 template <typename S, typename Op, int CELLS, int THREADS>
 void launch_gather_t(fv_ctx* ctx, const fv_mesh* m, const Op& op, const S* V, S coeff, S* out) {
     const int nBlocks = (m->nCells + CELLS - 1) / CELLS;
-    const int perXcd = (nBlocks + kXcd - 1) / kXcd;
     if (!Op::kLists) {
-        k_gather_stencil<S, Op, CELLS, THREADS><<<perXcd * kXcd, THREADS, 0, ctx->stream>>>(
-            m->nCells, nBlocks, perXcd, split_of(m), op, V, coeff, out);
+        k_gather_stencil<S, Op, CELLS, THREADS><<<nBlocks, THREADS, 0, ctx->stream>>>(
+            m->nCells, nBlocks, split_of(m), op, V, coeff, out);
         FV_LAUNCH_CHECK();
         return;
     }
@@ -573,8 +571,8 @@ void launch_gather_t(fv_ctx* ctx, const fv_mesh* m, const Op& op, const S* V, S 
     const GatherLists& gl = gather_lists(const_cast<fv_mesh*>(m), CELLS);
     const Lists L{gl.extOff.p, gl.extFace.p, gl.extOwn.p, gl.bndOff.p, gl.bndFace.p,
                   gl.extCell.p, gl.bndCell.p};
-    k_gather<S, Op, CELLS, THREADS><<<perXcd * kXcd, THREADS, 0, ctx->stream>>>(
-        m->nCells, nBlocks, perXcd, split_of(m), L, op, V, coeff, out);
+    k_gather<S, Op, CELLS, THREADS><<<nBlocks, THREADS, 0, ctx->stream>>>(
+        m->nCells, nBlocks, split_of(m), L, op, V, coeff, out);
     FV_LAUNCH_CHECK();
 }
 
