@@ -56,11 +56,11 @@ OPS = ["interp_linear", "grad", "div_linear", "div_upwind", "laplacian", "sngrad
 KERNEL_OF = {
     ("interp_linear", "f64"): "k_interp<double>", ("interp_linear", "f32"): "k_interp<float>",
     ("sngrad", "f64"): "k_sngrad<double>", ("sngrad", "f32"): "k_sngrad<float>",
-    ("grad", "f64"): "k_gather_stencil<double, GradOp<double>, 512, 512>",
-    ("grad", "f32"): "k_gather<float, GradListOp<float>, 256, 256>",
-    ("div_linear", "f64"): "k_gather<double, DivOp<double, 0>, 256, 256>",
+    ("grad", "f64"): "k_gather<double, GradOp<double>, 256, 256>",
+    ("grad", "f32"): "k_gather<float, GradOp<float>, 256, 256>",
+    ("div_linear", "f64"): "k_gather<double, DivOp<double, 0>, 512, 512>",
     ("div_linear", "f32"): "k_gather<float, DivOp<float, 0>, 256, 256>",
-    ("div_upwind", "f64"): "k_gather<double, DivOp<double, 1>, 256, 256>",
+    ("div_upwind", "f64"): "k_gather<double, DivOp<double, 1>, 512, 512>",
     ("div_upwind", "f32"): "k_gather<float, DivOp<float, 1>, 256, 256>",
     ("laplacian", "f64"): "k_gather<double, LapOp<double>, 256, 256>",
     ("laplacian", "f32"): "k_gather<float, LapOp<float>, 256, 256>",

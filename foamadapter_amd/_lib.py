@@ -11,7 +11,8 @@ import os
 from pathlib import Path
 
 _HERE = Path(__file__).resolve().parent
-LIB_PATH = _HERE / "libfvhip.so"
+# FVHIP_LIB names another build of the library (e.g. one made with a different ARCH)
+LIB_PATH = Path(os.environ.get("FVHIP_LIB") or _HERE / "libfvhip.so")
 
 # status codes (include/fvhip.h)
 FV_OK = 0

@@ -33,6 +33,13 @@ Geo<float> geo32(fv_mesh* m) {
     return {g.V.p, g.Sf.p, g.magSf.p, g.w.p, g.nodc.p, g.Sfb.p, g.magSfb.p, g.dcb.p};
 }
 
+// Geometry of the staged gather's external lower faces, packed in list order
+// (record k of the per-workgroup lists): w, magSf, nonOrthDeltaCoeffs, Sf.
+template <typename S>
+struct ExGeo {
+    const S *w, *magSf, *nodc, *Sf;
+};
+
 // ---- cell-result operators -------------------------------------------------
 
 template <typename S>
@@ -154,23 +161,21 @@ __global__ __launch_bounds__(kBlock) void k_laplacian(
 
 // ---- LDS-staged cell gathers (meshes in upper-triangular face order) --------
 //
-// A workgroup owns CELLS consecutive cells [c0, c1) (256, 512 or 1024).  In OpenFOAM face order the
-// internal faces those cells own are one contiguous range [fs, fe), so:
-//   phase 1: the workgroup streams that face range with coalesced loads,
-//            computes every face contribution once and keeps it in LDS;
-//   phase 2: each thread takes cells in turn and accumulates, in ascending face order, its
-//            lower faces (owned by lower-numbered cells: read from LDS when the
-//            owner is in this workgroup, recomputed otherwise), its own faces
-//            (from LDS), then its boundary faces, and divides by V.
-// The accumulation order is the serial face order of the other kernels, so the
+// A workgroup owns CELLS consecutive cells [c0, c1) (256, 512 or 1024).  In
+// OpenFOAM face order the internal faces those cells own are one contiguous
+// range [fs, fe), streamed with coalesced loads; the faces owned by cells
+// before c0 whose neighbour is in [c0, c1) (external lower faces) and the
+// boundary faces of [c0, c1) come from per-workgroup lists built once per mesh
+// (GatherLists, with the external faces' geometry packed in list order).  Every
+// contribution is computed once into LDS, then each cell accumulates its faces
+// in ascending face order -- the serial face order of the other kernels, so the
 // result is the same bit for bit.  On a Z-order renumbered mesh a workgroup is a
-// compact brick (8x8x4, 8x8x8 or 16x8x8 cells) and only the faces on its lower
-// sides are recomputed; larger bricks recompute fewer faces per cell.
+// compact brick (8x8x4, 8x8x8 or 16x8x8 cells), and only the faces on its lower
+// sides are external.
 // Workgroups run in their natural order, which the hardware deals round-robin
 // over the 8 XCDs: measured at 256^3 this is as fast as or faster (FP64
 // gradient +7%) than remapping each XCD onto one contiguous range of bricks.
 
-constexpr int kU = 4;  // faces per thread whose loads a staged-gather phase 1 keeps in flight
 // LDS face slots of a CELLS-cell workgroup: 3 owned faces per hex cell plus slack;
 // faces beyond it (unusual meshes) are recomputed in phase 2
 template <int CELLS>
@@ -180,119 +185,132 @@ struct Split {
     const int32_t *own, *nbr, *ownStart, *lowOff, *lowFace, *lowOwn, *bOff, *bFace;
 };
 
-// An operator gives each face a contribution of NC components.  The staged
-// gather keeps a face's NS-component state in LDS (state/bstate) and expands it
-// to the contribution when it accumulates (contrib/bcontrib): the gradient keeps
-// the face value (8 bytes instead of 24) and multiplies by Sf when it adds.
+// An operator gives each face a contribution of NC components.  For the
+// staged gather it splits a face into its loads (load/eload/bload: global reads
+// only, into an In/BIn record) and the arithmetic on them (make/bmake), so the
+// kernel can issue the loads of every face it holds -- own, external lower and
+// boundary -- before it waits for any.  The arithmetic is the serial kernels'
+// own expression, so the bits are the same.  face/bface (make of load) serve
+// the per-cell fallback.
+template <typename S, typename D, int NCOMP>
+struct FaceOp {
+    static constexpr int NC = NCOMP;
+    static constexpr int kDefaultCells = 256;
+    // own faces per thread and pass of k_gather (measured at 256^3, 256-cell
+    // workgroups: 3 beats 4 and 2 -- loads in flight against registers)
+    static constexpr int kFacesPerPass = 3;
+    __device__ const D& self() const { return *static_cast<const D*>(this); }
+    __device__ void face(int f, int o, int n, S* v) const { self().make(self().load(f, o, n), v); }
+    __device__ void bface(int b, int c, S* v) const { self().bmake(self().bload(b, c), v); }
+};
+
 template <typename S>
-struct GradOp {
-    static constexpr int NC = 3, NS = 1;
-    static constexpr bool kLists = false;  // runs k_gather_stencil
-    static constexpr int kDefaultCells = 512;
+struct GradOp : FaceOp<S, GradOp<S>, 3> {
     const S *T, *w, *Sf, *Sfb, *Tb;
-    __device__ void state(int f, int o, int n, S* s) const {
-        const S TP = T[o], TN = T[n];
-        s[0] = w[f] * (TP - TN) + TN;
+    struct In {
+        S w, TP, TN, sx, sy, sz;
+    };
+    struct BIn {
+        S tb, sx, sy, sz;
+    };
+    __device__ In load(int f, int o, int n) const {
+        const int64_t f3 = 3 * (int64_t)f;
+        return {w[f], T[o], T[n], Sf[f3], Sf[f3 + 1], Sf[f3 + 2]};
     }
-    __device__ void contrib(int f, const S* s, S* v) const {
-        v[0] = Sf[3 * (int64_t)f] * s[0];
-        v[1] = Sf[3 * (int64_t)f + 1] * s[0];
-        v[2] = Sf[3 * (int64_t)f + 2] * s[0];
+    __device__ In eload(int k, int, int o, int n, const ExGeo<S>& x) const {
+        const int64_t k3 = 3 * (int64_t)k;
+        return {x.w[k], T[o], T[n], x.Sf[k3], x.Sf[k3 + 1], x.Sf[k3 + 2]};
     }
-    __device__ void bstate(int b, int, S* s) const { s[0] = Tb[b]; }
-    __device__ void bcontrib(int b, const S* s, S* v) const {
-        v[0] = Sfb[3 * b] * s[0];
-        v[1] = Sfb[3 * b + 1] * s[0];
-        v[2] = Sfb[3 * b + 2] * s[0];
+    __device__ void make(const In& a, S* v) const {
+        const S s0 = a.w * (a.TP - a.TN) + a.TN;
+        v[0] = a.sx * s0;
+        v[1] = a.sy * s0;
+        v[2] = a.sz * s0;
     }
-    __device__ void face(int f, int o, int n, S* v) const {
-        S s[NS];
-        state(f, o, n, s);
-        contrib(f, s, v);
+    __device__ BIn bload(int b, int) const { return {Tb[b], Sfb[3 * b], Sfb[3 * b + 1], Sfb[3 * b + 2]}; }
+    __device__ void bmake(const BIn& a, S* v) const {
+        v[0] = a.sx * a.tb;
+        v[1] = a.sy * a.tb;
+        v[2] = a.sz * a.tb;
     }
-    __device__ void bface(int b, int c, S* v) const {
-        S s[NS];
-        bstate(b, c, s);
-        bcontrib(b, s, v);
-    }
-};
-
-// The gradient on the LDS-list kernel: the state is the whole contribution
-// (three components), so phase 2 reads no geometry
-template <typename S>
-struct GradListOp : GradOp<S> {
-    static constexpr int NS = 3;
-    static constexpr bool kLists = true;
-    static constexpr int kDefaultCells = 256;
-    __device__ void state(int f, int o, int n, S* s) const { GradOp<S>::face(f, o, n, s); }
-    __device__ void contrib(int, const S* s, S* v) const {
-        v[0] = s[0];
-        v[1] = s[1];
-        v[2] = s[2];
-    }
-    __device__ void bstate(int b, int c, S* s) const { GradOp<S>::bface(b, c, s); }
-    __device__ void bcontrib(int, const S* s, S* v) const {
-        v[0] = s[0];
-        v[1] = s[1];
-        v[2] = s[2];
-    }
-};
-
-// operators whose state is the contribution itself
-template <typename S, typename D>
-struct PlainOp {
-    static constexpr int NC = 1, NS = 1;
-    static constexpr bool kLists = true;  // runs k_gather
-    static constexpr int kDefaultCells = 256;
-    __device__ void state(int f, int o, int n, S* s) const {
-        static_cast<const D*>(this)->face(f, o, n, s);
-    }
-    __device__ void contrib(int, const S* s, S* v) const { v[0] = s[0]; }
-    __device__ void bstate(int b, int c, S* s) const { static_cast<const D*>(this)->bface(b, c, s); }
-    __device__ void bcontrib(int, const S* s, S* v) const { v[0] = s[0]; }
 };
 
 template <typename S, int SCHEME>  // 0 linear, 1 upwind
-struct DivOp : PlainOp<S, DivOp<S, SCHEME>> {
+struct DivOp : FaceOp<S, DivOp<S, SCHEME>, 1> {
+    // measured at 256^3: 512-cell workgroups are 3-5% faster in FP64, slower in FP32
+    static constexpr int kDefaultCells = sizeof(S) == 8 ? 512 : 256;
     const S *T, *w, *phi, *Tb;
     int F;
-    __device__ void face(int f, int o, int n, S* v) const {
-        const S ph = phi[f];
-        const S lam = SCHEME == 0 ? w[f] : (ph >= S(0) ? S(1) : S(0));
-        const S TP = T[o], TN = T[n];
-        v[0] = ph * (lam * (TP - TN) + TN);
+    struct In {
+        S ph, w, TP, TN;
+    };
+    struct BIn {
+        S ph, tb;
+    };
+    __device__ In load(int f, int o, int n) const {
+        return {phi[f], SCHEME == 0 ? w[f] : S(0), T[o], T[n]};
     }
-    __device__ void bface(int b, int, S* v) const { v[0] = phi[F + b] * Tb[b]; }
+    __device__ In eload(int k, int f, int o, int n, const ExGeo<S>& x) const {
+        return {phi[f], SCHEME == 0 ? x.w[k] : S(0), T[o], T[n]};
+    }
+    __device__ void make(const In& a, S* v) const {
+        const S lam = SCHEME == 0 ? a.w : (a.ph >= S(0) ? S(1) : S(0));
+        v[0] = a.ph * (lam * (a.TP - a.TN) + a.TN);
+    }
+    __device__ BIn bload(int b, int) const { return {phi[F + b], Tb[b]}; }
+    __device__ void bmake(const BIn& a, S* v) const { v[0] = a.ph * a.tb; }
 };
 
 template <typename S>
-struct IntegrateOp : PlainOp<S, IntegrateOp<S>> {  // surfaceIntegrate(phi): div with a unit field
+struct IntegrateOp : FaceOp<S, IntegrateOp<S>, 1> {  // surfaceIntegrate(phi): div with a unit field
     const S* phi;
     int F;
-    __device__ void face(int f, int, int, S* v) const { v[0] = phi[f]; }
-    __device__ void bface(int b, int, S* v) const { v[0] = phi[F + b]; }
+    struct In {
+        S ph;
+    };
+    using BIn = In;
+    __device__ In load(int f, int, int) const { return {phi[f]}; }
+    __device__ In eload(int, int f, int, int, const ExGeo<S>&) const { return {phi[f]}; }
+    __device__ void make(const In& a, S* v) const { v[0] = a.ph; }
+    __device__ BIn bload(int b, int) const { return {phi[F + b]}; }
+    __device__ void bmake(const BIn& a, S* v) const { v[0] = a.ph; }
 };
 
 template <typename S>
-struct LapOp : PlainOp<S, LapOp<S>> {
+struct LapOp : FaceOp<S, LapOp<S>, 1> {
     const S *T, *gamma, *magSf, *nodc, *magSfb, *dcb, *Tb, *Tg;
     const int8_t* kind;
     int F;
-    __device__ void face(int f, int o, int n, S* v) const {
-        v[0] = (gamma[f] * magSf[f]) * (nodc[f] * (T[n] - T[o]));
+    struct In {
+        S g, m, nd, TN, TP;
+    };
+    struct BIn {
+        S g, m, tg, d, tb, tc;
+        int k;
+    };
+    __device__ In load(int f, int o, int n) const {
+        return {gamma[f], magSf[f], nodc[f], T[n], T[o]};
     }
-    __device__ void bface(int b, int c, S* v) const {
-        const S sn = kind[b] == FV_BC_FIXED_GRADIENT ? (Tg ? Tg[b] : S(0)) : dcb[b] * (Tb[b] - T[c]);
-        v[0] = (gamma[F + b] * magSfb[b]) * sn;
+    __device__ In eload(int k, int f, int o, int n, const ExGeo<S>& x) const {
+        return {gamma[f], x.magSf[k], x.nodc[k], T[n], T[o]};
+    }
+    __device__ void make(const In& a, S* v) const { v[0] = (a.g * a.m) * (a.nd * (a.TN - a.TP)); }
+    __device__ BIn bload(int b, int c) const {
+        return {gamma[F + b], magSfb[b], Tg ? Tg[b] : S(0), dcb[b], Tb[b], T[c], (int)kind[b]};
+    }
+    __device__ void bmake(const BIn& a, S* v) const {
+        const S sn = a.k == FV_BC_FIXED_GRADIENT ? a.tg : a.d * (a.tb - a.tc);
+        v[0] = (a.g * a.m) * sn;
     }
 };
 
-// One cell of the per-cell path: LDS holds, for the workgroup's first nUp own
-// faces, the face state (kState, NS components) or the contribution (NC).
-template <typename S, typename Op, bool kState>
+// One cell of the per-cell path: LDS holds the contributions (NC components,
+// `stride` apart) of the workgroup's first nUp own faces; the others are
+// recomputed.
+template <typename S, typename Op>
 __device__ inline void gather_cell(int c, int fs, int nUp, const Split& t, const Op& op,
                                    S vol, S coeff, S* __restrict__ out, const S* st0, int stride) {
-    constexpr int NC = Op::NC, NS = kState ? Op::NS : Op::NC;
+    constexpr int NC = Op::NC;
     S acc[NC];
     for (int j = 0; j < NC; ++j) acc[j] = S(0);
     const int l1 = t.lowOff[c + 1];
@@ -301,12 +319,7 @@ __device__ inline void gather_cell(int c, int fs, int nUp, const Split& t, const
         S v[NC];
         const int i = f - fs;
         if (i >= 0 && i < nUp) {
-            S sv[NS];
-            for (int j = 0; j < NS; ++j) sv[j] = st0[j * stride + i];
-            if (kState)
-                op.contrib(f, sv, v);
-            else
-                for (int j = 0; j < NC; ++j) v[j] = sv[j];
+            for (int j = 0; j < NC; ++j) v[j] = st0[j * stride + i];
         } else {
             op.face(f, t.lowOwn[k], c, v);
         }
@@ -317,12 +330,7 @@ __device__ inline void gather_cell(int c, int fs, int nUp, const Split& t, const
         S v[NC];
         const int i = f - fs;
         if (i < nUp) {
-            S sv[NS];
-            for (int j = 0; j < NS; ++j) sv[j] = st0[j * stride + i];
-            if (kState)
-                op.contrib(f, sv, v);
-            else
-                for (int j = 0; j < NC; ++j) v[j] = sv[j];
+            for (int j = 0; j < NC; ++j) v[j] = st0[j * stride + i];
         } else {
             op.face(f, c, t.nbr[f], v);
         }
@@ -335,60 +343,6 @@ __device__ inline void gather_cell(int c, int fs, int nUp, const Split& t, const
         for (int j = 0; j < NC; ++j) acc[j] += v[j];
     }
     for (int j = 0; j < NC; ++j) out[(int64_t)NC * c + j] = coeff * (acc[j] / vol);
-}
-
-// Staged gather with the per-cell stencil lists read from HBM: phase 1 keeps the
-// contribution (NC components) of every own face in LDS, phase 2 runs
-// gather_cell per cell.  Lighter in LDS and in phase-1 work than k_gather, it
-// is the faster variant for the gradient (three components per face), whose
-// phase 2 would otherwise re-read Sf per face.
-template <typename S, typename Op, int CELLS, int THREADS>
-__global__ __launch_bounds__(THREADS) void k_gather_stencil(int N, int nBlocks,
-                                                            Split t, Op op,
-                                                            const S* __restrict__ V, S coeff,
-                                                            S* __restrict__ out) {
-    constexpr int NC = Op::NC;
-    constexpr int kMaxUp = max_up<CELLS>();
-    __shared__ S flux[NC][kMaxUp];
-    const int lb = (int)blockIdx.x;
-    if (lb >= nBlocks) return;
-    const int c0 = lb * CELLS, c1 = min(c0 + CELLS, N);
-    const int fs = t.ownStart[c0], fe = t.ownStart[c1];
-    const int nUp = min(fe - fs, kMaxUp);
-    // the volumes phase 2 divides by, loaded now so their latency hides behind phase 1
-    constexpr int kPer = CELLS / THREADS;
-    S vols[kPer];
-#pragma unroll
-    for (int q = 0; q < kPer; ++q) {
-        const int c = c0 + (int)threadIdx.x + q * THREADS;
-        vols[q] = c < c1 ? V[c] : S(1);
-    }
-    // kU faces per thread in flight: their loads are issued together (lanes past
-    // the range work on the last face and store nothing)
-    for (int i0 = threadIdx.x; i0 < nUp; i0 += THREADS * kU) {
-        int oo[kU], nn[kU];
-#pragma unroll
-        for (int u = 0; u < kU; ++u) {
-            const int f = fs + min(i0 + u * THREADS, nUp - 1);
-            oo[u] = t.own[f];
-            nn[u] = t.nbr[f];
-        }
-        S v[kU][NC];
-#pragma unroll
-        for (int u = 0; u < kU; ++u) op.face(fs + min(i0 + u * THREADS, nUp - 1), oo[u], nn[u], v[u]);
-#pragma unroll
-        for (int u = 0; u < kU; ++u) {
-            const int i = i0 + u * THREADS;
-            if (i < nUp)
-                for (int j = 0; j < NC; ++j) flux[j][i] = v[u][j];
-        }
-    }
-    __syncthreads();
-#pragma unroll
-    for (int q = 0; q < kPer; ++q) {
-        const int c = c0 + (int)threadIdx.x + q * THREADS;
-        if (c < c1) gather_cell<S, Op, false>(c, fs, nUp, t, op, vols[q], coeff, out, flux[0], kMaxUp);
-    }
 }
 
 struct Lists {
@@ -410,15 +364,17 @@ constexpr int kMaxLow = 8;  // lower faces per cell from inside the workgroup, i
 // gather_cell instead; both give the same bits.
 template <typename S, typename Op, int CELLS, int THREADS>
 __global__ __launch_bounds__(THREADS) void k_gather(int N, int nBlocks, Split t,
-                                                    Lists L, Op op, const S* __restrict__ V,
-                                                    S coeff, S* __restrict__ out) {
-    constexpr int NC = Op::NC, NS = Op::NS;
+                                                    Lists L, ExGeo<S> xg, Op op,
+                                                    const S* __restrict__ V, S coeff,
+                                                    S* __restrict__ out) {
+    constexpr int NC = Op::NC;
+    constexpr int kUL = Op::kFacesPerPass;
     constexpr int kMaxUp = max_up<CELLS>();
     constexpr int kMaxExt = CELLS / 2;
     constexpr int kMaxBnd = CELLS / 4;
-    __shared__ S fluxU[NS][kMaxUp];
-    __shared__ S fluxE[NS][kMaxExt];
-    __shared__ S fluxB[NS][kMaxBnd];
+    __shared__ S fluxU[NC][kMaxUp];
+    __shared__ S fluxE[NC][kMaxExt];
+    __shared__ S fluxB[NC][kMaxBnd];
     __shared__ uint16_t upS[CELLS], upE[CELLS], exS[CELLS], exE[CELLS], bS[CELLS], bE[CELLS];
     __shared__ uint16_t lowL[CELLS][kMaxLow];
     __shared__ int lowN[CELLS];
@@ -446,56 +402,92 @@ __global__ __launch_bounds__(THREADS) void k_gather(int N, int nBlocks, Split t,
     if (threadIdx.x == 0) overflow = fits ? 0 : 1;
     __syncthreads();
     const int nUp = min(nF, kMaxUp);
-    for (int i0 = threadIdx.x; i0 < nUp; i0 += THREADS * kU) {
-      // kU faces per thread in flight (see k_gather_stencil)
-      int oo[kU], nn[kU];
-      S svs[kU][NS];
+    using In = typename Op::In;
+    using BIn = typename Op::BIn;
+    // One pass over kUL own faces per thread (index i0 + u*THREADS).  The first
+    // pass also takes this thread's list records (at most one external lower
+    // face and one boundary face: the lists of a fitting workgroup are no longer
+    // than the workgroup).  Loads go out in two rounds -- indices, then the
+    // operator's inputs for all faces held -- before any arithmetic waits.
+    static_assert(kMaxExt <= THREADS && kMaxBnd <= THREADS, "one list record per thread");
+    const int ke = e0 + (int)threadIdx.x, kb = b0 + (int)threadIdx.x;
+    const bool hasE = fits && ke < e1, hasB = fits && kb < b1;
+    int ef = 0, eo = 0, ecl = 0, ecp = 0, ecn = 0, bf = 0, bcl = 0, bcp = 0, bcn = 0;
+    if (hasE) {
+        ef = L.extFace[ke];
+        eo = L.extOwn[ke];
+        ecl = L.extCell[ke];
+        ecp = L.extCell[max(ke - 1, e0)];
+        ecn = L.extCell[min(ke + 1, e1 - 1)];
+    }
+    if (hasB) {
+        bf = L.bndFace[kb];
+        bcl = L.bndCell[kb];
+        bcp = L.bndCell[max(kb - 1, b0)];
+        bcn = L.bndCell[min(kb + 1, b1 - 1)];
+    }
+    for (int i0 = threadIdx.x, pass = 0; i0 < nUp || pass == 0; i0 += THREADS * kUL, ++pass) {
+        // (a workgroup may own no face: then pass 0 only takes the list records;
+        // lanes past the range load the last face and store nothing)
+        int oo[kUL] = {}, nn[kUL] = {}, op_[kUL] = {}, on_[kUL] = {};
+        if (nUp > 0) {
 #pragma unroll
-      for (int u = 0; u < kU; ++u) {
-          const int f = fs + min(i0 + u * THREADS, nUp - 1);
-          oo[u] = t.own[f];
-          nn[u] = t.nbr[f];
-      }
-#pragma unroll
-      for (int u = 0; u < kU; ++u) op.state(fs + min(i0 + u * THREADS, nUp - 1), oo[u], nn[u], svs[u]);
-#pragma unroll
-      for (int u = 0; u < kU; ++u) {
-        const int i = i0 + u * THREADS;
-        if (i >= nUp) break;
-        const int f = fs + i;
-        const int o = oo[u], n = nn[u];
-        for (int j = 0; j < NS; ++j) fluxU[j][i] = svs[u][j];
-        if (fits) {
-            const int ol = o - c0;
-            if (i == 0 || t.own[f - 1] != o) upS[ol] = (uint16_t)i;
-            if (i == nF - 1 || t.own[f + 1] != o) upE[ol] = (uint16_t)(i + 1);
-            if (n < c1) {
-                const int nl = n - c0;
-                const int slot = atomicAdd(&lowN[nl], 1);
-                if (slot < kMaxLow)
-                    lowL[nl][slot] = (uint16_t)i;
-                else
-                    overflow = 1;
+            for (int u = 0; u < kUL; ++u) {
+                const int f = fs + min(i0 + u * THREADS, nUp - 1);
+                oo[u] = t.own[f];
+                nn[u] = t.nbr[f];
+                op_[u] = t.own[max(f - 1, fs)];
+                on_[u] = t.own[min(f + 1, fs + nUp - 1)];
             }
         }
-      }
-    }
-    if (fits) {
-        for (int k = e0 + threadIdx.x; k < e1; k += THREADS) {
-            const int cl = L.extCell[k];
-            S sv[NS];
-            op.state(L.extFace[k], L.extOwn[k], c0 + cl, sv);
-            for (int j = 0; j < NS; ++j) fluxE[j][k - e0] = sv[j];
-            if (k == e0 || L.extCell[k - 1] != cl) exS[cl] = (uint16_t)(k - e0);
-            if (k == e1 - 1 || L.extCell[k + 1] != cl) exE[cl] = (uint16_t)(k - e0 + 1);
+        In ein{};
+        BIn bin{};
+        if (pass == 0) {
+            if (hasE) ein = op.eload(ke, ef, eo, c0 + ecl, xg);
+            if (hasB) bin = op.bload(bf, c0 + bcl);
         }
-        for (int k = b0 + threadIdx.x; k < b1; k += THREADS) {
-            const int cl = L.bndCell[k];
-            S sv[NS];
-            op.bstate(L.bndFace[k], c0 + cl, sv);
-            for (int j = 0; j < NS; ++j) fluxB[j][k - b0] = sv[j];
-            if (k == b0 || L.bndCell[k - 1] != cl) bS[cl] = (uint16_t)(k - b0);
-            if (k == b1 - 1 || L.bndCell[k + 1] != cl) bE[cl] = (uint16_t)(k - b0 + 1);
+        In fin[kUL] = {};
+        if (nUp > 0) {
+#pragma unroll
+            for (int u = 0; u < kUL; ++u) fin[u] = op.load(fs + min(i0 + u * THREADS, nUp - 1), oo[u], nn[u]);
+        }
+        if (pass == 0) {
+            if (hasE) {
+                S sv[NC];
+                op.make(ein, sv);
+                for (int j = 0; j < NC; ++j) fluxE[j][ke - e0] = sv[j];
+                if (ke == e0 || ecp != ecl) exS[ecl] = (uint16_t)(ke - e0);
+                if (ke == e1 - 1 || ecn != ecl) exE[ecl] = (uint16_t)(ke - e0 + 1);
+            }
+            if (hasB) {
+                S sv[NC];
+                op.bmake(bin, sv);
+                for (int j = 0; j < NC; ++j) fluxB[j][kb - b0] = sv[j];
+                if (kb == b0 || bcp != bcl) bS[bcl] = (uint16_t)(kb - b0);
+                if (kb == b1 - 1 || bcn != bcl) bE[bcl] = (uint16_t)(kb - b0 + 1);
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < kUL; ++u) {
+            const int i = i0 + u * THREADS;
+            if (i >= nUp) break;
+            const int o = oo[u], n = nn[u];
+            S sv[NC];
+            op.make(fin[u], sv);
+            for (int j = 0; j < NC; ++j) fluxU[j][i] = sv[j];
+            if (fits) {
+                const int ol = o - c0;
+                if (i == 0 || op_[u] != o) upS[ol] = (uint16_t)i;
+                if (i == nF - 1 || on_[u] != o) upE[ol] = (uint16_t)(i + 1);
+                if (n < c1) {
+                    const int nl = n - c0;
+                    const int slot = atomicAdd(&lowN[nl], 1);
+                    if (slot < kMaxLow)
+                        lowL[nl][slot] = (uint16_t)i;
+                    else
+                        overflow = 1;
+                }
+            }
         }
     }
     __syncthreads();
@@ -503,7 +495,7 @@ __global__ __launch_bounds__(THREADS) void k_gather(int N, int nBlocks, Split t,
 #pragma unroll
         for (int q = 0; q < kPer; ++q) {
             const int c = c0 + (int)threadIdx.x + q * THREADS;
-            if (c < c1) gather_cell<S, Op, true>(c, fs, nUp, t, op, vols[q], coeff, out, fluxU[0], kMaxUp);
+            if (c < c1) gather_cell<S, Op>(c, fs, nUp, t, op, vols[q], coeff, out, fluxU[0], kMaxUp);
         }
         return;
     }
@@ -514,10 +506,9 @@ __global__ __launch_bounds__(THREADS) void k_gather(int N, int nBlocks, Split t,
         const int l = c - c0;
         S acc[NC];
         for (int j = 0; j < NC; ++j) acc[j] = S(0);
-        S sv[NS], v[NC];
+        S v[NC];
         for (int k = exS[l]; k < exE[l]; ++k) {
-            for (int j = 0; j < NS; ++j) sv[j] = fluxE[j][k];
-            op.contrib(L.extFace[e0 + k], sv, v);
+            for (int j = 0; j < NC; ++j) v[j] = fluxE[j][k];
             for (int j = 0; j < NC; ++j) acc[j] -= v[j];
         }
         // the inner lower faces arrived in any order: sort this cell's list
@@ -534,18 +525,15 @@ __global__ __launch_bounds__(THREADS) void k_gather(int N, int nBlocks, Split t,
         }
         for (int q = 0; q < n; ++q) {
             const int i = lst[q];
-            for (int j = 0; j < NS; ++j) sv[j] = fluxU[j][i];
-            op.contrib(fs + i, sv, v);
+            for (int j = 0; j < NC; ++j) v[j] = fluxU[j][i];
             for (int j = 0; j < NC; ++j) acc[j] -= v[j];
         }
         for (int i = upS[l]; i < upE[l]; ++i) {
-            for (int j = 0; j < NS; ++j) sv[j] = fluxU[j][i];
-            op.contrib(fs + i, sv, v);
+            for (int j = 0; j < NC; ++j) v[j] = fluxU[j][i];
             for (int j = 0; j < NC; ++j) acc[j] += v[j];
         }
         for (int k = bS[l]; k < bE[l]; ++k) {
-            for (int j = 0; j < NS; ++j) sv[j] = fluxB[j][k];
-            op.bcontrib(L.bndFace[b0 + k], sv, v);
+            for (int j = 0; j < NC; ++j) v[j] = fluxB[j][k];
             for (int j = 0; j < NC; ++j) acc[j] += v[j];
         }
         const S vol = vols[q];
@@ -561,25 +549,23 @@ Split split_of(const fv_mesh* m) {
 template <typename S, typename Op, int CELLS, int THREADS>
 void launch_gather_t(fv_ctx* ctx, const fv_mesh* m, const Op& op, const S* V, S coeff, S* out) {
     const int nBlocks = (m->nCells + CELLS - 1) / CELLS;
-    if (!Op::kLists) {
-        k_gather_stencil<S, Op, CELLS, THREADS><<<nBlocks, THREADS, 0, ctx->stream>>>(
-            m->nCells, nBlocks, split_of(m), op, V, coeff, out);
-        FV_LAUNCH_CHECK();
-        return;
-    }
     // the lists are built once per mesh and workgroup size (a cache on the mesh)
-    const GatherLists& gl = gather_lists(const_cast<fv_mesh*>(m), CELLS);
+    const GatherLists& gl = gather_lists(const_cast<fv_mesh*>(m), CELLS, sizeof(S) == 4);
     const Lists L{gl.extOff.p, gl.extFace.p, gl.extOwn.p, gl.bndOff.p, gl.bndFace.p,
                   gl.extCell.p, gl.bndCell.p};
+    ExGeo<S> xg;
+    if constexpr (sizeof(S) == 8)
+        xg = {gl.exW.p, gl.exMagSf.p, gl.exNodc.p, gl.exSf.p};
+    else
+        xg = {gl.exW32.p, gl.exMagSf32.p, gl.exNodc32.p, gl.exSf32.p};
     k_gather<S, Op, CELLS, THREADS><<<nBlocks, THREADS, 0, ctx->stream>>>(
-        m->nCells, nBlocks, split_of(m), L, op, V, coeff, out);
+        m->nCells, nBlocks, split_of(m), L, xg, op, V, coeff, out);
     FV_LAUNCH_CHECK();
 }
 
 template <typename S, typename Op>
 void launch_gather(fv_ctx* ctx, const fv_mesh* m, const Op& op, const S* V, S coeff, S* out) {
-    // default workgroup size per operator (measured at 256^3: the gradient's
-    // stencil-list kernel is fastest with 512 cells, the LDS-list kernel with 256)
+    // default workgroup size per operator (measured at 256^3)
     const int cells = m->gatherMode == 0 ? Op::kDefaultCells : gather_cells(m);
     switch (cells) {
         case 1024: launch_gather_t<S, Op, 1024, 512>(ctx, m, op, V, coeff, out); break;
@@ -668,13 +654,7 @@ __global__ __launch_bounds__(kBlock) void k_correct_bc(int B, const int32_t* __r
 template <typename S>
 void run_grad(fv_ctx* ctx, const fv_mesh* m, Geo<S> g, const S* T, const S* Tb, S coeff, S* out) {
     if (gather_cells(m)) {
-        // measured at 256^3: FP64 is fastest on the stencil-list kernel (512-cell
-        // workgroups), FP32 on the LDS-list kernel (256): 3.7 vs 3.6 TB/s and
-        // 3.5 vs 4.1 TB/s algorithmic
-        if (sizeof(S) == 8)
-            launch_gather<S>(ctx, m, GradOp<S>{T, g.w, g.Sf, g.Sfb, Tb}, g.V, coeff, out);
-        else
-            launch_gather<S>(ctx, m, GradListOp<S>{{T, g.w, g.Sf, g.Sfb, Tb}}, g.V, coeff, out);
+        launch_gather<S>(ctx, m, GradOp<S>{{}, T, g.w, g.Sf, g.Sfb, Tb}, g.V, coeff, out);
         return;
     }
     const int N = m->nCells;

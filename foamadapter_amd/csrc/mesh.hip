@@ -250,10 +250,47 @@ void build_stencil(fv_mesh* m, const int32_t* h_own, const int32_t* h_nbr,
     FV_HIP(hipStreamSynchronize(m->ctx->stream));
 }
 
-const GatherLists& gather_lists(fv_mesh* m, int cells) {
+template <typename S>
+__global__ void k_pack_ext_geo(int64_t n, const int32_t* __restrict__ face, const S* __restrict__ w,
+                               const S* __restrict__ magSf, const S* __restrict__ nodc,
+                               const S* __restrict__ Sf, S* __restrict__ ow, S* __restrict__ om,
+                               S* __restrict__ on, S* __restrict__ os) {
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const int64_t f = face[k];
+    ow[k] = w[f];
+    om[k] = magSf[f];
+    on[k] = nodc[f];
+    for (int j = 0; j < 3; ++j) os[3 * k + j] = Sf[3 * f + j];
+}
+
+template <typename S>
+void pack_ext_geo(fv_ctx* ctx, const DBuf<int32_t>& face, int64_t n, const S* w, const S* magSf,
+                  const S* nodc, const S* Sf, DBuf<S>& ow, DBuf<S>& om, DBuf<S>& on, DBuf<S>& os) {
+    ow.alloc(ctx, n);
+    om.alloc(ctx, n);
+    on.alloc(ctx, n);
+    os.alloc(ctx, 3 * n);
+    if (n)
+        k_pack_ext_geo<S><<<grid_for(n, 256), 256, 0, ctx->stream>>>(
+            n, face.p, w, magSf, nodc, Sf, ow.p, om.p, on.p, os.p);
+    FV_LAUNCH_CHECK();
+    FV_HIP(hipStreamSynchronize(ctx->stream));
+}
+
+const GatherLists& gather_lists(fv_mesh* m, int cells, bool f32) {
     const int slot = cells == 256 ? 0 : cells == 512 ? 1 : 2;
     GatherLists& L = m->glists[slot];
-    if (L.cells == cells) return L;
+    if (L.cells == cells) {
+        if (f32 && !L.ready32) {
+            ensure_f32(m);
+            const MeshF32& g = m->f32;
+            pack_ext_geo<float>(m->ctx, L.extFace, (int64_t)L.extFace.n, g.w.p, g.magSf.p,
+                                g.nodc.p, g.Sf.p, L.exW32, L.exMagSf32, L.exNodc32, L.exSf32);
+            L.ready32 = true;
+        }
+        return L;
+    }
     FV_REQUIRE(m->upperTri, "staged gather lists need upper-triangular face order");
     const int64_t N = m->nCells, F = m->nInt, B = m->nB;
     std::vector<int32_t> lowOff(N + 1), lowFace(F), lowOwn(F), bOff(N + 1), bFace(B);
@@ -291,7 +328,11 @@ const GatherLists& gather_lists(fv_mesh* m, int cells) {
     L.bndFace.upload(ctx, bf.data(), bf.size());
     L.bndCell.upload(ctx, bc.data(), bc.size());
     FV_HIP(hipStreamSynchronize(ctx->stream));
+    pack_ext_geo<double>(ctx, L.extFace, (int64_t)ef.size(), m->w.p, m->magSf.p, m->nodc.p,
+                         m->Sf.p, L.exW, L.exMagSf, L.exNodc, L.exSf);
+    L.ready32 = false;
     L.cells = cells;
+    if (f32) return gather_lists(m, cells, true);
     return L;
 }
 

@@ -33,6 +33,12 @@ struct GatherLists {
     int cells = 0;  // 0: not built
     fvhip::DBuf<int32_t> extOff, extFace, extOwn, bndOff, bndFace;
     fvhip::DBuf<uint16_t> extCell, bndCell;
+    // geometry of the external lower faces packed in record order (w, magSf,
+    // nonOrthDeltaCoeffs, Sf xyz), so phase 1 reads it coalesced instead of
+    // gathering it by face index; FP32 copies built on first FP32 use
+    fvhip::DBuf<double> exW, exMagSf, exNodc, exSf;
+    fvhip::DBuf<float> exW32, exMagSf32, exNodc32, exSf32;
+    bool ready32 = false;
 };
 
 struct fv_mesh {
@@ -71,8 +77,9 @@ void build_stencil(fv_mesh* m, const int32_t* h_own, const int32_t* h_nbr,
                    const int32_t* h_faceCells);
 // FP32 copies of the geometry (lazy).
 void ensure_f32(fv_mesh* m);
-// The staged gather's per-workgroup lists for `cells` cells per workgroup.
-const GatherLists& gather_lists(fv_mesh* m, int cells);
+// The staged gather's per-workgroup lists for `cells` cells per workgroup
+// (with the FP32 packed geometry too when f32).
+const GatherLists& gather_lists(fv_mesh* m, int cells, bool f32 = false);
 // Cells per workgroup of the LDS-staged gather for the cell-result operators,
 // 0 when the per-cell stencil walk runs instead (faces not in upper-triangular
 // order, or forced by fv_mesh_set_gather); kDefaultGatherCells stands for the
