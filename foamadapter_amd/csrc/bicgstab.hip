@@ -67,7 +67,7 @@ __device__ inline double sell_row(const BiSys& P, int r, const double* __restric
 
 // r = b - A x, rhat = r; partials r.r, rhat.r (= r.r) and b.b
 __global__ __launch_bounds__(kB) void kbi_start(const BiSys* __restrict__ Pp) {
-    __shared__ double sh[kB / 64];
+    __shared__ double sh[kCgShared];
     const BiSys& P = *Pp;
     int row0, row1, slot;
     if (!bi_locate(P, row0, row1, slot)) return;
@@ -90,7 +90,7 @@ __global__ __launch_bounds__(kB) void kbi_start(const BiSys* __restrict__ Pp) {
 // one block: ||b|| and the initial residual
 __global__ __launch_bounds__(kB) void kbi_start_state(const BiSys* __restrict__ Pp, double tol,
                                                       double atol, int maxit) {
-    __shared__ double sh[kB / 64];
+    __shared__ double sh[kCgShared];
     const BiSys& P = *Pp;
     double tb[1], tr[2];
     sum_slots<1>(P.pb, 0, P.ntiles, sh, tb);
@@ -113,7 +113,7 @@ __global__ __launch_bounds__(kB) void kbi_start_state(const BiSys* __restrict__ 
 }
 
 __global__ __launch_bounds__(kB) void kbi_dir(const BiSys* __restrict__ Pp) {
-    __shared__ double sh[kB / 64];
+    __shared__ double sh[kCgShared];
     const BiSys& P = *Pp;
     int row0, row1, slot;
     if (!bi_locate(P, row0, row1, slot)) return;
@@ -157,7 +157,7 @@ __global__ __launch_bounds__(kB) void kbi_dir(const BiSys* __restrict__ Pp) {
 
 // v = A y; partials rhat.v
 __global__ __launch_bounds__(kB) void kbi_spmv_v(const BiSys* __restrict__ Pp) {
-    __shared__ double sh[kB / 64];
+    __shared__ double sh[kCgShared];
     const BiSys& P = *Pp;
     int row0, row1, slot;
     if (!bi_locate(P, row0, row1, slot)) return;
@@ -176,7 +176,7 @@ __global__ __launch_bounds__(kB) void kbi_spmv_v(const BiSys* __restrict__ Pp) {
 
 // alpha = rho / rhat.v; s = r - alpha v; z = M^-1 s
 __global__ __launch_bounds__(kB) void kbi_s(const BiSys* __restrict__ Pp) {
-    __shared__ double sh[kB / 64];
+    __shared__ double sh[kCgShared];
     const BiSys& P = *Pp;
     int row0, row1, slot;
     if (!bi_locate(P, row0, row1, slot)) return;
@@ -196,7 +196,7 @@ __global__ __launch_bounds__(kB) void kbi_s(const BiSys* __restrict__ Pp) {
 
 // t = A z; partials t.s, t.t
 __global__ __launch_bounds__(kB) void kbi_spmv_t(const BiSys* __restrict__ Pp) {
-    __shared__ double sh[kB / 64];
+    __shared__ double sh[kCgShared];
     const BiSys& P = *Pp;
     int row0, row1, slot;
     if (!bi_locate(P, row0, row1, slot)) return;
@@ -216,7 +216,7 @@ __global__ __launch_bounds__(kB) void kbi_spmv_t(const BiSys* __restrict__ Pp) {
 
 // omega = t.s / t.t; x += alpha y + omega z; r = s - omega t; partials r.r, rhat.r
 __global__ __launch_bounds__(kB) void kbi_update(const BiSys* __restrict__ Pp) {
-    __shared__ double sh[kB / 64];
+    __shared__ double sh[kCgShared];
     const BiSys& P = *Pp;
     int row0, row1, slot;
     if (!bi_locate(P, row0, row1, slot)) return;

@@ -93,9 +93,12 @@ __device__ inline bool cg_locate(const CgGrid<S>* __restrict__ gp, const CgSys<S
     return true;
 }
 
+// rows per thread whose vector loads the update and direction kernels issue together
+constexpr int kCgU = 4;
+
 template <typename S>
 __global__ __launch_bounds__(kCgBlock) void kc_spmv(const CgGrid<S>* __restrict__ g, CgPartials red) {
-    __shared__ double sh[kCgBlock / 64];
+    __shared__ double sh[kCgShared];
     const CgSys<S>* Pp;
     int row0, row1, slot;
     if (!cg_locate(g, Pp, row0, row1, slot)) return;
@@ -107,19 +110,47 @@ __global__ __launch_bounds__(kCgBlock) void kc_spmv(const CgGrid<S>* __restrict_
         if (!stop) st->it += 1;
     }
     if (stop) return;
+    // One row per thread and pass, the first kSpE entries with their loads
+    // issued together (values and columns, then the gathers of p): three
+    // dependent memory round trips per row instead of two per entry (measured
+    // at 128^3: the FP64 solve 94 -> 72 ms; two rows per pass gain nothing
+    // more).  The row sums its entries in column order.
+    constexpr int kSpE = 8;
+    const int64_t* __restrict__ soff = P.sellOff;
+    const int32_t* __restrict__ scol = P.sellCol;
+    const S* __restrict__ sval = P.sval;
+    const S* __restrict__ pvec = P.p;
+    S* __restrict__ qvec = P.q;
+    const int N = P.N;
     double pq = 0.0;
     for (int r = row0 + threadIdx.x; r < row1; r += kCgBlock) {
         const int sl = r >> 6, l = r & 63;
-        const int64_t b = P.sellOff[sl];
-        const int w = (int)((P.sellOff[sl + 1] - b) >> 6);
-        S s = 0;
-        for (int k = 0; k < w; ++k) {
+        const int64_t b = soff[sl];
+        const int w = (int)((soff[sl + 1] - b) >> 6);
+        S v[kSpE], pv[kSpE];
+        int col[kSpE];
+#pragma unroll
+        for (int k = 0; k < kSpE; ++k)
+            if (k < w) {
+                const int64_t pos = b + 64 * (int64_t)k + l;
+                v[k] = sval[pos];
+                col[k] = scol[pos];
+            }
+        const S own = r < N ? pvec[r] : S(0);
+#pragma unroll
+        for (int k = 0; k < kSpE; ++k)
+            if (k < w) pv[k] = pvec[col[k]];
+        S sum = 0;
+#pragma unroll
+        for (int k = 0; k < kSpE; ++k)
+            if (k < w) sum += v[k] * pv[k];
+        for (int k = kSpE; k < w; ++k) {  // rows wider than kSpE entries
             const int64_t pos = b + 64 * (int64_t)k + l;
-            s += P.sval[pos] * P.p[P.sellCol[pos]];
+            sum += sval[pos] * pvec[scol[pos]];
         }
-        if (r < P.N) {
-            P.q[r] = s;
-            pq += (double)P.p[r] * (double)s;
+        if (r < N) {
+            qvec[r] = sum;
+            pq += (double)own * (double)sum;
         }
     }
     double v[1] = {pq};
@@ -128,13 +159,32 @@ __global__ __launch_bounds__(kCgBlock) void kc_spmv(const CgGrid<S>* __restrict_
 
 template <typename S>
 __global__ __launch_bounds__(kCgBlock) void kc_update(const CgGrid<S>* __restrict__ g, CgPartials red) {
-    __shared__ double sh[kCgBlock / 64];
+    __shared__ double sh[kCgShared];
     const CgSys<S>* Pp;
     int row0, row1, slot;
     if (!cg_locate(g, Pp, row0, row1, slot)) return;
     const CgSys<S>& P = *Pp;
     CgState* st = P.st;
     if (st->done) return;
+    // rows are taken kCgU per thread at a time, their loads issued together;
+    // the first group's loads go out before the reduction of p.q
+    const int r1 = min(row1, P.N);
+    const int i0 = row0 + (int)threadIdx.x;
+    S dv[kCgU], pv[kCgU], rv[kCgU], qv[kCgU], iv[kCgU];
+    auto load = [&](int base) {
+#pragma unroll
+        for (int u = 0; u < kCgU; ++u) {
+            const int i = base + u * kCgBlock;
+            if (i < r1) {
+                dv[u] = P.d[i];
+                pv[u] = P.p[i];
+                rv[u] = P.r[i];
+                qv[u] = P.q[i];
+                iv[u] = P.dinv[i];
+            }
+        }
+    };
+    load(i0);
     double tot[1];
     sum_slots<1>(red.pq, P.tile0, P.ntiles, sh, tot);
     const int it = st->it;
@@ -142,15 +192,21 @@ __global__ __launch_bounds__(kCgBlock) void kc_update(const CgGrid<S>* __restric
     if (row0 == 0 && threadIdx.x == 0) st->alpha = alpha;
     const S a = (S)alpha;
     double rr = 0.0, rz = 0.0;
-    const int r1 = min(row1, P.N);
-    for (int i = row0 + threadIdx.x; i < r1; i += kCgBlock) {
-        P.d[i] += a * P.p[i];
-        const S ri = P.r[i] - a * P.q[i];
-        P.r[i] = ri;
-        const S zi = P.dinv[i] * ri;
-        P.z[i] = zi;
-        rr += (double)ri * (double)ri;
-        rz += (double)ri * (double)zi;
+    for (int base = i0; base < r1; base += kCgU * kCgBlock) {
+        if (base != i0) load(base);
+#pragma unroll
+        for (int u = 0; u < kCgU; ++u) {
+            const int i = base + u * kCgBlock;
+            if (i < r1) {
+                P.d[i] = dv[u] + a * pv[u];
+                const S ri = rv[u] - a * qv[u];
+                P.r[i] = ri;
+                const S zi = iv[u] * ri;
+                P.z[i] = zi;
+                rr += (double)ri * (double)ri;
+                rz += (double)ri * (double)zi;
+            }
+        }
     }
     double v[2] = {rr, rz};
     write_slot<2>(v, red.rz, slot, sh);
@@ -163,7 +219,7 @@ __global__ __launch_bounds__(kCgBlock) void kc_update(const CgGrid<S>* __restric
 // MODE 2: after the start (replacement of the initial guess; p = z already).
 template <typename S, int MODE>
 __global__ __launch_bounds__(kCgBlock) void kc_direction(const CgGrid<S>* __restrict__ g, CgPartials red) {
-    __shared__ double sh[kCgBlock / 64];
+    __shared__ double sh[kCgShared];
     const CgSys<S>* Pp;
     int row0, row1, slot;
     if (!cg_locate(g, Pp, row0, row1, slot)) return;
@@ -171,6 +227,21 @@ __global__ __launch_bounds__(kCgBlock) void kc_direction(const CgGrid<S>* __rest
     CgState* st = P.st;
     const int done = st->done;
     if ((MODE == 0 && done) || (MODE == 1 && done != 1)) return;
+    // the first kCgU rows' z and p are loaded before the reduction
+    const int r1 = min(row1, P.N);
+    const int i0 = row0 + (int)threadIdx.x;
+    S zv[kCgU], pv[kCgU];
+    auto load = [&](int base) {
+#pragma unroll
+        for (int u = 0; u < kCgU; ++u) {
+            const int i = base + u * kCgBlock;
+            if (i < r1) {
+                zv[u] = P.z[i];
+                pv[u] = P.p[i];
+            }
+        }
+    };
+    if (MODE != 2) load(i0);
     double tot[2];
     sum_slots<2>(red.rz, P.tile0, P.ntiles, sh, tot);
     const int it = st->it;
@@ -197,8 +268,14 @@ __global__ __launch_bounds__(kCgBlock) void kc_direction(const CgGrid<S>* __rest
     }
     if (MODE == 2 || stop != 0 || it == 0) return;
     const S beta = (S)(tot[1] / st->rz[(it - 1) & 1]);
-    const int r1 = min(row1, P.N);
-    for (int i = row0 + threadIdx.x; i < r1; i += kCgBlock) P.p[i] = P.z[i] + beta * P.p[i];
+    for (int base = i0; base < r1; base += kCgU * kCgBlock) {
+        if (base != i0) load(base);
+#pragma unroll
+        for (int u = 0; u < kCgU; ++u) {
+            const int i = base + u * kCgBlock;
+            if (i < r1) P.p[i] = zv[u] + beta * pv[u];
+        }
+    }
 }
 
 template <typename S>
@@ -218,7 +295,7 @@ __global__ __launch_bounds__(kCgBlock) void kc_accum(const CgGrid<S>* __restrict
 // r64 = b - A x in FP64 (CSR rows in column order), partials of r.r and b.b
 template <typename S, int START>
 __global__ __launch_bounds__(kCgBlock) void kc_resid(const CgGrid<S>* __restrict__ g, CgPartials red) {
-    __shared__ double sh[kCgBlock / 64];
+    __shared__ double sh[kCgShared];
     const CgSys<S>* Pp;
     int row0, row1, slot;
     if (!cg_locate(g, Pp, row0, row1, slot)) return;
@@ -243,7 +320,7 @@ __global__ __launch_bounds__(kCgBlock) void kc_resid(const CgGrid<S>* __restrict
 // START: p = z and ||b||; partials of r.r and r.z
 template <typename S, int START>
 __global__ __launch_bounds__(kCgBlock) void kc_replace(const CgGrid<S>* __restrict__ g, CgPartials red) {
-    __shared__ double sh[kCgBlock / 64];
+    __shared__ double sh[kCgShared];
     const CgSys<S>* Pp;
     int row0, row1, slot;
     if (!cg_locate(g, Pp, row0, row1, slot)) return;
@@ -288,7 +365,7 @@ __global__ __launch_bounds__(kCgBlock) void kc_replace(const CgGrid<S>* __restri
 template <typename S>
 __global__ __launch_bounds__(kCgBlock) void kc_resid_norm(const CgSys<S>* __restrict__ P,
                                                           CgPartials red, CgState* out) {
-    __shared__ double sh[kCgBlock / 64];
+    __shared__ double sh[kCgShared];
     double tot[2];
     sum_slots<2>(red.rb, P->tile0, P->ntiles, sh, tot);
     if (threadIdx.x == 0) {

@@ -28,18 +28,33 @@ struct CgState {
     double tol, atol;  // relative and absolute (0: off) residual targets
 };
 
+// Fixed-shape block reduction of NV values at once: wavefront shuffles, then
+// thread 0 adds the wavefronts' sums in order.  sh holds NV * (blockDim/64)
+// doubles.  The totals are valid in thread 0.
+template <int NV>
+__device__ inline void block_sum_n(double (&v)[NV], double* sh) {
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = (int)(blockDim.x >> 6);
+    for (int j = 0; j < NV; ++j)
+        for (int o = 32; o > 0; o >>= 1) v[j] += __shfl_down(v[j], o, 64);
+    if (lane == 0)
+        for (int j = 0; j < NV; ++j) sh[j * nw + wid] = v[j];
+    __syncthreads();
+    if (threadIdx.x == 0)
+        for (int j = 0; j < NV; ++j) {
+            double r = 0.0;
+            for (int w = 0; w < nw; ++w) r += sh[j * nw + w];
+            v[j] = r;
+        }
+    __syncthreads();
+}
+
+// LDS the reductions of a kCgBlock-thread kernel need (up to 2 values each)
+constexpr int kCgShared = 2 * (kCgBlock / 64);
+
 __device__ inline double block_sum(double v, double* sh) {
-    // fixed-shape tree: wavefront shuffle then across the block's wavefronts
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
-    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-    if (lane == 0) sh[wid] = v;
-    __syncthreads();
-    double r = 0.0;
-    if (threadIdx.x == 0) {
-        for (int w = 0; w < (int)(blockDim.x >> 6); ++w) r += sh[w];
-    }
-    __syncthreads();
-    return r;  // valid in thread 0
+    double a[1] = {v};
+    block_sum_n<1>(a, sh);
+    return a[0];  // valid in thread 0
 }
 
 // The block's NV partial sums go to slot `slot` of `partials` (NV doubles per
@@ -47,7 +62,8 @@ __device__ inline double block_sum(double v, double* sh) {
 template <int NV>
 __device__ inline void write_slot(const double (&v)[NV], double* partials, int slot, double* sh) {
     double s[NV];
-    for (int j = 0; j < NV; ++j) s[j] = block_sum(v[j], sh);
+    for (int j = 0; j < NV; ++j) s[j] = v[j];
+    block_sum_n<NV>(s, sh);
     if (threadIdx.x == 0)
         for (int j = 0; j < NV; ++j) partials[(int64_t)slot * NV + j] = s[j];
 }
@@ -55,18 +71,19 @@ __device__ inline void write_slot(const double (&v)[NV], double* partials, int s
 // Every block of the consuming kernel sums the producer's slots [base,
 // base+count) itself, in one fixed order (strided per thread, then the
 // fixed-shape block tree), so all blocks obtain bitwise the same totals and the
-// result depends only on the system's own slot count.
+// result depends only on the system's own slot count.  The NV values of a slot
+// are loaded together.
 template <int NV>
 __device__ inline void sum_slots(const double* partials, int base, int count, double* sh,
                                  double (&tot)[NV]) {
     __shared__ double bc[NV];
-    for (int j = 0; j < NV; ++j) {
-        double a = 0.0;
-        for (int b = threadIdx.x; b < count; b += blockDim.x)
-            a += partials[(int64_t)(base + b) * NV + j];
-        a = block_sum(a, sh);
-        if (threadIdx.x == 0) bc[j] = a;
-    }
+    double a[NV];
+    for (int j = 0; j < NV; ++j) a[j] = 0.0;
+    for (int b = threadIdx.x; b < count; b += blockDim.x)
+        for (int j = 0; j < NV; ++j) a[j] += partials[(int64_t)(base + b) * NV + j];
+    block_sum_n<NV>(a, sh);
+    if (threadIdx.x == 0)
+        for (int j = 0; j < NV; ++j) bc[j] = a[j];
     __syncthreads();
     for (int j = 0; j < NV; ++j) tot[j] = bc[j];
     __syncthreads();

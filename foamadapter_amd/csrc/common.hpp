@@ -86,6 +86,11 @@ struct fv_ctx {
     static constexpr int kEvents = 64;
     hipEvent_t events[kEvents] = {};
     uint64_t gen = 0;  // unique per context of the process
+    // pinned host area and events for polls that do not stall the stream
+    // (solver state copied back while the next iterations are queued)
+    static constexpr size_t kPollBytes = 4096;
+    void* pollHost = nullptr;
+    hipEvent_t pollEv[2] = {};
 };
 
 namespace fvhip {
@@ -96,6 +101,8 @@ void ctx_free(fv_ctx* ctx, void* p);
 // Generation of a live context (0 if it was destroyed); a buffer released after
 // its context is gone was freed with the context and is dropped.
 uint64_t ctx_generation(fv_ctx* ctx);
+// The context's pinned poll area (kPollBytes) and its two events, made on first use.
+void* ctx_poll_host(fv_ctx* ctx);
 
 // RAII device buffer owned by a library handle.
 template <typename T>

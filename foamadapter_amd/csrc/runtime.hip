@@ -17,6 +17,15 @@ uint64_t ctx_generation(fv_ctx* ctx) {
     return it == g_live.end() ? 0 : it->second;
 }
 
+void* ctx_poll_host(fv_ctx* ctx) {
+    if (!ctx->pollHost) {
+        FV_HIP(hipSetDevice(ctx->device));
+        FV_HIP(hipHostMalloc(&ctx->pollHost, fv_ctx::kPollBytes, hipHostMallocDefault));
+        for (auto& e : ctx->pollEv) FV_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    }
+    return ctx->pollHost;
+}
+
 void set_error(const std::string& msg) { g_last_error = msg; }
 void clear_error() { g_last_error.clear(); }
 
@@ -182,6 +191,9 @@ int fv_ctx_destroy(fv_ctx* ctx) {
         ctx->cached = 0;
         for (auto& e : ctx->events)
             if (e) (void)hipEventDestroy(e);
+        for (auto& e : ctx->pollEv)
+            if (e) (void)hipEventDestroy(e);
+        if (ctx->pollHost) (void)hipHostFree(ctx->pollHost);
         (void)hipStreamDestroy(ctx->stream);
         delete ctx;
     });

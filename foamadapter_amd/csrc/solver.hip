@@ -31,6 +31,7 @@ struct CgWork : fv_solver_cache {
     DBuf<CgGrid<S>> gdev;  // device copy of grid (the kernels' launch description)
     CgPartials red{};
     int G = 0;  // launch grid (multiple of 8)
+    int lastIters = 0;  // iterations of the previous solve (time steps repeat them)
 
     CgWork(fv_ctx* c, const fv_linsys* ls, bool separateCorrection) {
         const fv_sparsity* sp = ls->sp;
@@ -121,15 +122,26 @@ struct CgWork : fv_solver_cache {
         FV_HIP(hipStreamSynchronize(s));
         return o;
     }
-    // explicit FP64 relative residual ||b - A x|| / ||b|| of the bound x
-    double true_residual(hipStream_t s) {
+    // explicit FP64 relative residual ||b - A x|| / ||b|| of the bound x into st2
+    void enqueue_true_residual(hipStream_t s) {
         kc_resid<S, 1><<<G, kB, 0, s>>>(gdev.p, red);
         kc_resid_norm<S><<<1, kB, 0, s>>>(sys.p, red, st2.p);
         FV_LAUNCH_CHECK();
+    }
+    double true_residual(hipStream_t s) {
+        enqueue_true_residual(s);
         CgState o{};
         FV_HIP(hipMemcpyAsync(&o, st2.p, sizeof o, hipMemcpyDeviceToHost, s));
         FV_HIP(hipStreamSynchronize(s));
         return o.res;
+    }
+    // the state and (when enqueued) the explicit residual, one synchronisation
+    CgState state2(hipStream_t s, CgState* o2) {
+        CgState o{};
+        FV_HIP(hipMemcpyAsync(&o, st.p, sizeof o, hipMemcpyDeviceToHost, s));
+        FV_HIP(hipMemcpyAsync(o2, st2.p, sizeof *o2, hipMemcpyDeviceToHost, s));
+        FV_HIP(hipStreamSynchronize(s));
+        return o;
     }
 };
 
@@ -140,26 +152,65 @@ CgWork<S>& work_of(fv_linsys* ls, bool mixed) {
     return *static_cast<CgWork<S>*>(slot.get());
 }
 
-// Plain FP64 CG from x; stops on the recurrence residual estimate.
-CgState run_cg(fv_ctx* ctx, CgWork<double>& w, double tol, int maxit, int check, double atol) {
+// Plain FP64 CG from x; stops on the recurrence residual estimate.  *true_res
+// receives the explicit FP64 residual of the result.
+//
+// Iterations follow the start without a round trip (a system converged at the
+// start turns them into no-ops).  Polls do not stall the stream: each chunk's
+// state is copied to pinned memory behind an event and read only after the
+// next chunk is queued, so the device never waits for the host; a solve ends
+// at most one chunk of no-op launches after the state reports a stop.  Where
+// the previous solve of this system (a time step before) says the solve should
+// be done, the poll blocks instead and brings the explicit residual with it,
+// so a solve that repeats its iteration count costs one synchronisation.
+CgState run_cg(fv_ctx* ctx, CgWork<double>& w, double tol, int maxit, int check, double atol,
+               double* true_res) {
     hipStream_t s = ctx->stream;
+    CgState* mirror = static_cast<CgState*>(ctx_poll_host(ctx));
+    static_assert(2 * sizeof(CgState) <= fv_ctx::kPollBytes, "poll area too small");
     w.start(s, tol, maxit, 0.0, atol);
     FV_LAUNCH_CHECK();
-    // iterations follow the start without a round trip (a system converged at
-    // the start turns them into no-ops); poll early for short solves, then
-    // every `check` iterations
-    CgState h{};
-    int launched = 0, batch = std::min(4, check);
+    CgState h{}, t{};
+    int launched = 0, batch = std::min(w.lastIters > 0 ? w.lastIters : 4, std::max(check, 1));
+    int slot = 0, pending = -1;  // mirror slot of the poll in flight
+    bool stopped = false, haveTrue = false;
     while (launched < maxit) {
         const int n = std::min(batch, maxit - launched);
-        batch = std::min(2 * batch, check);
+        batch = std::min(std::max(2 * batch, 4), check);
         for (int i = 0; i < n; ++i) w.iteration(s, true);
         FV_LAUNCH_CHECK();
         launched += n;
-        h = w.state(s);
-        if (h.stop) break;
+        if ((w.lastIters > 0 && launched >= w.lastIters) || launched >= maxit) {
+            const bool withTrue = w.lastIters > 0 && launched >= w.lastIters;
+            if (withTrue) w.enqueue_true_residual(s);
+            h = w.state2(s, &t);
+            haveTrue = withTrue;
+            pending = -1;
+            if (h.stop) {
+                stopped = true;
+                break;
+            }
+            continue;
+        }
+        FV_HIP(hipMemcpyAsync(&mirror[slot], w.st.p, sizeof(CgState), hipMemcpyDeviceToHost, s));
+        FV_HIP(hipEventRecord(ctx->pollEv[slot], s));
+        if (pending >= 0) {
+            FV_HIP(hipEventSynchronize(ctx->pollEv[pending]));
+            if (mirror[pending].stop) {  // the chunk queued since ran as no-ops
+                FV_HIP(hipEventSynchronize(ctx->pollEv[slot]));
+                h = mirror[slot];
+                haveTrue = false;
+                stopped = true;
+                break;
+            }
+        }
+        pending = slot;
+        slot ^= 1;
     }
-    if (launched == 0) h = w.state(s);
+    if (!stopped) h = w.state(s);
+    // a system that needed no iteration polls (blocking) after the first
+    w.lastIters = stopped ? std::max(h.it, 1) : h.it;
+    *true_res = haveTrue ? t.res : w.true_residual(s);
     return h;
 }
 
@@ -221,13 +272,14 @@ int fv_solve_cg(fv_linsys* ls, void* d_x, const fv_solve_opts* o, fv_solver_stat
         if (o->mode == FV_SOLVE_F64) {
             CgWork<double>& w = work_of<double>(ls, false);
             w.bind(ctx, ls, x);
-            CgState h = run_cg(ctx, w, o->rel_tol, o->max_iter, check, o->abs_tol);
+            double tr = 0.0;
+            CgState h = run_cg(ctx, w, o->rel_tol, o->max_iter, check, o->abs_tol, &tr);
             local.init_res = h.res0;
             local.final_res = h.res;
             bnorm = h.bnorm;
             local.iters = h.it;
             local.outer_iters = 1;
-            local.true_res = w.true_residual(ctx->stream);
+            local.true_res = tr;
         } else if (o->mode == FV_SOLVE_F32_IR) {
             CgWork<float>& w = work_of<float>(ls, true);
             w.bind(ctx, ls, x);
