@@ -59,11 +59,11 @@ KERNEL_OF = {
     ("grad", "f64"): "k_gather<double, GradOp<double>, 256, 256>",
     ("grad", "f32"): "k_gather<float, GradOp<float>, 256, 256>",
     ("div_linear", "f64"): "k_gather<double, DivOp<double, 0>, 512, 512>",
-    ("div_linear", "f32"): "k_gather<float, DivOp<float, 0>, 256, 256>",
+    ("div_linear", "f32"): "k_gather<float, DivOp<float, 0>, 1024, 512>",
     ("div_upwind", "f64"): "k_gather<double, DivOp<double, 1>, 512, 512>",
-    ("div_upwind", "f32"): "k_gather<float, DivOp<float, 1>, 256, 256>",
+    ("div_upwind", "f32"): "k_gather<float, DivOp<float, 1>, 1024, 512>",
     ("laplacian", "f64"): "k_gather<double, LapOp<double>, 256, 256>",
-    ("laplacian", "f32"): "k_gather<float, LapOp<float>, 256, 256>",
+    ("laplacian", "f32"): "k_gather<float, LapOp<float>, 1024, 512>",
 }
 TRAFFIC_PROFILE = ROOT / "profiles" / "faceops_sweep_256cube.md"
 
