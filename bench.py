@@ -317,8 +317,9 @@ def bench_batched(ctx, P=512, seed=1004, steps=5):
     """512 independent heat problems on cubic meshes n ~ logU[12, 96], kappa ~
     logU[1e-4, 1e-1], dt = 1e-3, T0 ~ U[300,400], walls ~ U[300,400], 5 Euler
     steps, to 1e-8.  Plain: one problem at a time, FP64 fv_solve_cg per step.
-    New: all problems in one batched launch sequence per step (FP32 CG with FP64
-    residual replacement, hipGraph chunks, autotuned tile shape)."""
+    New: per step, the 512 assemblies in one launch (fv_assemble_ddt_laplacian_many)
+    and all solves in one batched launch sequence (FP32 CG with FP64 residual
+    replacement, hipGraph chunks, autotuned tile shape)."""
     import foamadapter_amd as fa
     from foamadapter_amd import _lib, la
     from foamadapter_amd.la import solver_opts
@@ -369,15 +370,13 @@ def bench_batched(ctx, P=512, seed=1004, steps=5):
         ctx.sync()
         return iters
 
-    def run_batched(batch):
+    def run_batched(batch, heat):
         for p in probs:
             _lib.call("fv_memcpy_d2d", ctx.handle, p["x"].ptr, p["T0"].ptr, p["x"].nbytes)
             _lib.call("fv_memcpy_d2d", ctx.handle, p["xo"].ptr, p["T0"].ptr, p["x"].nbytes)
         iters = 0
         for s in range(steps):
-            for p in probs:
-                _lib.call("fv_assemble_ddt_laplacian", p["ls"].handle, p["xo"].ptr, 1e-3,
-                          p["kap"].ptr, p["bc"].handle, p["Tb"].ptr, None, -1.0)
+            heat.assemble(-1.0)                           # all 512 systems, one launch
             stats = batch.solve(solver, use_graph=True)   # x_old <- x after the solve
             iters += sum(st.numIter for st in stats)
         ctx.sync()
@@ -408,9 +407,11 @@ def bench_batched(ctx, P=512, seed=1004, steps=5):
     shape = batch.autotune(solver, str(ROOT / "foamadapter_amd" / "batch_tuning.json"))
     out["autotune"] = {"tile_rows": shape[0], "chunk": shape[1],
                        "seconds": round(time.perf_counter() - t, 2)}
-    run_batched(batch)                                    # warm (graph capture)
+    heat = la.HeatAssembly([(p["ls"], p["xo"], 1e-3, p["kap"], p["bc"], p["Tb"], None)
+                            for p in probs])
+    run_batched(batch, heat)                              # warm (graph capture)
     t = time.perf_counter()
-    it_b = run_batched(batch)
+    it_b = run_batched(batch, heat)
     wall_b = time.perf_counter() - t
     err = max(float(np.max(np.abs(p["x"].numpy() - xp)) / np.max(np.abs(xp)))
               for p, xp in zip(probs[:8], x_plain))

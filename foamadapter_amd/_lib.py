@@ -137,6 +137,7 @@ _SIGS = {
     "fv_assemble_laplacian": [_vp, _vp, _vp, _vp, _vp, _d],
     "fv_assemble_div": [_vp, C.c_int, _vp, _vp, _vp, _vp, _d],
     "fv_assemble_ddt_laplacian": [_vp, _vp, _d, _vp, _vp, _vp, _vp, _d],
+    "fv_assemble_ddt_laplacian_many": [C.c_int, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _d],
     "fv_set_reference": [_vp, C.c_int, _d],
     "fv_residual": [_vp, _vp, _vp],
     "fv_compute_rau": [_vp, _vp, _vp],

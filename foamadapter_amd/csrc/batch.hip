@@ -344,10 +344,20 @@ void batch_step(fv_batch* b, std::vector<CgState>& states) {
         step_t<double>(b, states);
 }
 
-void batch_copy_old(fv_batch* b, BMember* m) {
-    if (!m->xold) return;
-    FV_HIP(hipMemcpyAsync(m->xold, m->x, m->N * sizeof(double), hipMemcpyDeviceToDevice,
-                          b->ctx->stream));
+template <typename S>
+void copy_old_t(fv_batch* b) {
+    const int G = cg_grid(b->tileCap);
+    kc_copy_old<S><<<G, kB, 0, b->ctx->stream>>>((const CgGrid<S>*)b->gridDev.p);
+    FV_LAUNCH_CHECK();
+}
+
+// x_old = x of every member that has a copy target, in one launch over the tiles
+void batch_copy_old(fv_batch* b) {
+    if (b->members.empty()) return;
+    if (b->prec == FV_SOLVE_F32_IR)
+        copy_old_t<float>(b);
+    else
+        copy_old_t<double>(b);
 }
 
 }  // namespace fvhip
@@ -581,6 +591,7 @@ int fv_batch_solve(fv_batch* b, const fv_solve_opts* o, int use_graph, fv_solver
                 if (st.stop < 2 && st.it < st.maxit) running = true;
             if (!running) break;
         }
+        batch_copy_old(b);
         int bad = 0;
         for (size_t i = 0; i < b->members.size(); ++i) {
             BMember* m = b->members[i];
@@ -589,7 +600,6 @@ int fv_batch_solve(fv_batch* b, const fv_solve_opts* o, int use_graph, fv_solver
             m->stats.true_res = hs[i].res;
             m->stats.iters = hs[i].it;
             m->stats.outer_iters = chunks;
-            batch_copy_old(b, m);
             if (stats) stats[i] = m->stats;
             if (!(hs[i].res <= o->rel_tol)) ++bad;
         }

@@ -63,6 +63,6 @@ void batch_restart(fv_batch* b, BMember* m);
 void batch_step(fv_batch* b, std::vector<CgState>& states);
 // Launch-only part of a step (no host synchronisation), for callers that
 // overlap the next step's host work.
-void batch_copy_old(fv_batch* b, BMember* m);
+void batch_copy_old(fv_batch* b);  // x_old = x of every member with a copy target
 
 }  // namespace fvhip

@@ -292,6 +292,20 @@ __global__ __launch_bounds__(kCgBlock) void kc_accum(const CgGrid<S>* __restrict
     }
 }
 
+// x_old = x for the systems that have a copy target (after a batched solve)
+template <typename S>
+__global__ __launch_bounds__(kCgBlock) void kc_copy_old(const CgGrid<S>* __restrict__ g) {
+    const CgSys<S>* Pp;
+    int row0, row1, slot;
+    if (!cg_locate(g, Pp, row0, row1, slot)) return;
+    const CgSys<S>& P = *Pp;
+    double* xo = P.xold;
+    if (!xo) return;
+    const double* x = P.x;
+    const int r1 = min(row1, P.N);
+    for (int i = row0 + threadIdx.x; i < r1; i += kCgBlock) xo[i] = x[i];
+}
+
 // r64 = b - A x in FP64 (CSR rows in column order), partials of r.r and b.b
 template <typename S, int START>
 __global__ __launch_bounds__(kCgBlock) void kc_resid(const CgGrid<S>* __restrict__ g, CgPartials red) {
@@ -301,16 +315,38 @@ __global__ __launch_bounds__(kCgBlock) void kc_resid(const CgGrid<S>* __restrict
     if (!cg_locate(g, Pp, row0, row1, slot)) return;
     const CgSys<S>& P = *Pp;
     if (!START && P.st->stop != 1) return;
+    // a row's first kRE entries have their loads issued together (values and
+    // columns, then the gathers of x), as in the SpMV; same summation order
+    constexpr int kRE = 8;
+    const int32_t* __restrict__ ro = P.rowOffs;
+    const int32_t* __restrict__ ci = P.colIdxs;
+    const double* __restrict__ va = P.vals;
+    const double* __restrict__ xv = P.x;
     double rr = 0.0, bb = 0.0;
     const int r1 = min(row1, P.N);
     for (int i = row0 + threadIdx.x; i < r1; i += kCgBlock) {
+        const int k0 = ro[i], k1 = ro[i + 1];
+        const double bi = P.rhs[i];
+        double v[kRE], xg[kRE];
+        int c[kRE];
+#pragma unroll
+        for (int k = 0; k < kRE; ++k)
+            if (k0 + k < k1) {
+                v[k] = va[k0 + k];
+                c[k] = ci[k0 + k];
+            }
+#pragma unroll
+        for (int k = 0; k < kRE; ++k)
+            if (k0 + k < k1) xg[k] = xv[c[k]];
         double s = 0.0;
-        const int e = P.rowOffs[i + 1];
-        for (int k = P.rowOffs[i]; k < e; ++k) s += P.vals[k] * P.x[P.colIdxs[k]];
-        const double ri = P.rhs[i] - s;
+#pragma unroll
+        for (int k = 0; k < kRE; ++k)
+            if (k0 + k < k1) s += v[k] * xg[k];
+        for (int k = k0 + kRE; k < k1; ++k) s += va[k] * xv[ci[k]];
+        const double ri = bi - s;
         P.r64[i] = ri;
         rr += ri * ri;
-        bb += P.rhs[i] * P.rhs[i];
+        bb += bi * bi;
     }
     double v[2] = {rr, bb};
     write_slot<2>(v, red.rb, slot, sh);

@@ -96,51 +96,122 @@ __global__ __launch_bounds__(kBlock) void k_laplacian(
     rhs[c] += coeff * bsum;
 }
 
-// Fused Euler ddt + coeff * laplacian, overwriting values and rhs: the same
-// arithmetic as zero -> ddt -> laplacian in that order (0 + x is exact), in one
-// pass over the cell's stencil.
-__global__ __launch_bounds__(kBlock) void k_ddt_laplacian(
-    int N, int F, Stencil st, const int32_t* __restrict__ rowOffs,
-    const int32_t* __restrict__ diag, const double* __restrict__ V,
-    const double* __restrict__ Told, double rdt, const double* __restrict__ magSf,
-    const double* __restrict__ nodc, const double* __restrict__ magSfb,
-    const double* __restrict__ dcb, const int8_t* __restrict__ kind,
-    const double* __restrict__ gamma, const double* __restrict__ Tb,
-    const double* __restrict__ Tg, double coeff, double* __restrict__ values,
-    double* __restrict__ rhs, double* __restrict__ bMat, double* __restrict__ bRhs) {
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= N) return;
-    const int row = rowOffs[c];
-    double dsum = 0.0, bsum = 0.0;
-    const int e1 = st.off[c + 1];
-    for (int k = st.off[c]; k < e1; ++k) {
-        const int e = st.idx[k];
-        if (e >= F) {
-            const int b = e - F;
-            const double gm = gamma[e] * magSfb[b];
-            if (kind[b] == FV_BC_FIXED_GRADIENT) {
-                const double g = gm * (Tg ? Tg[b] : 0.0);
-                bsum -= g;
-                bMat[b] = 0.0;
-                bRhs[b] = coeff * g;
+// Inputs of the fused heat assembly of one system (the kernel arguments of
+// fv_assemble_ddt_laplacian, a table entry of its batched form).
+struct HeatDesc {
+    int N, F;
+    Stencil st;
+    const int32_t *rowOffs, *diag;
+    const double *V, *Told;
+    double rdt;
+    const double *magSf, *nodc, *magSfb, *dcb;
+    const int8_t* kind;
+    const double *gamma, *Tb, *Tg;
+    double coeff;
+    double *values, *rhs, *bMat, *bRhs;
+};
+
+// Fused Euler ddt + coeff * laplacian of cell c, overwriting its row and rhs:
+// the same arithmetic as zero -> ddt -> laplacian in that order (0 + x is
+// exact), in one pass over the cell's stencil.
+__device__ inline void ddt_laplacian_cell(const HeatDesc& D, int c) {
+    // The first kSE stencil entries have their loads issued in two rounds
+    // (entries, then the face data of all of them) before any arithmetic; the
+    // arithmetic and its order are those of the loop over the stencil.
+    constexpr int kSE = 8;
+    const int row = D.rowOffs[c], dg = D.diag[c];
+    const double v = D.V[c], told = D.Told[c];
+    const int k0 = D.st.off[c], k1 = D.st.off[c + 1];
+    int ev[kSE], sl[kSE];
+#pragma unroll
+    for (int j = 0; j < kSE; ++j)
+        if (k0 + j < k1) {
+            ev[j] = D.st.idx[k0 + j];
+            sl[j] = D.st.slot[k0 + j];
+        }
+    double ga[kSE], ma[kSE], da[kSE], tb[kSE], tg[kSE];
+    int kd[kSE];
+#pragma unroll
+    for (int j = 0; j < kSE; ++j)
+        if (k0 + j < k1) {
+            const int e = ev[j];
+            if (e >= D.F) {
+                const int b = e - D.F;
+                ga[j] = D.gamma[e];
+                ma[j] = D.magSfb[b];
+                kd[j] = D.kind[b];
+                da[j] = D.dcb[b];
+                tb[j] = D.Tb[b];
+                tg[j] = D.Tg ? D.Tg[b] : 0.0;
             } else {
-                const double ic = gm * dcb[b];
-                const double it = ic * Tb[b];
+                const int f = e >= 0 ? e : ~e;
+                ga[j] = D.gamma[f];
+                ma[j] = D.magSf[f];
+                da[j] = D.nodc[f];
+            }
+        }
+    double dsum = 0.0, bsum = 0.0;
+    auto entry = [&](int e, int slot, double g0, double m0, double d0, int k, double tb0,
+                     double tg0) {
+        if (e >= D.F) {
+            const int b = e - D.F;
+            const double gm = g0 * m0;
+            if (k == FV_BC_FIXED_GRADIENT) {
+                const double g = gm * tg0;
+                bsum -= g;
+                D.bMat[b] = 0.0;
+                D.bRhs[b] = D.coeff * g;
+            } else {
+                const double ic = gm * d0;
+                const double it = ic * tb0;
                 dsum -= ic;
                 bsum -= it;
-                bMat[b] = coeff * ic;
-                bRhs[b] = coeff * it;
+                D.bMat[b] = D.coeff * ic;
+                D.bRhs[b] = D.coeff * it;
             }
         } else {
-            const int f = e >= 0 ? e : ~e;
-            const double up = nodc[f] * (gamma[f] * magSf[f]);
-            values[row + st.slot[k]] = coeff * up;
+            const double up = d0 * (g0 * m0);
+            D.values[row + slot] = D.coeff * up;
             dsum -= up;
         }
+    };
+#pragma unroll
+    for (int j = 0; j < kSE; ++j)
+        if (k0 + j < k1) {
+            const bool bnd = ev[j] >= D.F;
+            entry(ev[j], sl[j], ga[j], ma[j], da[j], bnd ? kd[j] : 0, bnd ? tb[j] : 0.0,
+                  bnd ? tg[j] : 0.0);
+        }
+    for (int k = k0 + kSE; k < k1; ++k) {  // cells with more than kSE faces
+        const int e = D.st.idx[k];
+        if (e >= D.F) {
+            const int b = e - D.F;
+            entry(e, 0, D.gamma[e], D.magSfb[b], D.dcb[b], D.kind[b], D.Tb[b], D.Tg ? D.Tg[b] : 0.0);
+        } else {
+            const int f = e >= 0 ? e : ~e;
+            entry(e, D.st.slot[k], D.gamma[f], D.magSf[f], D.nodc[f], 0, 0.0, 0.0);
+        }
     }
-    const double v = V[c];
-    values[row + diag[c]] = (rdt * v) + coeff * dsum;
-    rhs[c] = ((rdt * Told[c]) * v) + coeff * bsum;
+    D.values[row + dg] = (D.rdt * v) + D.coeff * dsum;
+    D.rhs[c] = ((D.rdt * told) * v) + D.coeff * bsum;
+}
+
+__global__ __launch_bounds__(kBlock) void k_ddt_laplacian(HeatDesc D) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c < D.N) ddt_laplacian_cell(D, c);
+}
+
+// Batched form: block b assembles cells [c0, c0 + kBlock) of system sys.
+struct HeatBlock {
+    int32_t sys, c0;
+};
+
+__global__ __launch_bounds__(kBlock) void k_ddt_laplacian_many(const HeatDesc* __restrict__ desc,
+                                                               const HeatBlock* __restrict__ blocks) {
+    const HeatBlock hb = blocks[blockIdx.x];
+    const HeatDesc D = desc[hb.sys];
+    const int c = hb.c0 + (int)threadIdx.x;
+    if (c < D.N) ddt_laplacian_cell(D, c);
 }
 
 __global__ __launch_bounds__(kBlock) void k_div(
@@ -219,6 +290,19 @@ void check_ls(const fv_linsys* ls) { FV_REQUIRE(ls && ls->sp && ls->sp->mesh, "N
 
 void check_bc(const fv_linsys* ls, const fv_bc* bc) {
     FV_REQUIRE(bc && bc->mesh == mesh_of(ls), "boundary conditions belong to another mesh");
+}
+
+HeatDesc heat_desc(fv_linsys* ls, const double* Told, double dt, const double* gamma,
+                   const fv_bc* bc, const double* Tb, const double* Tg, double coeff) {
+    check_ls(ls);
+    check_bc(ls, bc);
+    fv_mesh* m = mesh_of(ls);
+    FV_REQUIRE(Told && gamma && (Tb || m->nB == 0) && dt > 0.0, "bad arguments");
+    fv_sparsity* sp = ls->sp;
+    return HeatDesc{m->nCells, m->nInt, Stencil{m->cfOff.p, m->cfIdx.p, sp->entrySlot.p},
+                    sp->rowOffs.p, sp->diagOffset.p, m->V.p, Told, 1.0 / dt, m->magSf.p,
+                    m->nodc.p, m->magSfb.p, m->dcb.p, bc->faceKind.p, gamma, Tb, Tg, coeff,
+                    ls->values.p, ls->rhs.p, ls->bMat.p, ls->bRhs.p};
 }
 
 }  // namespace
@@ -446,18 +530,41 @@ int fv_assemble_laplacian(fv_linsys* ls, const void* gamma, const fv_bc* bc, con
 int fv_assemble_ddt_laplacian(fv_linsys* ls, const void* Told, double dt, const void* gamma,
                               const fv_bc* bc, const void* Tb, const void* Tg, double coeff) {
     return guarded([&] {
-        check_ls(ls);
-        check_bc(ls, bc);
-        fv_mesh* m = mesh_of(ls);
-        FV_REQUIRE(Told && gamma && (Tb || m->nB == 0) && dt > 0.0, "bad arguments");
-        fv_sparsity* sp = ls->sp;
-        Stencil st{m->cfOff.p, m->cfIdx.p, sp->entrySlot.p};
-        k_ddt_laplacian<<<grid_for(m->nCells, kBlock), kBlock, 0, m->ctx->stream>>>(
-            m->nCells, m->nInt, st, sp->rowOffs.p, sp->diagOffset.p, m->V.p, (const double*)Told,
-            1.0 / dt, m->magSf.p, m->nodc.p, m->magSfb.p, m->dcb.p, bc->faceKind.p,
-            (const double*)gamma, (const double*)Tb, (const double*)Tg, coeff, ls->values.p,
-            ls->rhs.p, ls->bMat.p, ls->bRhs.p);
+        const HeatDesc d = heat_desc(ls, (const double*)Told, dt, (const double*)gamma, bc,
+                                     (const double*)Tb, (const double*)Tg, coeff);
+        k_ddt_laplacian<<<grid_for(d.N, kBlock), kBlock, 0, mesh_of(ls)->ctx->stream>>>(d);
         FV_LAUNCH_CHECK();
+    });
+}
+
+int fv_assemble_ddt_laplacian_many(int n, fv_linsys* const* ls, const void* const* Told,
+                                   const double* dt, const void* const* gamma,
+                                   const fv_bc* const* bc, const void* const* Tb,
+                                   const void* const* Tg, double coeff) {
+    return guarded([&] {
+        FV_REQUIRE(n >= 0 && (n == 0 || (ls && Told && dt && gamma && bc && Tb)), "NULL argument");
+        if (n == 0) return;
+        std::vector<HeatDesc> desc(n);
+        std::vector<HeatBlock> blocks;
+        fv_ctx* ctx = nullptr;
+        for (int i = 0; i < n; ++i) {
+            desc[i] = heat_desc(ls[i], (const double*)Told[i], dt[i], (const double*)gamma[i],
+                                bc[i], (const double*)Tb[i], Tg ? (const double*)Tg[i] : nullptr,
+                                coeff);
+            fv_ctx* c = mesh_of(ls[i])->ctx;
+            FV_REQUIRE(!ctx || c == ctx, "systems of one call must share a context");
+            ctx = c;
+            for (int c0 = 0; c0 < desc[i].N; c0 += kBlock) blocks.push_back(HeatBlock{i, c0});
+        }
+        if (blocks.empty()) return;
+        DBuf<HeatDesc> dd;
+        DBuf<HeatBlock> db;
+        dd.upload(ctx, desc.data(), desc.size());
+        db.upload(ctx, blocks.data(), blocks.size());
+        k_ddt_laplacian_many<<<(unsigned)blocks.size(), kBlock, 0, ctx->stream>>>(dd.p, db.p);
+        FV_LAUNCH_CHECK();
+        // the tables' blocks return to the context's pool: later work on the
+        // same stream is ordered after this launch
     });
 }
 

@@ -284,6 +284,37 @@ class Batch:
             pass
 
 
+class HeatAssembly:
+    """ddt(T) + coeff*laplacian(gamma, T) (Euler implicit) of many systems in one
+    launch (fv_assemble_ddt_laplacian_many), each bit-identical to its own
+    fv_assemble_ddt_laplacian.  items: (LinearSystem, T_old, dt, gamma, bc, T_b,
+    T_grad or None) per system; the argument tables are built once, so a time
+    step costs one call."""
+
+    def __init__(self, items):
+        n = len(items)
+        self.n = n
+        self._keep = list(items)        # the device arrays stay alive with the tables
+        vp = lambda xs: (C.c_void_p * max(n, 1))(*[C.c_void_p(_ptr(x)) for x in xs])  # noqa: E731
+        self.ls = vp([it[0].handle for it in items])
+        self.Told = vp([it[1].ptr for it in items])
+        self.dt = (C.c_double * max(n, 1))(*[float(it[2]) for it in items])
+        self.gamma = vp([it[3].ptr for it in items])
+        self.bc = vp([it[4].handle for it in items])
+        self.Tb = vp([it[5].ptr for it in items])
+        self.Tg = vp([it[6].ptr if it[6] is not None else None for it in items])
+
+    def assemble(self, coeff: float = -1.0):
+        call("fv_assemble_ddt_laplacian_many", self.n, self.ls, self.Told, self.dt, self.gamma,
+             self.bc, self.Tb, self.Tg, float(coeff))
+
+
+def _ptr(x):
+    if x is None:
+        return None
+    return x.value if isinstance(x, C.c_void_p) else int(x)
+
+
 def solve_cg(ls: LinearSystem, x: DeviceArray, solver: dict | None = None,
              raise_on_noconverge: bool = True) -> SolverStats:
     """Solve with the method the dictionary names (Jacobi-PCG by default, Jacobi-

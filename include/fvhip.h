@@ -308,6 +308,15 @@ int fv_assemble_laplacian(fv_linsys* ls, const void* d_gamma, const fv_bc* bc, c
 int fv_assemble_ddt_laplacian(fv_linsys* ls, const void* d_Told, double dt, const void* d_gamma,
                               const fv_bc* bc, const void* d_Tb, const void* d_Tgrad,
                               double coeff);
+/* fv_assemble_ddt_laplacian of n systems (any meshes of one context) in one
+ * launch, each bit-identical to its own call: entry i of every array belongs to
+ * system i (Tgrad may be NULL, or hold NULL entries).  No counterpart in the
+ * reference, which assembles one system per call; for the batched and served
+ * heat steps, whose per-system launches would otherwise dominate small solves. */
+int fv_assemble_ddt_laplacian_many(int n, fv_linsys* const* ls, const void* const* d_Told,
+                                   const double* dt, const void* const* d_gamma,
+                                   const fv_bc* const* bc, const void* const* d_Tb,
+                                   const void* const* d_Tgrad, double coeff);
 /* DivOperator(Implicit, phi, T, "Gauss <scheme>").implicitOperation(ls)
  * (test_implicitOperators.cpp:188-195): lower = -w phi, upper = lower + phi. */
 int fv_assemble_div(fv_linsys* ls, fv_scheme scheme, const void* d_phi, const fv_bc* bc,

@@ -118,6 +118,62 @@ def test_fused_heat_assembly_bitwise(ctx, systems):
     assert np.array_equal(a.values(), b.values()) and np.array_equal(a.rhs(), b.rhs())
 
 
+def test_heat_assembly_many_bitwise(ctx, systems):
+    """One fv_assemble_ddt_laplacian_many launch over meshes of several sizes ==
+    each system's own fv_assemble_ddt_laplacian, bit for bit (matrix, rhs and
+    boundary coefficients), with and without a gradient field."""
+    import ctypes as C
+    from foamadapter_amd import _lib, la
+
+    def bcoeffs(ls, B):
+        pm, pr = C.c_void_p(), C.c_void_p()
+        _lib.call("fv_linsys_boundary_coeffs", ls.handle, C.byref(pm), C.byref(pr))
+        out = []
+        for ptr in (pm, pr):
+            h = np.empty(B)
+            _lib.call("fv_memcpy_d2h", ctx.handle, C.c_void_p(h.ctypes.data), ptr, h.nbytes)
+            out.append(h)
+        return out
+
+    items, singles = [], []
+    for i, s in enumerate(systems):
+        m, T = s["mesh"], s["T"]
+        F, B = m.n_internal_faces, m.n_boundary_faces
+        gam = ctx.upload(orc.uniform(F + B, 1e-3, 1e-1, 300 + i, 0))
+        tg = ctx.upload(orc.uniform(B, -1.0, 1.0, 300 + i, 1)) if i % 2 else None
+        dt = 1e-3 * (1 + i)
+        sp = la.SparsityPattern.readOrCreate(m)
+        many, one = la.LinearSystem(sp), la.LinearSystem(sp)
+        items.append((many, T.internal, dt, gam, T.bcs, T.boundary, tg))
+        _lib.call("fv_assemble_ddt_laplacian", one.handle, T.internal.ptr, dt, gam.ptr,
+                  T.bcs.handle, T.boundary.ptr, tg.ptr if tg is not None else None, -1.0)
+        singles.append((one, B))
+    la.HeatAssembly(items).assemble(-1.0)
+    for (many, *_), (one, B) in zip(items, singles):
+        assert np.array_equal(many.values(), one.values())
+        assert np.array_equal(many.rhs(), one.rhs())
+        for a, b in zip(bcoeffs(many, B), bcoeffs(one, B)):
+            assert np.array_equal(a, b)
+
+
+def test_batch_copies_old_solution(ctx, systems):
+    """fv_batch_solve leaves x_old = x for members added with a copy target."""
+    from foamadapter_amd import la
+    b = la.Batch(ctx, "mixed", 256, 8)
+    xs, xos = [], []
+    for i in (0, 3, 5):
+        x = ctx.upload(systems[i]["x0"])
+        xo = ctx.zeros((len(systems[i]["x0"]),))
+        b.add(systems[i]["ls"], x, xo if i != 3 else None)
+        xs.append(x)
+        xos.append(xo)
+    b.solve({"relTol": 1e-8, "maxIter": 2000})
+    assert np.array_equal(xos[0].numpy(), xs[0].numpy())
+    assert not np.any(xos[1].numpy())          # no copy target: untouched
+    assert np.array_equal(xos[2].numpy(), xs[2].numpy())
+    b.close()
+
+
 def test_autotune_picks_and_caches(ctx, systems, tmp_path):
     from foamadapter_amd import la
     b = la.Batch(ctx, "mixed", 1024, 16)
