@@ -123,7 +123,8 @@ void build_tables(fv_batch* b) {
         p.p = v + 3 * (int64_t)N;
         p.q = v + 4 * (int64_t)N;
         p.dinv = v + 5 * (int64_t)N;
-        p.r64 = (double*)(m->vec.p + 6 * (int64_t)N * sizeof(S));
+        p.p2 = v + 6 * (int64_t)N;
+        p.r64 = (double*)(m->vec.p + 7 * (int64_t)N * sizeof(S));
         p.st = b->states.p + m->slot;
     }
     // Launch table: logical blocks are numbered XCD-contiguously (block i runs on
@@ -183,16 +184,32 @@ void enqueue_chunk(fv_batch* b) {
     const auto* g = (const CgGrid<S>*)b->gridDev.p;
     const CgPartials red = partials_of<S>(b);
     const int G = cg_grid(b->tileCap);
-    for (int i = 0; i < b->chunk; ++i) {
+    // the first iteration with a plain SpMV, the others with the previous
+    // direction fused into the SpMV, then the last direction (or, for large
+    // FP64 batches, three kernels per iteration)
+    const int n = b->chunk;
+    int64_t rows = 0;
+    for (const BMember* m : b->members) rows += m->N;
+    if (!cg_fuse_direction<S>(rows)) {
+        for (int j = 0; j < n; ++j) {
+            kc_spmv<S><<<G, kB, 0, s>>>(g, red);
+            kc_update<S><<<G, kB, 0, s>>>(g, red, 0);
+            kc_direction<S, 0><<<G, kB, 0, s>>>(g, red, 0);
+        }
+    } else {
         kc_spmv<S><<<G, kB, 0, s>>>(g, red);
-        kc_update<S><<<G, kB, 0, s>>>(g, red);
-        kc_direction<S, 0><<<G, kB, 0, s>>>(g, red);
+        kc_update<S><<<G, kB, 0, s>>>(g, red, 0);
+        for (int j = 1; j < n; ++j) {
+            kc_dirspmv<S><<<G, kB, 0, s>>>(g, red, (j - 1) & 1, j & 1);
+            kc_update<S><<<G, kB, 0, s>>>(g, red, j & 1);
+        }
+        kc_direction<S, 0><<<G, kB, 0, s>>>(g, red, (n - 1) & 1);
     }
     // FP64 residual replacement of the systems that stopped in this chunk
     kc_accum<S><<<G, kB, 0, s>>>(g);
     kc_resid<S, 0><<<G, kB, 0, s>>>(g, red);
     kc_replace<S, 0><<<G, kB, 0, s>>>(g, red);
-    kc_direction<S, 1><<<G, kB, 0, s>>>(g, red);
+    kc_direction<S, 1><<<G, kB, 0, s>>>(g, red, 0);
 }
 
 template <typename S>
@@ -255,7 +272,7 @@ void start_members(fv_batch* b) {
     kc_start_tile<S><<<G, kB, 0, s>>>(g, (const CgStartParam*)b->startParams.p);
     kc_resid<S, 1><<<G, kB, 0, s>>>(g, red);
     kc_replace<S, 1><<<G, kB, 0, s>>>(g, red);
-    kc_direction<S, 2><<<G, kB, 0, s>>>(g, red);
+    kc_direction<S, 2><<<G, kB, 0, s>>>(g, red, 0);
     FV_LAUNCH_CHECK();
     // the upload buffer must outlive the launches
     FV_HIP(hipStreamSynchronize(s));
@@ -310,7 +327,7 @@ BMember* batch_add(fv_batch* b, fv_linsys* ls, double* x, double* xold, void* us
         m->N = ls->sp->nRows;
         m->user = user;
         const size_t e = elem(b->prec);
-        m->vec.alloc(b->ctx, 6 * (size_t)m->N * e + (size_t)m->N * sizeof(double));
+        m->vec.alloc(b->ctx, 7 * (size_t)m->N * e + (size_t)m->N * sizeof(double));
         m->sval.alloc(b->ctx, (size_t)ls->sp->sellTotal * e);
         m->slot = grow_slots(b);
     } catch (...) {

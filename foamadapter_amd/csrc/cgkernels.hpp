@@ -8,12 +8,23 @@
 // blocks are numbered XCD-contiguously (hardware block i runs on XCD i % 8), so
 // neighbouring row tiles -- and the SpMV gathers between them -- share an L2.
 //
-// One iteration is three kernels:
-//   spmv       q = A p (sliced ELL), partials of p.q
+// A chunk of n iterations is 2n + 1 kernels:
+//   spmv       q = A p (sliced ELL), partials of p.q          (first iteration)
 //   update     alpha from the p.q partials; d += alpha p, r -= alpha q,
 //              z = D^-1 r, partials of r.r and r.z
-//   direction  beta and the stop decision from the r.r/r.z partials; p = z + beta p
-// and the FP64 residual replacement of the mixed solver is
+//   dirspmv    beta and the stop decision from the r.r/r.z partials; the new
+//              direction p' = z + beta p of its own rows (stored) and of the
+//              SpMV's columns (recomputed from z and p, the same expression),
+//              q = A p', partials of p'.q                    (iterations 2..n)
+//   update
+//   direction  decision and p = z + beta p after the last update
+// The search direction alternates between two buffers inside a chunk (dirspmv
+// reads one and writes the other, since its SpMV reads the old direction of
+// neighbouring rows); the chunk starts and ends with p in the first buffer.
+// The results are bitwise those of a separate direction kernel per iteration,
+// which large FP64 systems use instead (spmv, update, direction per iteration;
+// see cg_fuse_direction).
+// The FP64 residual replacement of the mixed solver is
 //   accum      x += d, d = 0
 //   resid      r64 = b - A x (FP64 CSR), partials of r.r and b.b
 //   replace    relative residual from those partials; r = r64, z = D^-1 r,
@@ -46,6 +57,7 @@ struct CgSys {
     double* x;
     double* xold;                   // optional copy target after the solve (may be null)
     S *d, *r, *z, *p, *q, *dinv;    // d: correction (the solution itself in plain FP64)
+    S* p2;                          // second search-direction buffer
     double* r64;
     CgState* st;
 };
@@ -97,19 +109,27 @@ __device__ inline bool cg_locate(const CgGrid<S>* __restrict__ gp, const CgSys<S
 constexpr int kCgU = 4;
 
 template <typename S>
+__device__ inline S* cg_pbuf(const CgSys<S>& P, int b) {
+    return b ? P.p2 : P.p;
+}
+
+// MODE-0 stop decision from ||r|| (rn) after `it` iterations
+__device__ inline int cg_decide(const CgState* st, double rn, int it) {
+    const double est = rn / st->bnorm;
+    return (est <= st->tol || rn <= st->atol || it >= st->maxit || !(est == est) ||
+            (st->delta > 0.0 && rn < st->delta * st->rrep))
+               ? 1
+               : 0;
+}
+
+template <typename S>
 __global__ __launch_bounds__(kCgBlock) void kc_spmv(const CgGrid<S>* __restrict__ g, CgPartials red) {
     __shared__ double sh[kCgShared];
     const CgSys<S>* Pp;
     int row0, row1, slot;
     if (!cg_locate(g, Pp, row0, row1, slot)) return;
     const CgSys<S>& P = *Pp;
-    CgState* st = P.st;
-    const int stop = st->stop;
-    if (row0 == 0 && threadIdx.x == 0) {
-        st->done = stop;
-        if (!stop) st->it += 1;
-    }
-    if (stop) return;
+    if (P.st->stop) return;
     // One row per thread and pass, the first kSpE entries with their loads
     // issued together (values and columns, then the gathers of p): three
     // dependent memory round trips per row instead of two per entry (measured
@@ -157,8 +177,13 @@ __global__ __launch_bounds__(kCgBlock) void kc_spmv(const CgGrid<S>* __restrict_
     write_slot<1>(v, red.pq, slot, sh);
 }
 
+// Direction of the previous iteration fused with this iteration's SpMV.  pOld
+// and pNew select the search-direction buffers (uniform over the launch).  A
+// thread's first row is loaded (values, columns, then z and the old p of the
+// columns) before the reduction, which hides the reduction's latency.
 template <typename S>
-__global__ __launch_bounds__(kCgBlock) void kc_update(const CgGrid<S>* __restrict__ g, CgPartials red) {
+__global__ __launch_bounds__(kCgBlock) void kc_dirspmv(const CgGrid<S>* __restrict__ g,
+                                                       CgPartials red, int pOld, int pNew) {
     __shared__ double sh[kCgShared];
     const CgSys<S>* Pp;
     int row0, row1, slot;
@@ -166,6 +191,95 @@ __global__ __launch_bounds__(kCgBlock) void kc_update(const CgGrid<S>* __restric
     const CgSys<S>& P = *Pp;
     CgState* st = P.st;
     if (st->done) return;
+    constexpr int kSpE = 8;
+    const int64_t* __restrict__ soff = P.sellOff;
+    const int32_t* __restrict__ scol = P.sellCol;
+    const S* __restrict__ sval = P.sval;
+    const S* __restrict__ zv = P.z;
+    const S* __restrict__ po = cg_pbuf(P, pOld);
+    S* __restrict__ pn = cg_pbuf(P, pNew);
+    S* __restrict__ qvec = P.q;
+    const int N = P.N;
+    S v[kSpE], zc[kSpE], pc[kSpE], zr = S(0), pr = S(0);
+    int w = 0;
+    int64_t b = 0;
+    auto load = [&](int r) {
+        const int sl = r >> 6, l = r & 63;
+        b = soff[sl] + l;
+        w = (int)((soff[sl + 1] - soff[sl]) >> 6);
+        int col[kSpE];
+#pragma unroll
+        for (int k = 0; k < kSpE; ++k)
+            if (k < w) {
+                v[k] = sval[b + 64 * (int64_t)k];
+                col[k] = scol[b + 64 * (int64_t)k];
+            }
+        if (r < N) {
+            zr = zv[r];
+            pr = po[r];
+        }
+#pragma unroll
+        for (int k = 0; k < kSpE; ++k)
+            if (k < w) {
+                zc[k] = zv[col[k]];
+                pc[k] = po[col[k]];
+            }
+    };
+    const int r0 = row0 + (int)threadIdx.x;
+    if (r0 < row1) load(r0);
+    double tot[2];
+    sum_slots<2>(red.rz, P.tile0, P.ntiles, sh, tot);
+    const double rn = sqrt(tot[0]);
+    const int stop = cg_decide(st, rn, st->it);
+    if (row0 == 0 && threadIdx.x == 0) {
+        st->rho = tot[1];
+        st->res = rn / st->bnorm;
+        st->stop = stop;
+    }
+    if (stop) return;
+    const S beta = (S)(tot[1] / st->rhoPrev);
+    double pq = 0.0;
+    for (int r = r0; r < row1; r += kCgBlock) {
+        if (r != r0) load(r);
+        S sum = 0;
+#pragma unroll
+        for (int k = 0; k < kSpE; ++k)
+            if (k < w) sum += v[k] * (zc[k] + beta * pc[k]);
+        for (int k = kSpE; k < w; ++k) {  // rows wider than kSpE entries
+            const int c = scol[b + 64 * (int64_t)k];
+            sum += sval[b + 64 * (int64_t)k] * (zv[c] + beta * po[c]);
+        }
+        if (r < N) {
+            const S own = zr + beta * pr;
+            pn[r] = own;
+            qvec[r] = sum;
+            pq += (double)own * (double)sum;
+        }
+    }
+    double vv[1] = {pq};
+    write_slot<1>(vv, red.pq, slot, sh);
+}
+
+template <typename S>
+__global__ __launch_bounds__(kCgBlock) void kc_update(const CgGrid<S>* __restrict__ g, CgPartials red,
+                                                      int pb) {
+    __shared__ double sh[kCgShared];
+    const CgSys<S>* Pp;
+    int row0, row1, slot;
+    if (!cg_locate(g, Pp, row0, row1, slot)) return;
+    const CgSys<S>& P = *Pp;
+    CgState* st = P.st;
+    const int stop = st->stop;
+    if (row0 == 0 && threadIdx.x == 0) {
+        st->done = stop;
+        if (!stop) {
+            st->it += 1;
+            st->rhoPrev = st->rho;
+            st->pbuf = pb;
+        }
+    }
+    if (stop) return;
+    const S* __restrict__ pvec = cg_pbuf(P, pb);
     // rows are taken kCgU per thread at a time, their loads issued together;
     // the first group's loads go out before the reduction of p.q
     const int r1 = min(row1, P.N);
@@ -177,7 +291,7 @@ __global__ __launch_bounds__(kCgBlock) void kc_update(const CgGrid<S>* __restric
             const int i = base + u * kCgBlock;
             if (i < r1) {
                 dv[u] = P.d[i];
-                pv[u] = P.p[i];
+                pv[u] = pvec[i];
                 rv[u] = P.r[i];
                 qv[u] = P.q[i];
                 iv[u] = P.dinv[i];
@@ -187,8 +301,7 @@ __global__ __launch_bounds__(kCgBlock) void kc_update(const CgGrid<S>* __restric
     load(i0);
     double tot[1];
     sum_slots<1>(red.pq, P.tile0, P.ntiles, sh, tot);
-    const int it = st->it;
-    const double alpha = st->rz[(it - 1) & 1] / tot[0];
+    const double alpha = st->rho / tot[0];
     if (row0 == 0 && threadIdx.x == 0) st->alpha = alpha;
     const S a = (S)alpha;
     double rr = 0.0, rz = 0.0;
@@ -218,7 +331,8 @@ __global__ __launch_bounds__(kCgBlock) void kc_update(const CgGrid<S>* __restric
 //         the FP64 residual: 2 converged, 3 non-finite, 1 iteration limit);
 // MODE 2: after the start (replacement of the initial guess; p = z already).
 template <typename S, int MODE>
-__global__ __launch_bounds__(kCgBlock) void kc_direction(const CgGrid<S>* __restrict__ g, CgPartials red) {
+__global__ __launch_bounds__(kCgBlock) void kc_direction(const CgGrid<S>* __restrict__ g, CgPartials red,
+                                                         int pOld) {
     __shared__ double sh[kCgShared];
     const CgSys<S>* Pp;
     int row0, row1, slot;
@@ -227,6 +341,9 @@ __global__ __launch_bounds__(kCgBlock) void kc_direction(const CgGrid<S>* __rest
     CgState* st = P.st;
     const int done = st->done;
     if ((MODE == 0 && done) || (MODE == 1 && done != 1)) return;
+    // the old direction: the chunk's last buffer (mode 0) or that of the
+    // system's last update (mode 1); the new one goes to the first buffer
+    const S* __restrict__ po = cg_pbuf(P, MODE == 1 ? st->pbuf : pOld);
     // the first kCgU rows' z and p are loaded before the reduction
     const int r1 = min(row1, P.N);
     const int i0 = row0 + (int)threadIdx.x;
@@ -237,7 +354,7 @@ __global__ __launch_bounds__(kCgBlock) void kc_direction(const CgGrid<S>* __rest
             const int i = base + u * kCgBlock;
             if (i < r1) {
                 zv[u] = P.z[i];
-                pv[u] = P.p[i];
+                pv[u] = po[i];
             }
         }
     };
@@ -248,18 +365,14 @@ __global__ __launch_bounds__(kCgBlock) void kc_direction(const CgGrid<S>* __rest
     const double rn = sqrt(tot[0]);
     int stop;
     if (MODE == 0) {
-        const double est = rn / st->bnorm;
-        stop = (est <= st->tol || rn <= st->atol || it >= st->maxit || !(est == est) ||
-                (st->delta > 0.0 && rn < st->delta * st->rrep))
-                   ? 1
-                   : 0;
+        stop = cg_decide(st, rn, it);
     } else {
         const double res = st->res;
         const bool conv = res <= st->tol || res * st->bnorm <= st->atol;
         stop = conv ? 2 : !(res <= 1.7976931348623157e308) ? 3 : it >= st->maxit ? 1 : 0;
     }
     if (row0 == 0 && threadIdx.x == 0) {
-        st->rz[it & 1] = tot[1];
+        st->rho = tot[1];
         if (MODE == 0)
             st->res = rn / st->bnorm;
         else
@@ -267,13 +380,14 @@ __global__ __launch_bounds__(kCgBlock) void kc_direction(const CgGrid<S>* __rest
         st->stop = stop;
     }
     if (MODE == 2 || stop != 0 || it == 0) return;
-    const S beta = (S)(tot[1] / st->rz[(it - 1) & 1]);
+    const S beta = (S)(tot[1] / st->rhoPrev);
+    S* pn = P.p;
     for (int base = i0; base < r1; base += kCgU * kCgBlock) {
         if (base != i0) load(base);
 #pragma unroll
         for (int u = 0; u < kCgU; ++u) {
             const int i = base + u * kCgBlock;
-            if (i < r1) P.p[i] = zv[u] + beta * pv[u];
+            if (i < r1) pn[i] = zv[u] + beta * pv[u];
         }
     }
 }
@@ -480,6 +594,17 @@ __global__ void kc_dinv(int N, const int32_t* __restrict__ rowOffs, const int32_
                         const double* __restrict__ values, S* __restrict__ dinv) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < N) dinv[i] = (S)(1.0 / values[rowOffs[i] + diag[i]]);
+}
+
+// Whether a chunk fuses each direction into the next SpMV.  Measured on
+// MI355X: in FP32 the fused kernel is faster at every size (128^3 Poisson
+// 57 -> 55 us per iteration; 32^3 heat step 0.78 -> 0.60 ms); in FP64 its
+// second gather per entry (z and the old p) costs more than the launch it
+// saves once the system is large (128^3: 72.6 -> 74.6 ms per solve), while
+// small FP64 solves still gain (512-problem per-problem heat: +7 %).
+template <typename S>
+inline bool cg_fuse_direction(int64_t rows) {
+    return sizeof(S) == 4 || rows < (int64_t(1) << 20);
 }
 
 // Grid for `tiles` logical blocks: a multiple of 8 (one share per XCD)

@@ -12,20 +12,26 @@ constexpr int kCgBlock = 256;
 // kernel (the thread 0 of the system's first tile) and is never read by the
 // kernel that writes it, so no fence or atomic is needed: the kernel boundary
 // orders a write before every read of it.
-//   stop  decision of the latest direction/restart kernel: 0 running, 1 stopped
-//         (recurrence estimate at the tolerance, iteration limit, NaN, or the
-//         residual-drop trigger of the mixed solver), 2 converged in FP64,
-//         3 failed (non-finite FP64 residual)
-//   done  copy of stop taken by the SpMV kernel at the start of an iteration
-//         (the rest of the iteration reads it) and by the replacement kernel
-//         (the direction kernel after it reads it)
-//   it    completed iterations; the SpMV kernel of iteration k sets it = k + 1
-//   rz    rz[it & 1] holds r_it.z_it, so beta = rz[it&1] / rz[(it-1)&1]; a
-//         residual replacement overwrites rz[it & 1] with the replaced value
+//   stop     decision of the latest direction kernel: 0 running, 1 stopped
+//            (recurrence estimate at the tolerance, iteration limit, NaN, or the
+//            residual-drop trigger of the mixed solver), 2 converged in FP64,
+//            3 failed (non-finite FP64 residual); read by SpMV, update and the
+//            replacement kernels
+//   done     copy of stop taken by the update kernel (and by the replacement
+//            kernel, for the direction kernel after it); read by the direction
+//            kernels
+//   it       completed iterations (the update kernel advances it)
+//   rho      r.z of the latest residual, written by the direction kernels; the
+//            update divides it by p.q for alpha
+//   rhoPrev  rho as of the latest update (written by the update kernel); the
+//            direction kernels divide the new r.z by it for beta
+//   pbuf     which of the two search-direction buffers holds the p of the
+//            latest update (written by the update kernel)
 struct CgState {
-    double rz[2], alpha, bnorm, res, res0, rrep, delta;
+    double rho, rhoPrev, alpha, bnorm, res, res0, rrep, delta;
     int32_t it, done, maxit, stop;
     double tol, atol;  // relative and absolute (0: off) residual targets
+    int32_t pbuf, pad;
 };
 
 // Fixed-shape block reduction of NV values at once: wavefront shuffles, then

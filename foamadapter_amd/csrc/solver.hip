@@ -22,7 +22,7 @@ constexpr int kB = kCgBlock;
 // Workspace of one CG in value type S on one linear system.
 template <typename S>
 struct CgWork : fv_solver_cache {
-    DBuf<S> dbuf, r, z, p, q, dinv, sval;
+    DBuf<S> dbuf, r, z, p, p2, q, dinv, sval;
     DBuf<double> r64, pq, rz, rb;
     DBuf<CgState> st, st2;
     DBuf<CgSys<S>> sys;
@@ -42,6 +42,7 @@ struct CgWork : fv_solver_cache {
         r.alloc(c, N);
         z.alloc(c, N);
         p.alloc(c, N);
+        p2.alloc(c, N);
         q.alloc(c, N);
         dinv.alloc(c, N);
         sval.alloc(c, sp->sellTotal);
@@ -68,6 +69,7 @@ struct CgWork : fv_solver_cache {
         h.r = r.p;
         h.z = z.p;
         h.p = p.p;
+        h.p2 = p2.p;
         h.q = q.p;
         h.dinv = dinv.p;
         h.r64 = r64.p;
@@ -103,18 +105,33 @@ struct CgWork : fv_solver_cache {
         kc_init_state<<<1, 64, 0, s>>>(st.p, 1, tol, maxit, delta, atol);
         kc_resid<S, 1><<<G, kB, 0, s>>>(gdev.p, red);
         kc_replace<S, 1><<<G, kB, 0, s>>>(gdev.p, red);
-        kc_direction<S, 2><<<G, kB, 0, s>>>(gdev.p, red);
+        kc_direction<S, 2><<<G, kB, 0, s>>>(gdev.p, red, 0);
     }
-    void iteration(hipStream_t s, bool direction) {
+    // n iterations: the first with a plain SpMV, the others with the previous
+    // direction fused into the SpMV, then the last direction (or, for large
+    // FP64 systems, three kernels per iteration)
+    void chunk(hipStream_t s, int n) {
+        if (!cg_fuse_direction<S>(h.N)) {
+            for (int j = 0; j < n; ++j) {
+                kc_spmv<S><<<G, kB, 0, s>>>(gdev.p, red);
+                kc_update<S><<<G, kB, 0, s>>>(gdev.p, red, 0);
+                kc_direction<S, 0><<<G, kB, 0, s>>>(gdev.p, red, 0);
+            }
+            return;
+        }
         kc_spmv<S><<<G, kB, 0, s>>>(gdev.p, red);
-        kc_update<S><<<G, kB, 0, s>>>(gdev.p, red);
-        if (direction) kc_direction<S, 0><<<G, kB, 0, s>>>(gdev.p, red);
+        kc_update<S><<<G, kB, 0, s>>>(gdev.p, red, 0);
+        for (int j = 1; j < n; ++j) {
+            kc_dirspmv<S><<<G, kB, 0, s>>>(gdev.p, red, (j - 1) & 1, j & 1);
+            kc_update<S><<<G, kB, 0, s>>>(gdev.p, red, j & 1);
+        }
+        kc_direction<S, 0><<<G, kB, 0, s>>>(gdev.p, red, (n - 1) & 1);
     }
     void replacement(hipStream_t s) {
         kc_accum<S><<<G, kB, 0, s>>>(gdev.p);
         kc_resid<S, 0><<<G, kB, 0, s>>>(gdev.p, red);
         kc_replace<S, 0><<<G, kB, 0, s>>>(gdev.p, red);
-        kc_direction<S, 1><<<G, kB, 0, s>>>(gdev.p, red);
+        kc_direction<S, 1><<<G, kB, 0, s>>>(gdev.p, red, 0);
     }
     CgState state(hipStream_t s) {
         CgState o{};
@@ -177,7 +194,7 @@ CgState run_cg(fv_ctx* ctx, CgWork<double>& w, double tol, int maxit, int check,
     while (launched < maxit) {
         const int n = std::min(batch, maxit - launched);
         batch = std::min(std::max(2 * batch, 4), check);
-        for (int i = 0; i < n; ++i) w.iteration(s, true);
+        w.chunk(s, n);
         FV_LAUNCH_CHECK();
         launched += n;
         if ((w.lastIters > 0 && launched >= w.lastIters) || launched >= maxit) {
@@ -240,7 +257,7 @@ IrOut run_mixed(fv_ctx* ctx, CgWork<float>& w, double tol, int maxit, int chunk,
         // short chunks first (short solves end early), then `chunk` iterations
         const int n = std::min(batch, maxit - h.it);
         batch = std::min(2 * batch, chunk);
-        for (int i = 0; i < n; ++i) w.iteration(s, true);
+        w.chunk(s, n);
         w.replacement(s);  // only once the FP32 CG stopped (trigger, estimate, limit)
         FV_LAUNCH_CHECK();
         h = w.state(s);

@@ -153,6 +153,34 @@ def test_cg_against_scipy_and_determinism(ctx, meshes):
     assert relmax(xs[0], ref) < 1e-10
 
 
+def test_cg_large_system_both_iteration_forms(ctx):
+    """A system above 2^20 rows runs the FP64 CG as three kernels per iteration
+    (below it the direction is fused into the SpMV): both forms reach the
+    tolerance on the explicit residual (checked on the host), and the mixed
+    solver (always fused) agrees with the FP64 solution."""
+    import foamadapter_amd as fa
+    from foamadapter_amd import dsl, fvcc, la
+    n = 104                                   # 1,124,864 cells
+    print(f"[case] {n}^3 heat system, FP64 and mixed CG", flush=True)
+    m = fa.Mesh.from_poly(ctx, fa.PolyMesh.hex(n), renumber=True)
+    N, F, B = m.n_cells, m.n_internal_faces, m.n_boundary_faces
+    assert N >= 1 << 20
+    bc = fa.BoundaryConditions(m, ["fixedValue"] * 6)
+    T = fvcc.VolumeField(m, ctx.uniform(N, 300.0, 400.0, 41, 0),
+                         ctx.uniform(B, 300.0, 400.0, 41, 1), bc, "T")
+    kap = fvcc.SurfaceField(m, ctx.upload(np.full(F + B, 1e-1)))
+    ls = dsl.assemble(dsl.imp.ddt(T) - dsl.imp.laplacian(kap, T), T, 0.0, 1e-3)
+    A, b = ls.scipy(), ls.rhs()
+    xs = {}
+    for prec in ("double", "mixed"):
+        x = ctx.upload(T.internal.numpy())
+        st = la.solve_cg(ls, x, {"relTol": 1e-9, "maxIter": 5000, "precision": prec})
+        xs[prec] = x.numpy()
+        r = np.linalg.norm(b - A @ xs[prec]) / np.linalg.norm(b)
+        assert st.finalResNorm <= 1e-9 and r <= 1.01e-9, (prec, st, r)
+    assert relmax(xs["mixed"], xs["double"]) < 1e-7
+
+
 def test_mixed_precision_refinement(ctx):
     """Pressure-Poisson shape: laplacian(1,p) with p=0 on x0, zeroGradient
     elsewhere, b = div(phi) of a random velocity; FP32 CG inside FP64
