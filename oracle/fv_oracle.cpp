@@ -52,6 +52,9 @@
 #include <cstdint>
 #include <cstring>
 #include <vector>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 #include <algorithm>
 
 namespace {
@@ -679,6 +682,168 @@ double orc_courant(int nCells, int nInt, int nB, const int32_t* own, const int32
     double mx = 0.0;
     for (int c = 0; c < nCells; ++c) mx = std::max(mx, s[c] / V[c]);
     return 0.5 * mx * dt;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Multi-core restatement (the CPU baseline of the benchmark, the role of
+// NeoN's OpenMP executor).  Cell-result operators walk each cell's faces in
+// ascending face order -- internal faces in face order with the owner (+) /
+// neighbour (-) sign, then the cell's boundary faces -- which is the order in
+// which the serial face loops above accumulate into that cell, so any thread
+// count returns the serial bits (checked by the CPU tests).  Face-result
+// operators split the face range.
+// ---------------------------------------------------------------------------
+namespace {
+
+struct OmpStencil {
+    int nCells, nInt;
+    std::vector<int32_t> off, idx;  // entry: f (owner side), ~f (neighbour side), nInt + b
+};
+
+}  // namespace
+
+extern "C" {
+
+int orc_omp_threads() {
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
+}
+
+void* orc_omp_stencil(int nCells, int nInt, int nB, const int32_t* own, const int32_t* nbr,
+                      const int32_t* faceCells) {
+    auto* s = new OmpStencil{nCells, nInt, std::vector<int32_t>(nCells + 1, 0), {}};
+    for (int f = 0; f < nInt; ++f) {
+        ++s->off[own[f] + 1];
+        ++s->off[nbr[f] + 1];
+    }
+    for (int b = 0; b < nB; ++b) ++s->off[faceCells[b] + 1];
+    for (int c = 0; c < nCells; ++c) s->off[c + 1] += s->off[c];
+    s->idx.resize(s->off[nCells]);
+    std::vector<int32_t> pos(s->off.begin(), s->off.end() - 1);
+    for (int f = 0; f < nInt; ++f) {
+        s->idx[pos[own[f]]++] = f;
+        s->idx[pos[nbr[f]]++] = ~f;
+    }
+    for (int b = 0; b < nB; ++b) s->idx[pos[faceCells[b]]++] = nInt + b;
+    return s;
+}
+
+void orc_omp_free(void* s) { delete static_cast<OmpStencil*>(s); }
+
+int orc_omp_interp(int nInt, int nB, const int32_t* own, const int32_t* nbr, const double* w,
+                   const double* T, const double* Tb, double* Tf) {
+#pragma omp parallel for schedule(static)
+    for (int f = 0; f < nInt; ++f) Tf[f] = w[f] * (T[own[f]] - T[nbr[f]]) + T[nbr[f]];
+#pragma omp parallel for schedule(static)
+    for (int b = 0; b < nB; ++b) Tf[nInt + b] = Tb[b];
+    return 0;
+}
+
+int orc_omp_sngrad(int nInt, int nB, int nPatches, const int32_t* poff, const int32_t* pkind,
+                   const int32_t* own, const int32_t* nbr, const int32_t* faceCells,
+                   const double* nodc, const double* dcb, const double* T, const double* Tb,
+                   double* out) {
+#pragma omp parallel for schedule(static)
+    for (int f = 0; f < nInt; ++f) out[f] = nodc[f] * (T[nbr[f]] - T[own[f]]);
+#pragma omp parallel for schedule(static)
+    for (int b = 0; b < nB; ++b) {
+        const int k = pkind[patch_of(b, nPatches, poff)];
+        out[nInt + b] = bnd_sngrad<double>(k, dcb[b], Tb[b], T[faceCells[b]], 0.0);
+    }
+    return 0;
+}
+
+int orc_omp_grad(const void* sp, const int32_t* own, const int32_t* nbr, const double* w,
+                 const double* Sf, const double* Sfb, const double* V, const double* T,
+                 const double* Tb, double* out) {
+    const auto& s = *static_cast<const OmpStencil*>(sp);
+    const int F = s.nInt;
+#pragma omp parallel for schedule(static)
+    for (int c = 0; c < s.nCells; ++c) {
+        double a[3] = {0.0, 0.0, 0.0};
+        for (int k = s.off[c]; k < s.off[c + 1]; ++k) {
+            const int e = s.idx[k];
+            if (e >= F) {
+                const int b = e - F;
+                for (int j = 0; j < 3; ++j) a[j] += Sfb[3 * b + j] * Tb[b];
+            } else {
+                const int f = e >= 0 ? e : ~e;
+                const double tf = w[f] * (T[own[f]] - T[nbr[f]]) + T[nbr[f]];
+                for (int j = 0; j < 3; ++j) {
+                    const double v = Sf[3 * (size_t)f + j] * tf;
+                    if (e >= 0)
+                        a[j] += v;
+                    else
+                        a[j] -= v;
+                }
+            }
+        }
+        for (int j = 0; j < 3; ++j) out[3 * (size_t)c + j] = a[j] / V[c];
+    }
+    return 0;
+}
+
+int orc_omp_div(const void* sp, int scheme, const int32_t* own, const int32_t* nbr,
+                const double* w, const double* V, const double* phi, const double* T,
+                const double* Tb, double* out) {
+    const auto& s = *static_cast<const OmpStencil*>(sp);
+    const int F = s.nInt;
+#pragma omp parallel for schedule(static)
+    for (int c = 0; c < s.nCells; ++c) {
+        double a = 0.0;
+        for (int k = s.off[c]; k < s.off[c + 1]; ++k) {
+            const int e = s.idx[k];
+            if (e >= F) {
+                a += phi[e] * Tb[e - F];
+            } else {
+                const int f = e >= 0 ? e : ~e;
+                const double lam = scheme == 0 ? w[f] : (phi[f] >= 0.0 ? 1.0 : 0.0);
+                const double v = phi[f] * (lam * (T[own[f]] - T[nbr[f]]) + T[nbr[f]]);
+                if (e >= 0)
+                    a += v;
+                else
+                    a -= v;
+            }
+        }
+        out[c] = a / V[c];
+    }
+    return 0;
+}
+
+int orc_omp_laplacian(const void* sp, int nPatches, const int32_t* poff, const int32_t* pkind,
+                      const int32_t* own, const int32_t* nbr, const double* magSf,
+                      const double* nodc, const double* magSfb, const double* dcb,
+                      const double* V, const double* gamma, const double* T, const double* Tb,
+                      double* out) {
+    const auto& s = *static_cast<const OmpStencil*>(sp);
+    const int F = s.nInt;
+#pragma omp parallel for schedule(static)
+    for (int c = 0; c < s.nCells; ++c) {
+        double a = 0.0;
+        for (int k = s.off[c]; k < s.off[c + 1]; ++k) {
+            const int e = s.idx[k];
+            if (e >= F) {
+                const int b = e - F;
+                const int kk = pkind[patch_of(b, nPatches, poff)];
+                const double sn = bnd_sngrad<double>(kk, dcb[b], Tb[b], T[c], 0.0);
+                a += (gamma[e] * magSfb[b]) * sn;
+            } else {
+                const int f = e >= 0 ? e : ~e;
+                const double v = (gamma[f] * magSf[f]) * (nodc[f] * (T[nbr[f]] - T[own[f]]));
+                if (e >= 0)
+                    a += v;
+                else
+                    a -= v;
+            }
+        }
+        out[c] = a / V[c];
+    }
+    return 0;
 }
 
 }  // extern "C"

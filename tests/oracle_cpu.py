@@ -350,3 +350,80 @@ def uniform(n, lo, hi, seed, stream):
         k = _mix64(np.uint64(seed) ^ _mix64(np.uint64(stream)))
         u = (_mix64(k + i) >> np.uint64(11)).astype(np.float64) * (1.0 / 9007199254740992.0)
     return lo + (hi - lo) * u
+
+
+# ---- multi-core restatement (the benchmark's CPU baseline) -----------------
+
+lib.orc_omp_stencil.restype = C.c_void_p
+lib.orc_omp_stencil.argtypes = [C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]
+lib.orc_omp_free.argtypes = [C.c_void_p]
+lib.orc_omp_grad.argtypes = [C.c_void_p] * 10
+lib.orc_omp_div.argtypes = [C.c_void_p, C.c_int] + [C.c_void_p] * 8
+lib.orc_omp_laplacian.argtypes = [C.c_void_p, C.c_int] + [C.c_void_p] * 13
+lib.orc_omp_interp.argtypes = [C.c_int, C.c_int] + [C.c_void_p] * 6
+lib.orc_omp_sngrad.argtypes = [C.c_int, C.c_int, C.c_int] + [C.c_void_p] * 10
+
+
+class OmpOps:
+    """FP64 interpolate/grad/div/laplacian/snGrad of the multi-core restatement
+    (cell walks in ascending face order: the serial oracle's bits at any thread
+    count).  Holds the cell->face stencil it builds for g."""
+
+    def __init__(self, g):
+        self.g = g
+        N, F, B = _sizes(g)
+        self.N, self.F, self.B = N, F, B
+        self.h = lib.orc_omp_stencil(N, F, B, P(_i(g["owner"])), P(_i(g["neighbour"])),
+                                     P(_i(g["faceCells"])))
+
+    @staticmethod
+    def threads():
+        return lib.orc_omp_threads()
+
+    def interpolate(self, T, Tb):
+        g = self.g
+        out = np.empty(self.F + self.B)
+        lib.orc_omp_interp(self.F, self.B, P(g["owner"]), P(g["neighbour"]), P(g["weights"]),
+                           P(_f(T)), P(_f(Tb)), P(out))
+        return out
+
+    def grad(self, T, Tb):
+        g = self.g
+        out = np.empty((self.N, 3))
+        lib.orc_omp_grad(self.h, P(g["owner"]), P(g["neighbour"]), P(g["weights"]), P(g["Sf"]),
+                         P(g["Sf_b"]), P(g["V"]), P(_f(T)), P(_f(Tb)), P(out))
+        return out
+
+    def div(self, phi, T, Tb, scheme=0):
+        g = self.g
+        out = np.empty(self.N)
+        lib.orc_omp_div(self.h, scheme, P(g["owner"]), P(g["neighbour"]), P(g["weights"]),
+                        P(g["V"]), P(_f(phi)), P(_f(T)), P(_f(Tb)), P(out))
+        return out
+
+    def laplacian(self, gamma, T, Tb, kinds):
+        g = self.g
+        k, poff = _kinds(g, kinds)
+        out = np.empty(self.N)
+        lib.orc_omp_laplacian(self.h, len(poff) - 1, P(poff), P(k), P(g["owner"]),
+                              P(g["neighbour"]), P(g["magSf"]), P(g["nonOrthDeltaCoeffs"]),
+                              P(g["magSf_b"]), P(g["deltaCoeffs_b"]), P(g["V"]), P(_f(gamma)),
+                              P(_f(T)), P(_f(Tb)), P(out))
+        return out
+
+    def sngrad(self, T, Tb, kinds):
+        g = self.g
+        k, poff = _kinds(g, kinds)
+        out = np.empty(self.F + self.B)
+        lib.orc_omp_sngrad(self.F, self.B, len(poff) - 1, P(poff), P(k), P(g["owner"]),
+                           P(g["neighbour"]), P(g["faceCells"]), P(g["nonOrthDeltaCoeffs"]),
+                           P(g["deltaCoeffs_b"]), P(_f(T)), P(_f(Tb)), P(out))
+        return out
+
+    def close(self):
+        if self.h:
+            lib.orc_omp_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
