@@ -54,8 +54,8 @@ OPS = ["interp_linear", "grad", "div_linear", "div_upwind", "laplacian", "sngrad
 
 # kernel that runs each (op, precision) of the sweep on a renumbered mesh, as rocprofv3 names it
 KERNEL_OF = {
-    ("interp_linear", "f64"): "k_interp<double>", ("interp_linear", "f32"): "k_interp<float>",
-    ("sngrad", "f64"): "k_sngrad<double>", ("sngrad", "f32"): "k_sngrad<float>",
+    ("interp_linear", "f64"): "k_interp4<double>", ("interp_linear", "f32"): "k_interp4<float>",
+    ("sngrad", "f64"): "k_sngrad4<double>", ("sngrad", "f32"): "k_sngrad4<float>",
     ("grad", "f64"): "k_gather<double, GradOp<double>, 256, 256>",
     ("grad", "f32"): "k_gather<float, GradOp<float>, 256, 256>",
     ("div_linear", "f64"): "k_gather<double, DivOp<double, 0>, 512, 512>",

@@ -616,6 +616,102 @@ __global__ __launch_bounds__(kBlock) void k_sngrad(int F, int B, const int32_t* 
     }
 }
 
+// Four consecutive internal faces per thread with 16-byte loads and stores of
+// the face streams (owner, neighbour, weights/coefficients, result), then the
+// tail faces and the boundary faces one per thread; the per-face arithmetic is
+// that of k_interp / k_sngrad.  Used when the face streams are 16-byte aligned.
+template <typename T>
+struct alignas(16) Vec4 {
+    T v[4];
+};
+template <typename T>
+__device__ inline Vec4<T> ld4(const T* p) {
+    return *reinterpret_cast<const Vec4<T>*>(p);
+}
+template <typename T>
+__device__ inline void st4(T* p, const Vec4<T>& x) {
+    *reinterpret_cast<Vec4<T>*>(p) = x;
+}
+inline bool aligned16(const void* p) { return ((uintptr_t)p & 15) == 0; }
+
+template <typename S>
+__global__ __launch_bounds__(kBlock) void k_interp4(int F, int B, int scheme,
+                                                    const int32_t* __restrict__ own,
+                                                    const int32_t* __restrict__ nbr,
+                                                    const S* __restrict__ w,
+                                                    const S* __restrict__ phi,
+                                                    const S* __restrict__ T,
+                                                    const S* __restrict__ Tb, S* __restrict__ Tf) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t nv = F / 4;
+    if (t < nv) {
+        const int64_t f = 4 * t;
+        const Vec4<int32_t> o = ld4(own + f), n = ld4(nbr + f);
+        Vec4<S> lam;
+        if (scheme == 0) {
+            lam = ld4(w + f);
+        } else {
+            const Vec4<S> ph = ld4(phi + f);
+            for (int k = 0; k < 4; ++k) lam.v[k] = ph.v[k] >= S(0) ? S(1) : S(0);
+        }
+        S TO[4], TN[4];
+        for (int k = 0; k < 4; ++k) {
+            TN[k] = T[n.v[k]];
+            TO[k] = T[o.v[k]];
+        }
+        Vec4<S> out;
+        for (int k = 0; k < 4; ++k) out.v[k] = lam.v[k] * (TO[k] - TN[k]) + TN[k];
+        st4(Tf + f, out);
+        return;
+    }
+    const int64_t i = 4 * nv + (t - nv);  // tail faces, then boundary faces
+    if (i < F) {
+        const S lam = scheme == 0 ? w[i] : (phi[i] >= S(0) ? S(1) : S(0));
+        const S TN = T[nbr[i]];
+        Tf[i] = lam * (T[own[i]] - TN) + TN;
+    } else if (i < (int64_t)F + B) {
+        Tf[i] = Tb[i - F];
+    }
+}
+
+template <typename S>
+__global__ __launch_bounds__(kBlock) void k_sngrad4(int F, int B, const int32_t* __restrict__ own,
+                                                    const int32_t* __restrict__ nbr,
+                                                    const int32_t* __restrict__ fc,
+                                                    const S* __restrict__ nodc,
+                                                    const S* __restrict__ dcb,
+                                                    const int8_t* __restrict__ kind,
+                                                    const S* __restrict__ T,
+                                                    const S* __restrict__ Tb,
+                                                    const S* __restrict__ Tg, S* __restrict__ out) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t nv = F / 4;
+    if (t < nv) {
+        const int64_t f = 4 * t;
+        const Vec4<int32_t> o = ld4(own + f), n = ld4(nbr + f);
+        const Vec4<S> d = ld4(nodc + f);
+        S TO[4], TN[4];
+        for (int k = 0; k < 4; ++k) {
+            TN[k] = T[n.v[k]];
+            TO[k] = T[o.v[k]];
+        }
+        Vec4<S> r;
+        for (int k = 0; k < 4; ++k) r.v[k] = d.v[k] * (TN[k] - TO[k]);
+        st4(out + f, r);
+        return;
+    }
+    const int64_t i = 4 * nv + (t - nv);
+    if (i < F) {
+        out[i] = nodc[i] * (T[nbr[i]] - T[own[i]]);
+    } else if (i < (int64_t)F + B) {
+        const int b = (int)(i - F);
+        out[i] = bnd_sngrad<S>(kind[b], dcb[b], Tb[b], T[fc[b]], Tg, b);
+    }
+}
+
+// threads of a k_*4 launch over F internal and B boundary faces
+inline int64_t face4_threads(int F, int B) { return F / 4 + (F % 4) + (int64_t)B; }
+
 template <typename S>
 __global__ __launch_bounds__(kBlock) void k_flux(int F, int B, const int32_t* __restrict__ own,
                                                  const int32_t* __restrict__ nbr,
@@ -724,15 +820,29 @@ int fv_interpolate(fv_ctx* ctx, const fv_mesh* mc, fv_scheme scheme, fv_prec pre
         FV_REQUIRE(scheme == FV_LINEAR || phi, "upwind interpolation needs phi");
         const int n = m->nInt + m->nB;
         if (!n) return;
+        // the four-face kernel needs 16-byte aligned face streams (the caller's
+        // phi and Tf may be offset views)
+        const bool vec = aligned16(Tf) && (scheme == FV_LINEAR || aligned16(phi));
+        const int64_t n4 = face4_threads(m->nInt, m->nB);
         if (prec == FV_F64) {
-            k_interp<double><<<grid_for(n, kBlock), kBlock, 0, ctx->stream>>>(
-                m->nInt, m->nB, scheme, m->own.p, m->nbr.p, m->w.p, (const double*)phi,
-                (const double*)T, (const double*)Tb, (double*)Tf);
+            if (vec)
+                k_interp4<double><<<grid_for(n4, kBlock), kBlock, 0, ctx->stream>>>(
+                    m->nInt, m->nB, scheme, m->own.p, m->nbr.p, m->w.p, (const double*)phi,
+                    (const double*)T, (const double*)Tb, (double*)Tf);
+            else
+                k_interp<double><<<grid_for(n, kBlock), kBlock, 0, ctx->stream>>>(
+                    m->nInt, m->nB, scheme, m->own.p, m->nbr.p, m->w.p, (const double*)phi,
+                    (const double*)T, (const double*)Tb, (double*)Tf);
         } else {
             auto g = geo32(m);
-            k_interp<float><<<grid_for(n, kBlock), kBlock, 0, ctx->stream>>>(
-                m->nInt, m->nB, scheme, m->own.p, m->nbr.p, g.w, (const float*)phi,
-                (const float*)T, (const float*)Tb, (float*)Tf);
+            if (vec)
+                k_interp4<float><<<grid_for(n4, kBlock), kBlock, 0, ctx->stream>>>(
+                    m->nInt, m->nB, scheme, m->own.p, m->nbr.p, g.w, (const float*)phi,
+                    (const float*)T, (const float*)Tb, (float*)Tf);
+            else
+                k_interp<float><<<grid_for(n, kBlock), kBlock, 0, ctx->stream>>>(
+                    m->nInt, m->nB, scheme, m->own.p, m->nbr.p, g.w, (const float*)phi,
+                    (const float*)T, (const float*)Tb, (float*)Tf);
         }
         FV_LAUNCH_CHECK();
     });
@@ -798,17 +908,31 @@ int fv_sngrad_uncorrected(fv_ctx* ctx, const fv_mesh* mc, fv_prec prec, const vo
         FV_REQUIRE(T && out && (Tb || m->nB == 0), "NULL field");
         const int n = m->nInt + m->nB;
         if (!n) return;
-        if (prec == FV_F64)
-            k_sngrad<double><<<grid_for(n, kBlock), kBlock, 0, ctx->stream>>>(
-                m->nInt, m->nB, m->own.p, m->nbr.p, m->faceCells.p, m->nodc.p, m->dcb.p,
-                bc->faceKind.p, (const double*)T, (const double*)Tb, (const double*)Tg,
-                (double*)out);
-        else {
+        const bool vec = aligned16(out);  // the four-face kernel stores 16 bytes at a time
+        const int64_t n4 = face4_threads(m->nInt, m->nB);
+        if (prec == FV_F64) {
+            if (vec)
+                k_sngrad4<double><<<grid_for(n4, kBlock), kBlock, 0, ctx->stream>>>(
+                    m->nInt, m->nB, m->own.p, m->nbr.p, m->faceCells.p, m->nodc.p, m->dcb.p,
+                    bc->faceKind.p, (const double*)T, (const double*)Tb, (const double*)Tg,
+                    (double*)out);
+            else
+                k_sngrad<double><<<grid_for(n, kBlock), kBlock, 0, ctx->stream>>>(
+                    m->nInt, m->nB, m->own.p, m->nbr.p, m->faceCells.p, m->nodc.p, m->dcb.p,
+                    bc->faceKind.p, (const double*)T, (const double*)Tb, (const double*)Tg,
+                    (double*)out);
+        } else {
             auto g = geo32(m);
-            k_sngrad<float><<<grid_for(n, kBlock), kBlock, 0, ctx->stream>>>(
-                m->nInt, m->nB, m->own.p, m->nbr.p, m->faceCells.p, g.nodc, g.dcb,
-                bc->faceKind.p, (const float*)T, (const float*)Tb, (const float*)Tg,
-                (float*)out);
+            if (vec)
+                k_sngrad4<float><<<grid_for(n4, kBlock), kBlock, 0, ctx->stream>>>(
+                    m->nInt, m->nB, m->own.p, m->nbr.p, m->faceCells.p, g.nodc, g.dcb,
+                    bc->faceKind.p, (const float*)T, (const float*)Tb, (const float*)Tg,
+                    (float*)out);
+            else
+                k_sngrad<float><<<grid_for(n, kBlock), kBlock, 0, ctx->stream>>>(
+                    m->nInt, m->nB, m->own.p, m->nbr.p, m->faceCells.p, g.nodc, g.dcb,
+                    bc->faceKind.p, (const float*)T, (const float*)Tb, (const float*)Tg,
+                    (float*)out);
         }
         FV_LAUNCH_CHECK();
     });
