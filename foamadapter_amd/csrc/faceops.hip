@@ -355,6 +355,24 @@ struct Lists {
 
 constexpr int kMaxLow = 8;  // lower faces per cell from inside the workgroup, in LDS
 
+// Sorts 8 keys ascending in registers (the 19-comparator network for 8 inputs:
+// layers (0,2)(1,3)(4,6)(5,7) / (0,4)(1,5)(2,6)(3,7) / (0,1)(2,3)(4,5)(6,7) /
+// (2,4)(3,5) / (1,4)(3,6) / (1,2)(3,4)(5,6)).
+__device__ inline void sort8(uint16_t (&k)[8]) {
+    auto cx = [&](int a, int b) {
+        const uint16_t x = k[a], y = k[b];
+        k[a] = x < y ? x : y;
+        k[b] = x < y ? y : x;
+    };
+    cx(0, 2); cx(1, 3); cx(4, 6); cx(5, 7);
+    cx(0, 4); cx(1, 5); cx(2, 6); cx(3, 7);
+    cx(0, 1); cx(2, 3); cx(4, 5); cx(6, 7);
+    cx(2, 4); cx(3, 5);
+    cx(1, 4); cx(3, 6);
+    cx(1, 2); cx(3, 4); cx(5, 6);
+}
+static_assert(kMaxLow == 8, "sort8 sorts the per-cell lower-face lists");
+
 // A workgroup of CELLS cells.  Phase 1 computes, with coalesced loads, every
 // face contribution the workgroup needs into LDS -- its own faces (streamed),
 // its external lower faces and its boundary faces (from the per-workgroup
@@ -379,7 +397,7 @@ __global__ __launch_bounds__(THREADS) void k_gather(int N, int nBlocks, Split t,
     __shared__ S fluxE[NC][kMaxExt];
     __shared__ S fluxB[NC][kMaxBnd];
     __shared__ uint16_t upS[CELLS], upE[CELLS], exS[CELLS], exE[CELLS], bS[CELLS], bE[CELLS];
-    __shared__ uint16_t lowL[CELLS][kMaxLow];
+    __shared__ uint16_t lowL[kMaxLow][CELLS];  // slot-major: lanes read consecutive cells
     __shared__ int lowN[CELLS];
     __shared__ int overflow;
     const int lb = (int)blockIdx.x;
@@ -486,7 +504,7 @@ __global__ __launch_bounds__(THREADS) void k_gather(int N, int nBlocks, Split t,
                     const int nl = n - c0;
                     const int slot = atomicAdd(&lowN[nl], 1);
                     if (slot < kMaxLow)
-                        lowL[nl][slot] = (uint16_t)i;
+                        lowL[slot][nl] = (uint16_t)i;
                     else
                         overflow = 1;
                 }
@@ -514,23 +532,20 @@ __global__ __launch_bounds__(THREADS) void k_gather(int N, int nBlocks, Split t,
             for (int j = 0; j < NC; ++j) v[j] = fluxE[j][k];
             for (int j = 0; j < NC; ++j) acc[j] -= v[j];
         }
-        // the inner lower faces arrived in any order: sort this cell's list
+        // the inner lower faces arrived in any order: sort this cell's list in
+        // registers (unused slots sort last)
         const int n = lowN[l];
-        uint16_t* lst = lowL[l];
-        for (int a = 1; a < n; ++a) {
-            const uint16_t v = lst[a];
-            int b = a - 1;
-            while (b >= 0 && lst[b] > v) {
-                lst[b + 1] = lst[b];
-                --b;
+        uint16_t key[kMaxLow];
+#pragma unroll
+        for (int a = 0; a < kMaxLow; ++a) key[a] = a < n ? lowL[a][l] : uint16_t(0xFFFF);
+        sort8(key);
+#pragma unroll
+        for (int a = 0; a < kMaxLow; ++a)
+            if (a < n) {
+                const int i = key[a];
+                for (int j = 0; j < NC; ++j) v[j] = fluxU[j][i];
+                for (int j = 0; j < NC; ++j) acc[j] -= v[j];
             }
-            lst[b + 1] = v;
-        }
-        for (int q = 0; q < n; ++q) {
-            const int i = lst[q];
-            for (int j = 0; j < NC; ++j) v[j] = fluxU[j][i];
-            for (int j = 0; j < NC; ++j) acc[j] -= v[j];
-        }
         for (int i = upS[l]; i < upE[l]; ++i) {
             for (int j = 0; j < NC; ++j) v[j] = fluxU[j][i];
             for (int j = 0; j < NC; ++j) acc[j] += v[j];
