@@ -479,7 +479,7 @@ def _uniform_host(n, lo, hi, seed, stream):
     return lo + (hi - lo) * v
 
 
-def bench_served(n_req=96, seed=1005, rate=200.0):
+def bench_served(n_req=96, seed=1005, rate=200.0, tile_rows=1024, chunk=16):
     """Served request stream: Poisson arrivals, 70% heat (5 Euler steps, kappa ~
     logU[1e-4,1e-1]) / 30% pressure Poisson, on jittered permuted hex cubes with
     edge n ~ logU[16,160] (snapped to a catalogue of resident meshes), 200 GB
@@ -541,11 +541,16 @@ def bench_served(n_req=96, seed=1005, rate=200.0):
                 "p99_ms": round(float(np.percentile(lat, 99)), 2),
                 "mean_ms": round(float(lat.mean()), 2), "failed": bad,
                 "peak_active": s["peak_active"], "peak_gb": round(s["peak_bytes"] / 1e9, 2),
-                "warmup_completed": srv_stats0["completed"]}
+                "warmup_completed": srv_stats0["completed"],
+                "worker_ms": {k: round(s[k] - srv_stats0[k], 1)
+                              for k in ("admit_ms", "gpu_ms", "advance_ms")},
+                "chunks": s["chunks"] - srv_stats0["chunks"],
+                "table_builds": s["table_builds"] - srv_stats0["table_builds"],
+                "graph_builds": s["graph_builds"] - srv_stats0["graph_builds"]}
 
     results["plain_fp64"] = run("plain", precision="double", batched=False)
-    results["batched_mixed"] = run("batched", precision="mixed", batched=True, tile_rows=1024,
-                                   chunk=16)
+    results["batched_mixed"] = run("batched", precision="mixed", batched=True,
+                                   tile_rows=tile_rows, chunk=chunk)
     results["speedup_solves_per_s"] = round(results["batched_mixed"]["solves_per_s"] /
                                             results["plain_fp64"]["solves_per_s"], 2)
     results["stream"] = {"requests": n_req, "heat": int((kind == 0).sum()),
@@ -569,6 +574,9 @@ def main():
     ap.add_argument("--batch-problems", type=int, default=512)
     ap.add_argument("--served", action="store_true", help="also run the served request stream")
     ap.add_argument("--requests", type=int, default=96)
+    ap.add_argument("--served-chunk", type=int, default=16,
+                    help="CG iterations per chunk of the served batch")
+    ap.add_argument("--served-tile-rows", type=int, default=1024)
     ap.add_argument("--rate", type=float, default=200.0,
                     help="Poisson arrival rate (requests/s) of the served stream (0: all at once); "
                          "the default offers more than the plain path sustains, so both paths "
@@ -660,7 +668,8 @@ def main():
         rec["batched_heat"] = bench_batched(ctx, P=args.batch_problems)
     if args.served:
         log("served request stream")
-        rec["served"] = bench_served(n_req=args.requests, rate=args.rate)
+        rec["served"] = bench_served(n_req=args.requests, rate=args.rate,
+                                     tile_rows=args.served_tile_rows, chunk=args.served_chunk)
     print(json.dumps(rec), flush=True)
     ctx.close()
     return 0

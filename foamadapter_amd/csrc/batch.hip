@@ -149,6 +149,7 @@ void build_tables(fv_batch* b) {
     // the table holds kXcdGroups groups of perCap entries (the graph's grid);
     // perCap follows per within [per, 1.25 per]: growing past it or shrinking
     // below half of it reallocates and invalidates the captured graph
+    ++b->tableBuilds;
     const int perCap = b->tileCap / kXcdGroups;
     const bool regrow = per > perCap || per < perCap / 2 || P > b->probCap || !b->gridDev.p;
     if (regrow) {
@@ -230,6 +231,7 @@ void run_chunk(fv_batch* b) {
             throw Error{FV_EDEVICE, std::string("hipStreamEndCapture: ") + hipGetErrorString(e)};
         }
         FV_HIP(hipGraphInstantiate(&b->graphExec, b->graph, nullptr, nullptr, 0));
+        ++b->graphBuilds;
     }
     FV_HIP(hipGraphLaunch(b->graphExec, s));
 }

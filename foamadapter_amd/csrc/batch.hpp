@@ -46,6 +46,7 @@ struct fv_batch {
     bool dirty = true;
     hipGraph_t graph = nullptr;
     hipGraphExec_t graphExec = nullptr;
+    int64_t graphBuilds = 0, tableBuilds = 0;  // counters (server statistics)
     bool useGraph = true;
     ~fv_batch();
 };

@@ -85,6 +85,8 @@ struct fv_server {
     std::vector<double> latencies;
     bool haveFirst = false;
     Clock::time_point tFirst, tLast;
+    double admitMs = 0, gpuMs = 0, advanceMs = 0;  // worker time split (under qmu when read)
+    int64_t chunks = 0, tableBuilds = 0, graphBuilds = 0;  // copies of the batch's counters
     size_t peakBytes = 0;
     int peakActive = 0;
     size_t residentBytes = 0;  // meshes, their sparsity patterns and boundary conditions
@@ -286,6 +288,7 @@ void worker_loop(fv_server* s) {
         }
         std::lock_guard<std::mutex> g(s->gpumu);
         // admission
+        const Clock::time_point t0 = Clock::now();
         std::vector<Req*> adm = admit(s, !batched);
         for (Req* r : adm) {
             try {
@@ -302,10 +305,12 @@ void worker_loop(fv_server* s) {
                 finish(s, r, e.code);
             }
         }
+        const Clock::time_point t1 = Clock::now();
         {
             std::lock_guard<std::mutex> lk(s->qmu);
             s->peakActive = std::max(s->peakActive, (int)s->active.size());
             s->peakBytes = std::max(s->peakBytes, s->ctx->in_use);
+            s->admitMs += ms_between(t0, t1);
         }
         if (!batched) {
             Req* r = nullptr;
@@ -324,6 +329,7 @@ void worker_loop(fv_server* s) {
             {
                 std::lock_guard<std::mutex> lk(s->qmu);
                 s->active.erase(s->active.begin());
+                s->gpuMs += ms_between(t1, Clock::now());
             }
             finish(s, r, status);
             continue;
@@ -331,6 +337,11 @@ void worker_loop(fv_server* s) {
         if (s->batch->members.empty()) continue;
         try {
             batch_step(s->batch, states);
+            std::lock_guard<std::mutex> lk(s->qmu);
+            s->gpuMs += ms_between(t1, Clock::now());
+            ++s->chunks;
+            s->tableBuilds = s->batch->tableBuilds;
+            s->graphBuilds = s->batch->graphBuilds;
         } catch (const Error& e) {
             // a device error fails everything in flight
             set_error(e.msg);
@@ -347,6 +358,7 @@ void worker_loop(fv_server* s) {
             continue;
         }
         // retire / advance members (states are in member order)
+        const Clock::time_point t2 = Clock::now();
         std::vector<BMember*> members = s->batch->members;
         for (size_t i = 0; i < members.size(); ++i) {
             BMember* m = members[i];
@@ -392,6 +404,8 @@ void worker_loop(fv_server* s) {
                 }
             }
         }
+        std::lock_guard<std::mutex> lk(s->qmu);
+        s->advanceMs += ms_between(t2, Clock::now());
     }
 }
 
@@ -580,6 +594,12 @@ int fv_server_get_stats(fv_server* s, fv_server_stats* o) {
         o->peak_bytes = s->peakBytes;
         o->peak_active = s->peakActive;
         o->resident_bytes = s->residentBytes;
+        o->admit_ms = s->admitMs;
+        o->gpu_ms = s->gpuMs;
+        o->advance_ms = s->advanceMs;
+        o->chunks = s->chunks;
+        o->table_builds = s->tableBuilds;
+        o->graph_builds = s->graphBuilds;
         if (s->haveFirst && s->completed) o->wall_s = ms_between(s->tFirst, s->tLast) * 1e-3;
         if (o->wall_s > 0) o->solves_per_s = s->completed / o->wall_s;
         if (!s->latencies.empty()) {

@@ -45,7 +45,9 @@ class FvServerStats(C.Structure):
                 ("rejected", C.c_int64), ("cancelled", C.c_int64), ("wall_s", C.c_double),
                 ("solves_per_s", C.c_double), ("p50_ms", C.c_double), ("p99_ms", C.c_double),
                 ("mean_ms", C.c_double), ("max_ms", C.c_double), ("peak_bytes", C.c_size_t),
-                ("peak_active", C.c_int32), ("resident_bytes", C.c_size_t)]
+                ("peak_active", C.c_int32), ("resident_bytes", C.c_size_t),
+                ("admit_ms", C.c_double), ("gpu_ms", C.c_double), ("advance_ms", C.c_double),
+                ("chunks", C.c_int64), ("table_builds", C.c_int64), ("graph_builds", C.c_int64)]
 
 
 HEAT, POISSON = 0, 1

@@ -480,6 +480,13 @@ typedef struct {
     size_t peak_bytes;   /* peak device bytes in use */
     int32_t peak_active; /* peak requests in flight */
     size_t resident_bytes; /* device bytes of the resident meshes */
+    /* where the worker's time went (ms): admitting requests (allocation, inputs,
+     * assembly), GPU chunks (batched: one chunk of every member, with its
+     * synchronisation; plain: whole solves), and advancing requests between
+     * chunks (next Euler step, results, release); chunks run, launch tables
+     * built and chunk graphs captured (batched) */
+    double admit_ms, gpu_ms, advance_ms;
+    int64_t chunks, table_builds, graph_builds;
 } fv_server_stats;
 int fv_server_create(int device, const fv_server_opts* opts, fv_server** out);
 /* Adds a resident mesh (renumber != 0: Z-order renumbered internally). */

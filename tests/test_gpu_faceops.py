@@ -322,3 +322,59 @@ def test_renumbered_methods_bitwise(ctx, method):
     assert np.array_equal(div.numpy(), orc.div(g_ren, phid.numpy(), T[cm], Tb))
     assert relmax(m.to_caller(grad, "cell").numpy(), orc.grad(g, T, Tb)) <= TOL64
     assert relmax(m.to_caller(div, "cell").numpy(), orc.div(g, phi, T, Tb)) <= TOL64
+
+
+@pytest.mark.parametrize("dt", [np.float64, np.float32])
+def test_unaligned_caller_arrays(ctx, dt):
+    """Face arrays the caller passes at an address that is not 16-byte aligned
+    (an offset view): the staged gathers and the scalar interpolate/snGrad
+    kernels (taken when the result is unaligned) give the same bits as the
+    aligned call."""
+    import ctypes as C
+    import foamadapter_amd as fa
+    from foamadapter_amd import _lib
+    poly = fa.PolyMesh.hex(14, jitter=0.2, jitter_seed=61, perm_seed=62)
+    m = fa.Mesh.from_poly(ctx, poly, renumber=True)
+    g = orc.geometry(m.poly)
+    T, Tb, phi, gamma, _ = _fields(g, 63)
+    N, F, B = m.n_cells, m.n_internal_faces, m.n_boundary_faces
+    es = np.dtype(dt).itemsize
+    prec = _lib.FV_F64 if dt == np.float64 else _lib.FV_F32
+    bc = fa.BoundaryConditions(m, ["fixedValue", "zeroGradient"] * 3)
+    Td, Tbd = ctx.upload(T.astype(dt)), ctx.upload(Tb.astype(dt))
+
+    def shifted(a):                 # device copy of a, one element past a 16-byte boundary
+        buf = ctx.upload(np.concatenate([[0.0], a]).astype(dt))
+        return buf, C.c_void_p(buf.ptr.value + es)
+
+    pa, pu = ctx.upload(phi.astype(dt)), shifted(phi)
+    ga, gu = ctx.upload(gamma.astype(dt)), shifted(gamma)
+
+    def run(phi_p, gam_p, outs):
+        o = {k: ctx.empty(sh, dt) for k, sh in outs.items()}
+        o_ptr = {k: v.ptr for k, v in o.items()}
+        _lib.call("fv_div_gauss", ctx.handle, m.handle, 0, prec, 1.0, phi_p, Td.ptr, Tbd.ptr,
+                  o_ptr["div"])
+        _lib.call("fv_div_gauss", ctx.handle, m.handle, 1, prec, 1.0, phi_p, Td.ptr, Tbd.ptr,
+                  o_ptr["up"])
+        _lib.call("fv_laplacian_uncorrected", ctx.handle, m.handle, prec, 1.0, gam_p, Td.ptr,
+                  bc.handle, Tbd.ptr, None, o_ptr["lap"])
+        return {k: v.numpy() for k, v in o.items()}
+
+    shapes = {"div": (N,), "up": (N,), "lap": (N,)}
+    a = run(pa.ptr, ga.ptr, shapes)
+    u = run(pu[1], gu[1], shapes)
+    for k in a:
+        assert np.array_equal(a[k], u[k]), k
+    # an unaligned result of interpolate / snGrad
+    obuf = ctx.empty((F + B + 1,), dt)
+    optr = C.c_void_p(obuf.ptr.value + es)
+    ref = ctx.empty((F + B,), dt)
+    _lib.call("fv_interpolate", ctx.handle, m.handle, 0, prec, None, Td.ptr, Tbd.ptr, ref.ptr)
+    _lib.call("fv_interpolate", ctx.handle, m.handle, 0, prec, None, Td.ptr, Tbd.ptr, optr)
+    assert np.array_equal(obuf.numpy()[1:], ref.numpy())
+    _lib.call("fv_sngrad_uncorrected", ctx.handle, m.handle, prec, Td.ptr, bc.handle, Tbd.ptr,
+              None, ref.ptr)
+    _lib.call("fv_sngrad_uncorrected", ctx.handle, m.handle, prec, Td.ptr, bc.handle, Tbd.ptr,
+              None, optr)
+    assert np.array_equal(obuf.numpy()[1:], ref.numpy())
