@@ -122,9 +122,17 @@ def test_heat_transfer_steps_match_oracle(ctx):
     sp = orc.sparsity(g)
     Tref = T0.copy()
     gam = np.full(m.n_internal_faces + m.n_boundary_faces, kappa)
+    fc = g["faceCells"]
     for step in range(10):
+        before = T.numpy()
         st = dsl.solve(dsl.imp.ddt(T) - dsl.imp.laplacian(kap, T), T, step * dt, dt,
                        solver={"relTol": 1e-10, "maxIter": 1000})
+        # energy: heat gained by the cells = net flux through the walls (to the
+        # solver tolerance; internal fluxes cancel)
+        after = T.numpy()
+        gained = np.sum(g["V"] * (after - before)) / dt
+        wall = kappa * np.sum(g["magSf_b"] * g["deltaCoeffs_b"] * (Tb - after[fc]))
+        assert abs(gained - wall) <= 1e-6 * max(abs(gained), abs(wall)), (step, gained, wall)
         ls = orc.LinSys(g, sp)
         ls.ddt(Tref, dt)
         ls.laplacian(gam, Tb, [0] * 6, coeff=-1.0)

@@ -418,3 +418,42 @@ def test_multicore_restatement_matches_serial_bitwise():
     assert np.array_equal(ops.laplacian(gam, T, Tb, kinds), orc.laplacian(g, gam, T, Tb, kinds))
     assert np.array_equal(ops.sngrad(T, Tb, kinds), orc.sngrad(g, T, Tb, kinds))
     ops.close()
+
+
+def test_heat_time_march_dense_solve_and_energy_balance():
+    """Config-1 heat transfer (ddt(T) - laplacian(1e-2, T), Euler dt = 1e-3,
+    fixedValue walls 300 / 400 / 350) on a 4^3 mesh, 10 steps: the oracle's
+    assembled system solved densely (numpy.linalg.solve) and by the oracle CG
+    agree to 1e-12, and every step conserves energy -- the heat gained by the
+    cells, sum V (T - T_old) / dt, equals the net flux through the walls,
+    sum kappa |Sf_b| deltaCoeffs_b (T_b - T_P) (internal fluxes cancel).  The
+    balance pins the laplacian's boundary term and its sign independently of
+    the assembly code."""
+    poly = PolyMesh.hex(4)
+    g = orc.geometry(poly)
+    sp = orc.sparsity(g)
+    N, F, B = len(g["V"]), len(g["owner"]), len(g["faceCells"])
+    po = g["patchOffsets"]
+    Tb = np.full(B, 350.0)
+    Tb[po[0]:po[1]] = 300.0
+    Tb[po[1]:po[2]] = 400.0
+    T = orc.uniform(N, 300.0, 400.0, 1001, 0)
+    dt, kappa = 1e-3, 1e-2
+    gam = np.full(F + B, kappa)
+    ro, ci = sp["rowOffs"], sp["colIdxs"]
+    fc = g["faceCells"]
+    for step in range(10):
+        ls = orc.LinSys(g, sp)
+        ls.ddt(T, dt)
+        ls.laplacian(gam, Tb, [0] * 6, coeff=-1.0)
+        A = np.zeros((N, N))
+        for r in range(N):
+            A[r, ci[ro[r]:ro[r + 1]]] = ls.values[ro[r]:ro[r + 1]]
+        Td = np.linalg.solve(A, ls.rhs)
+        Tc, st = ls.cg(T, 1e-15, 1000)
+        assert relmax(Tc, Td) <= 1e-12, (step, relmax(Tc, Td))
+        gained = np.sum(g["V"] * (Td - T)) / dt
+        wall = kappa * np.sum(g["magSf_b"] * g["deltaCoeffs_b"] * (Tb - Td[fc]))
+        assert abs(gained - wall) <= 1e-10 * max(abs(gained), abs(wall)), (step, gained, wall)
+        assert Td.min() >= 300.0 and Td.max() <= 400.0          # discrete maximum principle
+        T = Td
