@@ -378,3 +378,41 @@ def test_unaligned_caller_arrays(ctx, dt):
     _lib.call("fv_sngrad_uncorrected", ctx.handle, m.handle, prec, Td.ptr, bc.handle, Tbd.ptr,
               None, optr)
     assert np.array_equal(obuf.numpy()[1:], ref.numpy())
+
+
+@pytest.mark.parametrize("dt", [np.float64, np.float32])
+def test_tiny_meshes_bitwise(ctx, dt):
+    """Edge sizes: one cell (no internal face), two cells, a 2x2x2 block and a
+    3x1x1 row, as given and renumbered: every operator bit-identical to the
+    oracle, and one implicit heat step solved to the oracle's dense solution."""
+    import foamadapter_amd as fa
+    from foamadapter_amd import dsl, fvcc
+    for n in (1, (2, 1, 1), 2, (3, 1, 1)):
+        for renumber in (False, True):
+            poly = fa.PolyMesh.hex(n)
+            print(f"[case] hex {n} renumber={renumber} {np.dtype(dt).name}", flush=True)
+            m = fa.Mesh.from_poly(ctx, poly, renumber=renumber)
+            g = orc.geometry(m.poly if renumber else poly)
+            res, ref = _run_all(ctx, m, g, dt)
+            for op in res:
+                assert np.array_equal(res[op], ref[op]), (n, renumber, op)
+            if dt != np.float64:
+                continue
+            N, F, B = m.n_cells, m.n_internal_faces, m.n_boundary_faces
+            bc = fa.BoundaryConditions(m, ["fixedValue"] * m.n_patches)
+            T0 = orc.uniform(N, 300.0, 400.0, 3, 0)
+            Tb = orc.uniform(B, 300.0, 400.0, 3, 1)
+            T = fvcc.VolumeField(m, ctx.upload(T0), ctx.upload(Tb), bc, "T")
+            kap = fvcc.SurfaceField(m, ctx.upload(np.full(F + B, 0.5)))
+            st = dsl.solve(dsl.imp.ddt(T) - dsl.imp.laplacian(kap, T), T, 0.0, 1e-2,
+                           solver={"relTol": 1e-12, "maxIter": 100})
+            ls = orc.LinSys(g, orc.sparsity(g))
+            ls.ddt(T0, 1e-2)
+            ls.laplacian(np.full(F + B, 0.5), Tb, [0] * m.n_patches, coeff=-1.0)
+            sp = orc.sparsity(g)
+            A = np.zeros((N, N))
+            for r in range(N):
+                A[r, sp["colIdxs"][sp["rowOffs"][r]:sp["rowOffs"][r + 1]]] = \
+                    ls.values[sp["rowOffs"][r]:sp["rowOffs"][r + 1]]
+            assert st.finalResNorm <= 1e-12
+            assert relmax(T.numpy(), np.linalg.solve(A, ls.rhs)) < 1e-11, (n, renumber)
