@@ -14,9 +14,15 @@
 //
 // Pinning (what fixes each routine independently of this file):
 //   face/cell geometry ...... closed forms on uniform hex meshes (exact on dyadic
-//                             spacings), sum(V) = domain volume, sum over a
-//                             cell's faces of +-Sf = 0; jittered geometry is
-//                             "parity unpinned" (needs OpenFOAM's primitiveMesh).
+//                             spacings), on affine images of them (skewed,
+//                             non-orthogonal) and on tapered meshes (trapezoid
+//                             faces: shoelace centroids, integrated cell
+//                             centroids); on jittered meshes sum(V) = domain
+//                             volume, sum over a cell's faces of +-Sf = 0, quad
+//                             Sf = half the diagonals' cross product, V = 1/3
+//                             sum Sf.Cf.  The centre weighting of non-planar faces
+//                             beyond these is "parity unpinned" (it needs
+//                             OpenFOAM's primitiveMesh to compare against).
 //   interpolate/grad/div .... golden OpenFOAM fields test/setup_operator/0/ofGradT,
 //                             ofDivT (and NeoN-serial gradT_Serial, divT_Serial).
 //   laplacian/snGrad ........ closed forms: lap(linear)=0, lap(x^2)=2, snGrad of a

@@ -23,6 +23,7 @@ def relmax(a, b):
 
 def _meshes(fixtures_dir):
     from foamadapter_amd.mesh import PolyMesh, read_polymesh
+    from skewed_meshes import affine_hex, tapered_hex
     return {
         "setup_operator": read_polymesh(fixtures_dir / "setup_operator"),
         "setup_stencil3D": read_polymesh(fixtures_dir / "setup_stencil3D"),
@@ -32,6 +33,9 @@ def _meshes(fixtures_dir):
         "hex7x5x3_jit": PolyMesh.hex((7, 5, 3), lengths=(0.7, 0.5, 0.3), jitter=0.3,
                                      jitter_seed=5),
         "hex40_jit_perm": PolyMesh.hex(40, jitter=0.2, jitter_seed=9, perm_seed=10),
+        # skewed planar faces with closed-form geometry (pinned in test_oracle_cpu)
+        "hex6_affine_perm": affine_hex(6, perm_seed=3)[1],
+        "hex4_tapered": tapered_hex(4)[1],
     }
 
 

@@ -111,6 +111,116 @@ def test_jittered_geometry_invariants():
     assert np.all(g["owner"] < g["neighbour"])
 
 
+def _all_faces(poly):
+    """Face centres and area vectors of every face (internal and boundary)."""
+    nF = poly.n_faces
+    Cf, Sf = np.zeros((nF, 3)), np.zeros((nF, 3))
+    orc.lib.orc_face_geometry(nF, orc.P(orc._i(poly.face_offsets)), orc.P(orc._i(poly.face_points)),
+                              orc.P(orc._f(poly.points)), orc.P(Cf), orc.P(Sf))
+    return Cf, Sf
+
+
+def test_jittered_quad_area_and_volume_identities():
+    """On non-planar jittered faces: the area vector of a quad's triangle fan about
+    any point is half the cross product of its diagonals, and each cell's
+    pyramid-decomposed volume is the divergence theorem's (1/3) sum of Sf.Cf over
+    its outward faces (sum Sf = 0 removes the apex)."""
+    poly = PolyMesh.hex(6, jitter=0.2, jitter_seed=1002, perm_seed=77)
+    Cf, Sf = _all_faces(poly)
+    q = poly.face_points.reshape(-1, 4)
+    p = poly.points
+    diag = 0.5 * np.cross(p[q[:, 2]] - p[q[:, 0]], p[q[:, 3]] - p[q[:, 1]])
+    h2 = (1.0 / 6) ** 2
+    assert np.max(np.abs(Sf - diag)) < 1e-15 * h2 * 10
+    g = orc.geometry(poly)
+    N, nI = len(g["V"]), poly.n_internal_faces
+    vol = np.zeros(N)
+    sfc = np.einsum("ij,ij->i", Sf, Cf)
+    np.add.at(vol, poly.owner, sfc)
+    np.subtract.at(vol, poly.neighbour, sfc[:nI])
+    vol /= 3.0
+    assert np.max(np.abs(vol - g["V"])) < 1e-14 * (1.0 / 6) ** 3
+    # a planted mistake (the vertex average for the face centre) is caught by the
+    # volume identity only at second order, so the tapered-mesh test pins Cf
+
+
+def test_affine_geometry_closed_form():
+    """x -> A x + b of the uniform mesh: faces become skewed, non-orthogonal
+    parallelograms; Sf -> det(A) A^-T Sf, centres map affinely, V -> det(A) V, the
+    linear weights stay 1/2, and Gauss grad / div of linear fields are exact."""
+    from skewed_meshes import AFFINE_A as A, AFFINE_B as b, affine_hex
+    poly0, poly1 = affine_hex(6, perm_seed=3)
+    g0, g1 = orc.geometry(poly0), orc.geometry(poly1)
+    det, AinvT = np.linalg.det(A), np.linalg.inv(A).T
+    tol = 1e-14
+    assert np.max(np.abs(g1["Sf"] - det * g0["Sf"] @ AinvT.T)) < tol
+    assert np.max(np.abs(g1["Sf_b"] - det * g0["Sf_b"] @ AinvT.T)) < tol
+    assert np.max(np.abs(g1["Cf"] - (g0["Cf"] @ A.T + b))) < tol
+    assert np.max(np.abs(g1["Cf_b"] - (g0["Cf_b"] @ A.T + b))) < tol
+    assert np.max(np.abs(g1["C"] - (g0["C"] @ A.T + b))) < tol
+    assert np.max(np.abs(g1["V"] - det * g0["V"])) < tol * g0["V"][0]
+    assert np.max(np.abs(g1["weights"] - 0.5)) < 1e-13
+    d = (g0["C"][g0["neighbour"]] - g0["C"][g0["owner"]]) @ A.T
+    assert np.max(np.abs(g1["deltaCoeffs"] * np.linalg.norm(d, axis=1) - 1.0)) < 1e-13
+    # the mesh really is non-orthogonal: face normals are not along the cell deltas
+    cosang = np.einsum("ij,ij->i", g1["Sf"], d) / (g1["magSf"] * np.linalg.norm(d, axis=1))
+    assert cosang.min() < 0.95
+    # Gauss gradient and Gauss linear divergence of linear fields: exact on
+    # planar faces with linear interpolation to the face centroid
+    a, c, U = np.array([0.7, -1.3, 0.4]), 2.0, np.array([0.3, -0.5, 0.9])
+    T = g1["C"] @ a + c
+    Tb = g1["Cf_b"] @ a + c
+    assert np.max(np.abs(orc.grad(g1, T, Tb) - a)) < 1e-12
+    phi = np.concatenate([g1["Sf"] @ U, g1["Sf_b"] @ U])
+    assert np.max(np.abs(orc.div(g1, phi, T, Tb) - U @ a)) < 1e-12
+    Tf = orc.interpolate(g1, T, Tb)
+    assert np.max(np.abs(Tf[:len(g1["owner"])] - (g1["Cf"] @ a + c))) < 1e-13
+
+
+def _shoelace_xy(poly_xy):
+    """Area and centroid of a planar polygon from its (x, y) vertices (shoelace)."""
+    x, y = poly_xy[:, 0], poly_xy[:, 1]
+    xn, yn = np.roll(x, -1), np.roll(y, -1)
+    cr = x * yn - xn * y
+    A = 0.5 * cr.sum()
+    return abs(A), np.array([((x + xn) * cr).sum(), ((y + yn) * cr).sum()]) / (6.0 * A)
+
+
+def test_tapered_geometry_centroids():
+    """y -> y (1 + alpha x): every face planar, z-faces trapezoids.  Their centres
+    are the polygon centroids (shoelace), not the vertex averages; cell volumes and
+    centroids follow from integrating the trapezoid (closed form below)."""
+    from skewed_meshes import TAPER_ALPHA as al, tapered_hex
+    n = 4
+    h = 1.0 / n
+    poly0, poly1 = tapered_hex(n)
+    Cf, Sf = _all_faces(poly1)
+    p = poly1.points
+    q = poly1.face_points.reshape(-1, 4)
+    zf = np.abs(Sf[:, 2]) > 0.5 * np.linalg.norm(Sf, axis=1)
+    avg_err = 0.0
+    for f in np.nonzero(zf)[0]:
+        area, cxy = _shoelace_xy(p[q[f]][:, :2])
+        assert abs(np.linalg.norm(Sf[f]) - area) < 1e-15
+        assert np.max(np.abs(Cf[f, :2] - cxy)) < 1e-15
+        assert abs(Cf[f, 2] - p[q[f][0], 2]) < 1e-15
+        avg_err = max(avg_err, np.max(np.abs(p[q[f]][:, :2].mean(0) - cxy)))
+    assert avg_err > 1e-3           # a vertex-average face centre would fail the check above
+    g0, g1 = orc.geometry(poly0), orc.geometry(poly1)
+    ijk = np.floor(g0["C"] / h).astype(int)
+    x0, y0, z0 = (ijk * h).T
+    x1, y1 = x0 + h, y0 + h
+    # trapezoid: x in [x0, x1], y' in [y0, y1] * (1 + al x)
+    I0 = x1 - x0
+    I1 = (x1 ** 2 - x0 ** 2) / 2
+    I2 = (x1 ** 3 - x0 ** 3) / 3
+    area = (y1 - y0) * (I0 + al * I1)
+    xbar = (y1 - y0) * (I1 + al * I2) / area
+    ybar = 0.5 * (y1 ** 2 - y0 ** 2) * (I0 + 2 * al * I1 + al * al * I2) / area
+    assert np.max(np.abs(g1["V"] - area * h)) < 1e-17
+    assert np.max(np.abs(g1["C"] - np.stack([xbar, ybar, z0 + h / 2], 1))) < 1e-15
+
+
 @pytest.mark.parametrize("perm", [0, 5])
 def test_laplacian_closed_forms(perm):
     n = 8
