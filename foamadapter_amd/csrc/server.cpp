@@ -61,7 +61,10 @@ struct Req {
     BMember* member = nullptr;
     int step = 0, iters = 0;
     double res = 0.0;
-    std::vector<double> result;
+    // the solution in the caller's numbering, in pinned host memory (from the
+    // server's pool; returned to it when the request is collected or dropped)
+    double* result = nullptr;
+    size_t resultN = 0;
 };
 
 }  // namespace
@@ -91,9 +94,53 @@ struct fv_server {
     int peakActive = 0;
     size_t residentBytes = 0;  // meshes, their sparsity patterns and boundary conditions
     size_t inFlightBytes = 0;  // estimates of the admitted, unfinished requests
+    // Pinned host buffers for results, cached by element count: a device-to-host
+    // copy into pinned memory runs at the link's rate (pageable memory goes
+    // through the runtime's staging, 3-4x slower), and hipHostMalloc itself is
+    // too slow to call per request.
+    std::mutex pinmu;
+    std::unordered_multimap<size_t, double*> pinFree;
+    size_t pinCached = 0;  // bytes in pinFree
 };
 
 namespace {
+
+constexpr size_t kPinCacheBytes = size_t(4) << 30;  // pinned result buffers kept for reuse
+
+double* pinned_take(fv_server* s, size_t n) {
+    {
+        std::lock_guard<std::mutex> lk(s->pinmu);
+        auto it = s->pinFree.find(n);
+        if (it != s->pinFree.end()) {
+            double* p = it->second;
+            s->pinFree.erase(it);
+            s->pinCached -= n * sizeof(double);
+            return p;
+        }
+    }
+    void* p = nullptr;
+    FV_HIP(hipHostMalloc(&p, std::max<size_t>(n, 1) * sizeof(double), hipHostMallocDefault));
+    return (double*)p;
+}
+
+void pinned_give(fv_server* s, double* p, size_t n) {
+    if (!p) return;
+    {
+        std::lock_guard<std::mutex> lk(s->pinmu);
+        if (s->pinCached + n * sizeof(double) <= kPinCacheBytes) {
+            s->pinFree.emplace(n, p);
+            s->pinCached += n * sizeof(double);
+            return;
+        }
+    }
+    (void)hipHostFree(p);
+}
+
+// Frees a request and the pinned buffer of its result.
+void drop_request(fv_server* s, Req* r) {
+    pinned_give(s, r->result, r->resultN);
+    delete r;
+}
 
 // Device bytes of one request: its linear system, fields and face coefficient,
 // and the solver's matrix copy and Krylov vectors in the server's precision.
@@ -198,14 +245,15 @@ void finish(fv_server* s, Req* r, int status) {
     MeshEntry& me = *s->meshes[r->d.mesh_id];
     fv_ctx* ctx = s->ctx;
     if (status == FV_OK) {
-        r->result.resize(me.N);
+        r->result = pinned_take(s, me.N);
+        r->resultN = me.N;
         if (me.renumbered) {
             DBuf<double> tmp;
             tmp.alloc(ctx, me.N);
             check(fv_permute(ctx, FV_F64, me.N, 1, 1, 0, me.cellMap.p, r->x.p, tmp.p));
-            tmp.download(r->result.data());
+            tmp.download(r->result);
         } else {
-            r->x.download(r->result.data());
+            r->x.download(r->result);
         }
     }
     free_request_device(r);
@@ -331,7 +379,12 @@ void worker_loop(fv_server* s) {
                 s->active.erase(s->active.begin());
                 s->gpuMs += ms_between(t1, Clock::now());
             }
-            finish(s, r, status);
+            try {
+                finish(s, r, status);
+            } catch (const Error& e) {
+                set_error(e.msg);
+                finish(s, r, e.code);
+            }
             continue;
         }
         if (s->batch->members.empty()) continue;
@@ -471,6 +524,18 @@ int fv_server_add_mesh(fv_server* s, const fv_polymesh_desc* d, int renumber, in
         if (np) kp[0] = FV_BC_FIXED_VALUE;
         check(fv_bc_create(me->mesh, kh.data(), &me->bcHeat));
         check(fv_bc_create(me->mesh, kp.data(), &me->bcPoisson));
+        {
+            // A Poisson request's explicit surface integral builds the mesh's
+            // staged-gather lists on first use (24 ms at 128^3, 55 ms at 160^3):
+            // build them with the mesh, by the same call, not in a request.
+            DBuf<double> phi, out;
+            phi.alloc(s->ctx, (size_t)me->mesh->nInt + me->mesh->nB);
+            out.alloc(s->ctx, (size_t)me->mesh->nCells);
+            check(fv_fill(s->ctx, FV_F64, phi.p, (int64_t)phi.n, 0.0));
+            check(fv_div_gauss(s->ctx, me->mesh, FV_LINEAR, FV_F64, 1.0, phi.p, nullptr, nullptr,
+                               out.p));
+            check(fv_sync(s->ctx));
+        }
         me->N = me->mesh->nCells;
         me->F = me->mesh->nInt;
         me->B = me->mesh->nB;
@@ -551,10 +616,11 @@ int fv_server_wait(fv_server* s, int64_t id, double* h_result, fv_request_stats*
             stats->queue_ms = r->tStart.time_since_epoch().count() ? ms_between(r->tSubmit, r->tStart) : 0.0;
             stats->latency_ms = ms_between(r->tSubmit, r->tDone);
         }
-        if (h_result && r->status == FV_OK) std::copy(r->result.begin(), r->result.end(), h_result);
+        if (h_result && r->status == FV_OK) std::copy(r->result, r->result + r->resultN, h_result);
         status = r->status;
         s->reqs.erase(it);
-        delete r;
+        lk.unlock();
+        drop_request(s, r);
     });
     if (rc != FV_OK) return rc;
     if (status != FV_OK) set_error("request finished with status " + std::to_string(status));
@@ -631,7 +697,9 @@ int fv_server_destroy(fv_server* s) {
         }
         if (s->worker.joinable()) s->worker.join();
         for (Req* r : s->active) free_request_device(r);
-        for (auto& kv : s->reqs) delete kv.second;
+        for (auto& kv : s->reqs) drop_request(s, kv.second);
+        for (auto& kv : s->pinFree) (void)hipHostFree(kv.second);
+        s->pinFree.clear();
         if (s->batch) fv_batch_destroy(s->batch);
         for (auto& me : s->meshes) {
             if (me->bcHeat) fv_bc_destroy(me->bcHeat);
