@@ -113,6 +113,43 @@ __device__ inline S* cg_pbuf(const CgSys<S>& P, int b) {
     return b ? P.p2 : P.p;
 }
 
+// Software pipeline of the sliced-ELL rows of one thread: fetch() issues a
+// row's slice offsets and its first K values and columns into the "next"
+// registers, promote() makes them the current row.  The caller issues the
+// current row's gathers and fetches the row after it before it waits for
+// them, so a row costs about one memory round trip instead of three (measured
+// at 128^3: FP64 solve 72 -> 67 ms, mixed 49 -> 45 ms; fetching two rows
+// ahead in FP32 gains nothing more).
+template <typename S, int K>
+struct CgRowPipe {
+    S v[K] = {}, vn[K] = {};
+    int col[K] = {}, cn[K] = {};
+    int64_t b = 0, bn = 0;  // entry 0 of the row (slice start + lane)
+    int w = 0, wn = 0;      // slice width
+    __device__ void fetch(const int64_t* __restrict__ soff, const int32_t* __restrict__ scol,
+                          const S* __restrict__ sval, int r) {
+        const int sl = r >> 6;
+        const int64_t s0 = soff[sl];
+        bn = s0 + (r & 63);
+        wn = (int)((soff[sl + 1] - s0) >> 6);
+#pragma unroll
+        for (int k = 0; k < K; ++k)
+            if (k < wn) {
+                vn[k] = sval[bn + 64 * (int64_t)k];
+                cn[k] = scol[bn + 64 * (int64_t)k];
+            }
+    }
+    __device__ void promote() {
+        b = bn;
+        w = wn;
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            v[k] = vn[k];
+            col[k] = cn[k];
+        }
+    }
+};
+
 // MODE-0 stop decision from ||r|| (rn) after `it` iterations
 __device__ inline int cg_decide(const CgState* st, double rn, int it) {
     const double est = rn / st->bnorm;
@@ -131,10 +168,11 @@ __global__ __launch_bounds__(kCgBlock) void kc_spmv(const CgGrid<S>* __restrict_
     const CgSys<S>& P = *Pp;
     if (P.st->stop) return;
     // One row per thread and pass, the first kSpE entries with their loads
-    // issued together (values and columns, then the gathers of p): three
-    // dependent memory round trips per row instead of two per entry (measured
-    // at 128^3: the FP64 solve 94 -> 72 ms; two rows per pass gain nothing
-    // more).  The row sums its entries in column order.
+    // issued together (values and columns, then the gathers of p), and the
+    // rows software-pipelined: the next row's slice offsets, values and
+    // columns are in flight while this row's gathers complete, so a row costs
+    // about one memory round trip instead of three.  The row sums its entries
+    // in column order.
     constexpr int kSpE = 8;
     const int64_t* __restrict__ soff = P.sellOff;
     const int32_t* __restrict__ scol = P.sellCol;
@@ -142,30 +180,24 @@ __global__ __launch_bounds__(kCgBlock) void kc_spmv(const CgGrid<S>* __restrict_
     const S* __restrict__ pvec = P.p;
     S* __restrict__ qvec = P.q;
     const int N = P.N;
+    CgRowPipe<S, kSpE> rp;
     double pq = 0.0;
-    for (int r = row0 + threadIdx.x; r < row1; r += kCgBlock) {
-        const int sl = r >> 6, l = r & 63;
-        const int64_t b = soff[sl];
-        const int w = (int)((soff[sl + 1] - b) >> 6);
-        S v[kSpE], pv[kSpE];
-        int col[kSpE];
-#pragma unroll
-        for (int k = 0; k < kSpE; ++k)
-            if (k < w) {
-                const int64_t pos = b + 64 * (int64_t)k + l;
-                v[k] = sval[pos];
-                col[k] = scol[pos];
-            }
+    const int r0 = row0 + (int)threadIdx.x;
+    if (r0 < row1) rp.fetch(soff, scol, sval, r0);
+    for (int r = r0; r < row1; r += kCgBlock) {
+        rp.promote();
+        S pv[kSpE];
         const S own = r < N ? pvec[r] : S(0);
 #pragma unroll
         for (int k = 0; k < kSpE; ++k)
-            if (k < w) pv[k] = pvec[col[k]];
+            if (k < rp.w) pv[k] = pvec[rp.col[k]];
+        if (r + kCgBlock < row1) rp.fetch(soff, scol, sval, r + kCgBlock);
         S sum = 0;
 #pragma unroll
         for (int k = 0; k < kSpE; ++k)
-            if (k < w) sum += v[k] * pv[k];
-        for (int k = kSpE; k < w; ++k) {  // rows wider than kSpE entries
-            const int64_t pos = b + 64 * (int64_t)k + l;
+            if (k < rp.w) sum += rp.v[k] * pv[k];
+        for (int k = kSpE; k < rp.w; ++k) {  // rows wider than kSpE entries
+            const int64_t pos = rp.b + 64 * (int64_t)k;
             sum += sval[pos] * pvec[scol[pos]];
         }
         if (r < N) {
@@ -200,33 +232,30 @@ __global__ __launch_bounds__(kCgBlock) void kc_dirspmv(const CgGrid<S>* __restri
     S* __restrict__ pn = cg_pbuf(P, pNew);
     S* __restrict__ qvec = P.q;
     const int N = P.N;
-    S v[kSpE], zc[kSpE], pc[kSpE], zr = S(0), pr = S(0);
-    int w = 0;
-    int64_t b = 0;
-    auto load = [&](int r) {
-        const int sl = r >> 6, l = r & 63;
-        b = soff[sl] + l;
-        w = (int)((soff[sl + 1] - soff[sl]) >> 6);
-        int col[kSpE];
-#pragma unroll
-        for (int k = 0; k < kSpE; ++k)
-            if (k < w) {
-                v[k] = sval[b + 64 * (int64_t)k];
-                col[k] = scol[b + 64 * (int64_t)k];
-            }
+    // rows software-pipelined as in kc_spmv: gather() promotes the fetched row
+    // and issues its gathers of z and p; the row after it is fetched before
+    // the current one is summed
+    CgRowPipe<S, kSpE> rp;
+    S zc[kSpE], pc[kSpE], zr = S(0), pr = S(0);
+    auto gather = [&](int r) {
+        rp.promote();
         if (r < N) {
             zr = zv[r];
             pr = po[r];
         }
 #pragma unroll
         for (int k = 0; k < kSpE; ++k)
-            if (k < w) {
-                zc[k] = zv[col[k]];
-                pc[k] = po[col[k]];
+            if (k < rp.w) {
+                zc[k] = zv[rp.col[k]];
+                pc[k] = po[rp.col[k]];
             }
     };
     const int r0 = row0 + (int)threadIdx.x;
-    if (r0 < row1) load(r0);
+    if (r0 < row1) {
+        rp.fetch(soff, scol, sval, r0);
+        gather(r0);
+        if (r0 + kCgBlock < row1) rp.fetch(soff, scol, sval, r0 + kCgBlock);
+    }
     double tot[2];
     sum_slots<2>(red.rz, P.tile0, P.ntiles, sh, tot);
     const double rn = sqrt(tot[0]);
@@ -240,14 +269,17 @@ __global__ __launch_bounds__(kCgBlock) void kc_dirspmv(const CgGrid<S>* __restri
     const S beta = (S)(tot[1] / st->rhoPrev);
     double pq = 0.0;
     for (int r = r0; r < row1; r += kCgBlock) {
-        if (r != r0) load(r);
+        if (r != r0) {
+            gather(r);
+            if (r + kCgBlock < row1) rp.fetch(soff, scol, sval, r + kCgBlock);
+        }
         S sum = 0;
 #pragma unroll
         for (int k = 0; k < kSpE; ++k)
-            if (k < w) sum += v[k] * (zc[k] + beta * pc[k]);
-        for (int k = kSpE; k < w; ++k) {  // rows wider than kSpE entries
-            const int c = scol[b + 64 * (int64_t)k];
-            sum += sval[b + 64 * (int64_t)k] * (zv[c] + beta * po[c]);
+            if (k < rp.w) sum += rp.v[k] * (zc[k] + beta * pc[k]);
+        for (int k = kSpE; k < rp.w; ++k) {  // rows wider than kSpE entries
+            const int c = scol[rp.b + 64 * (int64_t)k];
+            sum += sval[rp.b + 64 * (int64_t)k] * (zv[c] + beta * po[c]);
         }
         if (r < N) {
             const S own = zr + beta * pr;
