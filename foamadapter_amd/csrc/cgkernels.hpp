@@ -526,6 +526,8 @@ __global__ __launch_bounds__(kCgBlock) void kc_replace(const CgGrid<S>* __restri
         if (START) {
             st->bnorm = bn;
             st->res0 = res;
+        } else {
+            st->nrep += 1;
         }
     }
     double rr = 0.0, rz = 0.0;

@@ -27,11 +27,13 @@ constexpr int kCgBlock = 256;
 //            direction kernels divide the new r.z by it for beta
 //   pbuf     which of the two search-direction buffers holds the p of the
 //            latest update (written by the update kernel)
+//   nrep     FP64 residual replacements so far (advanced by the replacement
+//            kernel; read by the host)
 struct CgState {
     double rho, rhoPrev, alpha, bnorm, res, res0, rrep, delta;
     int32_t it, done, maxit, stop;
     double tol, atol;  // relative and absolute (0: off) residual targets
-    int32_t pbuf, pad;
+    int32_t pbuf, nrep;
 };
 
 // Fixed-shape block reduction of NV values at once: wavefront shuffles, then

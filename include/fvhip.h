@@ -342,7 +342,7 @@ typedef struct {
     double init_res;      /* SolverStats::initResNorm, relative */
     double final_res;     /* SolverStats::finalResNorm, relative */
     int32_t iters;        /* SolverStats::numIter (inner iterations for F32_IR) */
-    int32_t outer_iters;  /* refinement steps (1 for FP64) */
+    int32_t outer_iters;  /* FP64 residual replacements (F32_IR; 1 for FP64) */
     double true_res;      /* explicit FP64 ||b - A x|| / ||b|| at exit */
 } fv_solver_stats;
 /* Jacobi-preconditioned CG on the assembled system, x as initial guess and
