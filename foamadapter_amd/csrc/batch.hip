@@ -595,7 +595,7 @@ int fv_batch_solve(fv_batch* b, const fv_solve_opts* o, int use_graph, fv_solver
         FV_REQUIRE(o->rel_tol >= 0.0 && o->max_iter >= 0, "bad solver options");
         b->tol = o->rel_tol;
         b->maxit = o->max_iter;
-        b->delta = b->prec == FV_SOLVE_F32_IR ? (o->inner_rel_tol > 0.0 ? o->inner_rel_tol : 1e-3)
+        b->delta = b->prec == FV_SOLVE_F32_IR ? (o->inner_rel_tol > 0.0 ? o->inner_rel_tol : kIrDelta)
                                               : 0.0;
         if (b->useGraph != (use_graph != 0)) drop_graph(b);
         b->useGraph = use_graph != 0;

@@ -27,7 +27,7 @@ struct fv_batch {
     int prec = FV_SOLVE_F32_IR;  // inner precision of the whole batch
     int tileRows = 1024;         // rows per tile (multiple of 64)
     int chunk = 16;              // CG iterations per FP64 residual replacement
-    double tol = 1e-8, delta = 1e-3;
+    double tol = 1e-8, delta = fvhip::kIrDelta;
     int maxit = 1000;
     int nextId = 0;
     std::vector<BMember*> members;

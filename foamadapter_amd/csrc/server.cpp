@@ -308,7 +308,7 @@ void run_plain(fv_server* s, Req* r) {
     o.rel_tol = r->d.rel_tol;
     o.max_iter = r->d.max_iter;
     o.mode = s->opts.mode;
-    o.inner_rel_tol = 1e-3;
+    o.inner_rel_tol = 0.0;  // the library's default trigger
     o.check_every = s->opts.chunk > 0 ? s->opts.chunk : 32;
     const int steps = r->d.kind == FV_REQ_HEAT ? std::max(1, r->d.steps) : 1;
     for (int st = 0; st < steps; ++st) {

@@ -304,7 +304,7 @@ int fv_solve_cg(fv_linsys* ls, void* d_x, const fv_solve_opts* o, fv_solver_stat
             w.bind(ctx, ls, x);
             IrOut r = run_mixed(ctx, w, o->rel_tol, o->max_iter,
                                 o->check_every > 0 ? o->check_every : 64,
-                                o->inner_rel_tol > 0.0 ? o->inner_rel_tol : 1e-3,
+                                o->inner_rel_tol > 0.0 ? o->inner_rel_tol : kIrDelta,
                                 o->max_outer > 0 ? o->max_outer : (1 << 30), o->abs_tol);
             local.init_res = r.init_res;
             local.final_res = r.res;

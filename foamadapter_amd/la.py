@@ -215,7 +215,7 @@ def solver_opts(solver: dict | None) -> _lib.FvSolveOpts:
     # mixed precision: an FP64 residual replacement ends each FP32 chunk of at
     # most checkEvery iterations, or earlier once the FP32 residual fell by
     # innerRelTol; FP64: the host polls the stop flag every checkEvery iterations
-    o.inner_rel_tol = float(d.get("innerRelTol", 1e-3))
+    o.inner_rel_tol = float(d.get("innerRelTol", 0.0))  # 0: the library's default
     o.max_outer = int(d.get("maxOuter", 0))
     o.check_every = int(d.get("checkEvery", 32))
     _method(d)
