@@ -105,7 +105,7 @@ struct fv_server {
 
 namespace {
 
-constexpr size_t kPinCacheBytes = size_t(4) << 30;  // pinned result buffers kept for reuse
+constexpr size_t kPinCacheBytes = size_t(1) << 30;  // pinned result buffers kept for reuse
 
 double* pinned_take(fv_server* s, size_t n) {
     {
