@@ -114,6 +114,29 @@ def test_result_same_alone_and_in_mix(polys):
     srv.close()
 
 
+def test_results_through_pinned_pool(polys):
+    """Results wait in pinned host buffers reused from a pool: repeats of one
+    request, interleaved with requests on meshes of other sizes and collected
+    newest first, return the first run's bits; requests never collected are
+    released when the server closes."""
+    srv, ids = _server(polys, batched=True, precision="mixed", tile_rows=256, chunk=8)
+    ref, st = srv.wait(srv.submit_heat(ids[1], kappa=2e-2, steps=2, seed=5), ids[1])
+    assert st.status == 0
+    repeats, others = [], []
+    for k in range(3):
+        repeats.append(srv.submit_heat(ids[1], kappa=2e-2, steps=2, seed=5))
+        others.append((srv.submit_poisson(ids[k], seed=20 + k), ids[k]))
+    srv.submit_heat(ids[0], kappa=1e-2, seed=40)      # never collected
+    srv.submit_poisson(ids[2], seed=41)                # never collected
+    for r in reversed(repeats):
+        res, st = srv.wait(r, ids[1])
+        assert st.status == 0 and np.array_equal(res, ref)
+    for r, mid in others:
+        res, st = srv.wait(r, mid)
+        assert st.status == 0 and res.shape == (polys[mid].n_cells,) and np.all(np.isfinite(res))
+    srv.close()
+
+
 def test_budget_admission_and_rejection(polys):
     from foamadapter_amd import FvError
     from foamadapter_amd.serve import HEAT
