@@ -53,16 +53,38 @@ __device__ inline bool bi_locate(const BiSys& P, int& row0, int& row1, int& slot
     return true;
 }
 
-__device__ inline double sell_row(const BiSys& P, int r, const double* __restrict__ vec) {
-    const int sl = r >> 6, l = r & 63;
-    const int64_t b = P.sellOff[sl];
-    const int w = (int)((P.sellOff[sl + 1] - b) >> 6);
-    double s = 0.0;
-    for (int k = 0; k < w; ++k) {
-        const int64_t pos = b + 64 * (int64_t)k + l;
-        s += P.sval[pos] * vec[P.sellCol[pos]];
+// (A vec)_i for this thread's rows of [row0, row1), handed to emit(i, value)
+// for i < N.  The rows are software-pipelined as in the CG's SpMV: a row's
+// first kSpE values and columns are loaded together, its gathers are issued,
+// and the next row's slice offsets, values and columns are fetched before the
+// gathers are waited for.  Entries are summed in column order.
+template <typename F>
+__device__ inline void sell_rows(const BiSys& P, int row0, int row1,
+                                 const double* __restrict__ vec, F&& emit) {
+    constexpr int kSpE = 8;
+    const int64_t* __restrict__ soff = P.sellOff;
+    const int32_t* __restrict__ scol = P.sellCol;
+    const double* __restrict__ sval = P.sval;
+    CgRowPipe<double, kSpE> rp;
+    const int r0 = row0 + (int)threadIdx.x;
+    if (r0 < row1) rp.fetch(soff, scol, sval, r0);
+    for (int r = r0; r < row1; r += kB) {
+        rp.promote();
+        double g[kSpE];
+#pragma unroll
+        for (int k = 0; k < kSpE; ++k)
+            if (k < rp.w) g[k] = vec[rp.col[k]];
+        if (r + kB < row1) rp.fetch(soff, scol, sval, r + kB);
+        double sum = 0.0;
+#pragma unroll
+        for (int k = 0; k < kSpE; ++k)
+            if (k < rp.w) sum += rp.v[k] * g[k];
+        for (int k = kSpE; k < rp.w; ++k) {  // rows wider than kSpE entries
+            const int64_t pos = rp.b + 64 * (int64_t)k;
+            sum += sval[pos] * vec[scol[pos]];
+        }
+        if (r < P.N) emit(r, sum);
     }
-    return s;
 }
 
 // r = b - A x, rhat = r; partials r.r, rhat.r (= r.r) and b.b
@@ -72,16 +94,13 @@ __global__ __launch_bounds__(kB) void kbi_start(const BiSys* __restrict__ Pp) {
     int row0, row1, slot;
     if (!bi_locate(P, row0, row1, slot)) return;
     double rr = 0.0, bb = 0.0;
-    for (int i = row0 + threadIdx.x; i < row1; i += kB) {
-        const double ax = sell_row(P, i, P.x);
-        if (i < P.N) {
-            const double ri = P.rhs[i] - ax;
-            P.r[i] = ri;
-            P.rhat[i] = ri;
-            rr += ri * ri;
-            bb += P.rhs[i] * P.rhs[i];
-        }
-    }
+    sell_rows(P, row0, row1, P.x, [&](int i, double ax) {
+        const double ri = P.rhs[i] - ax;
+        P.r[i] = ri;
+        P.rhat[i] = ri;
+        rr += ri * ri;
+        bb += P.rhs[i] * P.rhs[i];
+    });
     double v[2] = {rr, rr}, w[1] = {bb};
     write_slot<2>(v, P.pr, slot, sh);
     write_slot<1>(w, P.pb, slot, sh);
@@ -163,13 +182,10 @@ __global__ __launch_bounds__(kB) void kbi_spmv_v(const BiSys* __restrict__ Pp) {
     if (!bi_locate(P, row0, row1, slot)) return;
     if (P.st->stop) return;
     double rv = 0.0;
-    for (int i = row0 + threadIdx.x; i < row1; i += kB) {
-        const double vi = sell_row(P, i, P.y);
-        if (i < P.N) {
-            P.v[i] = vi;
-            rv += P.rhat[i] * vi;
-        }
-    }
+    sell_rows(P, row0, row1, P.y, [&](int i, double vi) {
+        P.v[i] = vi;
+        rv += P.rhat[i] * vi;
+    });
     double a[1] = {rv};
     write_slot<1>(a, P.pv, slot, sh);
 }
@@ -202,14 +218,11 @@ __global__ __launch_bounds__(kB) void kbi_spmv_t(const BiSys* __restrict__ Pp) {
     if (!bi_locate(P, row0, row1, slot)) return;
     if (P.st->stop) return;
     double ts = 0.0, tt = 0.0;
-    for (int i = row0 + threadIdx.x; i < row1; i += kB) {
-        const double ti = sell_row(P, i, P.z);
-        if (i < P.N) {
-            P.t[i] = ti;
-            ts += ti * P.s[i];
-            tt += ti * ti;
-        }
-    }
+    sell_rows(P, row0, row1, P.z, [&](int i, double ti) {
+        P.t[i] = ti;
+        ts += ti * P.s[i];
+        tt += ti * ti;
+    });
     double a[2] = {ts, tt};
     write_slot<2>(a, P.pt, slot, sh);
 }
