@@ -22,8 +22,12 @@ Jacobi-CG to 1e-10; the steps run twice and the second run is timed, the first
 reported beside it) and the pressure-Poisson case (128^3 jittered, permuted,
 FP64 CG vs FP32-inner/FP64-refined CG to 1e-8).
 
+Also reported by default: 512 batched heat problems (batched_heat) and the
+served request stream (served: solves/s, p50/p99 latency, against plain FP64
+per-request solves); --no-batched / --no-served skip them.
+
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 256] [--no-cpu]
-                       [--no-implicit] [--no-as-given]
+                       [--no-implicit] [--no-as-given] [--no-batched] [--no-served]
 """
 from __future__ import annotations
 
@@ -578,10 +582,15 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-implicit", action="store_true")
     ap.add_argument("--no-as-given", action="store_true")
-    ap.add_argument("--batched", action="store_true",
-                    help="also run the 512-problem batched heat workload")
+    ap.add_argument("--no-batched", dest="batched", action="store_false",
+                    help="skip the 512-problem batched heat workload")
+    ap.add_argument("--batched", dest="batched", action="store_true",
+                    help="run the batched heat workload (the default)")
     ap.add_argument("--batch-problems", type=int, default=512)
-    ap.add_argument("--served", action="store_true", help="also run the served request stream")
+    ap.add_argument("--no-served", dest="served", action="store_false",
+                    help="skip the served request stream")
+    ap.add_argument("--served", dest="served", action="store_true",
+                    help="run the served request stream (the default)")
     ap.add_argument("--requests", type=int, default=96)
     ap.add_argument("--served-chunk", type=int, default=16,
                     help="CG iterations per chunk of the served batch")
