@@ -24,10 +24,12 @@ def relmax(a, b):
 @pytest.fixture(scope="module")
 def meshes(ctx, fixtures_dir):
     import foamadapter_amd as fa
+    from skewed_meshes import refined_block
     polys = {
         "setup_stencil3D": fa.read_polymesh(fixtures_dir / "setup_stencil3D"),
         "setup_operator": fa.read_polymesh(fixtures_dir / "setup_operator"),
         "hex10_jit_perm": fa.PolyMesh.hex(10, jitter=0.2, jitter_seed=21, perm_seed=22),
+        "refined_block": refined_block(3),   # polyhedral, ten-entry rows
     }
     out = {}
     for k, p in polys.items():
@@ -47,6 +49,29 @@ def test_sparsity_matches_oracle(meshes):
         assert np.array_equal(sp.ownerOffset(), ref["ownerOffset"]), name
         assert np.array_equal(sp.neighbourOffset(), ref["neighbourOffset"]), name
         sp.close()
+
+
+def test_wide_rows_solve(ctx, meshes):
+    """Rows wider than the SpMV's group of eight entries loaded together (the
+    refined block's coarse cells have ten): FP64 CG, mixed CG and BiCGStab agree
+    with scipy's direct solve."""
+    import scipy.sparse.linalg as spla
+    from foamadapter_amd import dsl, la
+    p, m, g = meshes["refined_block"]
+    T, kap, T0, Tb = _heat(ctx, m, g, dt=1e-2, kappa=1.0)
+    ls = dsl.assemble(dsl.imp.ddt(T) - dsl.imp.laplacian(kap, T), T, 0.0, 1e-2)
+    A, b = ls.scipy(), ls.rhs()
+    assert np.diff(A.indptr).max() == 10
+    ref = spla.spsolve(A.tocsc(), b)
+    for solver in ({"relTol": 1e-12, "maxIter": 500},
+                   {"relTol": 1e-12, "maxIter": 500, "precision": "mixed"},
+                   la.mapFvSolution({"solver": "PBiCGStab", "preconditioner": "DIC",
+                                     "relTol": 1e-12, "maxIter": 500})):
+        x = ctx.upload(T0)
+        st = la.solve_cg(ls, x, solver)
+        assert st.finalResNorm <= 1e-12, (solver, st)
+        assert relmax(x.numpy(), ref) < 1e-10, (solver, relmax(x.numpy(), ref))
+        x.close()
 
 
 def _inputs(g, seed=3):

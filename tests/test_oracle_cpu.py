@@ -177,6 +177,35 @@ def test_affine_geometry_closed_form():
     assert np.max(np.abs(Tf[:len(g1["owner"])] - (g1["Cf"] @ a + c))) < 1e-13
 
 
+def test_refined_polyhedral_geometry():
+    """Non-conforming refinement: coarse cells whose side faces carry hanging
+    points ((m + 3)-gons, geometrically squares) and whose rows hold m*m + 1
+    entries.  Volumes, centres and the polygon faces' centres and areas are
+    exact; the sparsity rows are as wide as the cells' face counts say."""
+    from skewed_meshes import refined_block
+    m = 3
+    poly = refined_block(m)
+    g = orc.geometry(poly)
+    N = len(g["V"])
+    V = np.array([1.0] + [1.0 / (m * m)] * (m * m) + [1.0])
+    assert np.max(np.abs(g["V"] - V)) < 1e-15
+    C = [(0.5, 0.5, 0.5)] + [(1.5, (j + 0.5) / m, (k + 0.5) / m)
+                             for k in range(m) for j in range(m)] + [(2.5, 0.5, 0.5)]
+    assert np.max(np.abs(g["C"] - np.array(C))) < 1e-14    # coordinates up to 2.5
+    Cf, Sf = _all_faces(poly)
+    nv = np.diff(poly.face_offsets)
+    big = np.nonzero(nv == m + 3)[0]
+    assert len(big) == 8                               # four side faces per coarse cell
+    assert np.max(np.abs(np.linalg.norm(Sf[big], axis=1) - 1.0)) < 1e-15
+    for f in big:                                      # centre of the unit square it covers
+        xyz = poly.points[poly.face_points[poly.face_offsets[f]:poly.face_offsets[f + 1]]]
+        centre = 0.5 * (xyz.min(0) + xyz.max(0))
+        assert np.max(np.abs(Cf[f] - centre)) < 1e-15
+    sp = orc.sparsity(g)
+    rows = np.diff(sp["rowOffs"])
+    assert rows[0] == rows[N - 1] == m * m + 1 and rows.max() == m * m + 1
+
+
 def _shoelace_xy(poly_xy):
     """Area and centroid of a planar polygon from its (x, y) vertices (shoelace)."""
     x, y = poly_xy[:, 0], poly_xy[:, 1]
