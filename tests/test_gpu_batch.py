@@ -72,6 +72,44 @@ def test_batch_converges_and_matches_direct(ctx, systems, precision, graph):
 
 
 @pytest.mark.parametrize("precision", ["double", "mixed"])
+def test_batch_wide_rows(ctx, systems, precision):
+    """A polyhedral member with ten-entry rows (non-conforming refinement) in a
+    batch with a hex member: converges, matches the direct solve, and gives the
+    same bits alone and in the batch."""
+    import scipy.sparse.linalg as spla
+    import foamadapter_amd as fa
+    from foamadapter_amd import dsl, fvcc, la
+    from skewed_meshes import refined_block
+    m = fa.Mesh.from_poly(ctx, refined_block(3))
+    N, F, B = m.n_cells, m.n_internal_faces, m.n_boundary_faces
+    bc = fa.BoundaryConditions(m, ["fixedValue"] * m.n_patches)
+    Tb = orc.uniform(B, 300.0, 400.0, 7, 1)
+    T = fvcc.VolumeField(m, ctx.uniform(N, 300.0, 400.0, 7, 0), ctx.upload(Tb), bc, "T")
+    kap = fvcc.SurfaceField(m, ctx.upload(np.full(F + B, 1.0)))
+    ls = la.LinearSystem(la.SparsityPattern(m))
+    dsl.assemble(dsl.imp.ddt(T) - dsl.imp.laplacian(kap, T), T, 0.0, 1e-2, ls)
+    A, b = ls.scipy(), ls.rhs()
+    assert np.diff(A.indptr).max() == 10
+    ref = spla.spsolve(A.tocsc(), b)
+    x0 = T.numpy()
+    out = []
+    for members in ([ls], [systems[1]["ls"], ls]):
+        bt = la.Batch(ctx, precision, 64, 8)
+        xs = [ctx.upload(x0 if s is ls else systems[1]["x0"]) for s in members]
+        for s, x in zip(members, xs):
+            bt.add(s, x)
+        st = bt.solve({"relTol": 1e-10, "maxIter": 500}, use_graph=True)
+        out.append(xs[-1].numpy())
+        assert st[-1].finalResNorm <= 1e-10, st[-1]
+        bt.close()
+    assert np.max(np.abs(out[0] - ref)) / np.max(np.abs(ref)) < 1e-9
+    assert np.array_equal(out[0], out[1])
+    ls.close()
+    bc.close()
+    m.close()
+
+
+@pytest.mark.parametrize("precision", ["double", "mixed"])
 def test_alone_equals_in_mix(ctx, systems, precision):
     """A system's bits do not depend on the rest of the batch."""
     mix, _ = _run(ctx, systems, [0, 1, 2, 3, 4, 5], precision, True)
