@@ -604,9 +604,8 @@ def main():
                          "for hardware traffic counters)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py measures one MI355X: the finite-volume path runs on one GPU in this "
-              "build (how it would shard is described in DESIGN.md); run with --gpus 1",
-              file=sys.stderr)
+        print("bench.py measures one MI355X: the finite-volume path runs on a single GPU "
+              "(no cross-GPU partitioning is built); run with --gpus 1", file=sys.stderr)
         return 2
     import foamadapter_amd as fa
 
