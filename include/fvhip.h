@@ -332,8 +332,10 @@ typedef struct {
     double rel_tol;       /* stop when ||r||/||b|| <= rel_tol (criteria relative_residual_norm) */
     int32_t max_iter;     /* criteria iteration (default 1000, fvSolution.cpp:114-118) */
     int32_t mode;         /* fv_solve_mode */
-    double inner_rel_tol; /* FP32 inner CG tolerance per refinement step (F32_IR) */
-    int32_t max_outer;    /* refinement steps (F32_IR) */
+    double inner_rel_tol; /* F32_IR: FP32 residual drop since the last FP64 residual
+                             replacement that triggers the next one; 0: 1e-5 */
+    int32_t max_outer;    /* F32_IR: at most this many rounds (a chunk of FP32
+                             iterations and a replacement); 0: no limit */
     int32_t check_every;  /* host polls the stop flag every this many iterations */
     double abs_tol;       /* also stop when ||r|| <= abs_tol (criteria absolute_residual_norm,
                              fvSolution.cpp:139-144); 0: off */
@@ -419,7 +421,8 @@ int fv_batch_clear(fv_batch* b);
 int fv_batch_size(const fv_batch* b, int* n);
 int fv_batch_set_shape(fv_batch* b, int tile_rows, int chunk);
 /* stats: array of fv_batch_size entries (in insertion order) or NULL.
- * opts->inner_rel_tol: FP32 residual drop that ends a chunk early (mixed). */
+ * opts->inner_rel_tol: FP32 residual drop that ends a chunk early (mixed;
+ * 0: 1e-5). */
 int fv_batch_solve(fv_batch* b, const fv_solve_opts* opts, int use_graph,
                    fv_solver_stats* stats);
 /* Autotuner: times full batch solves for tile_rows in {256..4096} x chunk in
