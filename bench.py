@@ -492,7 +492,7 @@ def _uniform_host(n, lo, hi, seed, stream):
     return lo + (hi - lo) * v
 
 
-def bench_served(n_req=96, seed=1005, rate=200.0, tile_rows=1024, chunk=16):
+def bench_served(n_req=96, seed=1005, rate=200.0, tile_rows=4096, chunk=16):
     """Served request stream: Poisson arrivals, 70% heat (5 Euler steps, kappa ~
     logU[1e-4,1e-1]) / 30% pressure Poisson, on jittered permuted hex cubes with
     edge n ~ logU[16,160] (snapped to a catalogue of resident meshes), 200 GB
@@ -594,7 +594,7 @@ def main():
     ap.add_argument("--requests", type=int, default=96)
     ap.add_argument("--served-chunk", type=int, default=16,
                     help="CG iterations per chunk of the served batch")
-    ap.add_argument("--served-tile-rows", type=int, default=1024)
+    ap.add_argument("--served-tile-rows", type=int, default=4096)
     ap.add_argument("--rate", type=float, default=200.0,
                     help="Poisson arrival rate (requests/s) of the served stream (0: all at once); "
                          "the default offers more than the plain path sustains, so both paths "

@@ -479,7 +479,9 @@ int fv_server_create(int device, const fv_server_opts* opts, fv_server** out) {
         s = new fv_server();
         s->opts = *opts;
         check(fv_ctx_create(device, 0, &s->ctx));
-        const int tr = opts->tile_rows > 0 ? opts->tile_rows : 1024;
+        // measured on the served stream of jittered cubes, n ~ logU[16,160]:
+        // 4096-row tiles 114 solves/s, 2048 112, 1024 107, 8192 112
+        const int tr = opts->tile_rows > 0 ? opts->tile_rows : 4096;
         const int ch = opts->chunk > 0 ? opts->chunk : 16;
         check(fv_batch_create(s->ctx, opts->mode, tr, ch, &s->batch));
         s->batch->useGraph = opts->use_graph != 0;

@@ -451,7 +451,7 @@ typedef struct {
     int32_t mode;        /* FV_SOLVE_F32_IR or FV_SOLVE_F64 */
     int32_t batched;     /* 1: continuous batching, 0: one request at a time via fv_solve_cg */
     int32_t max_active;  /* systems in flight (batched), 0: no limit */
-    int32_t tile_rows;   /* batch tile rows (multiple of 64), 0: 1024 */
+    int32_t tile_rows;   /* batch tile rows (multiple of 64), 0: 4096 */
     int32_t chunk;       /* CG iterations per FP64 residual replacement, 0: 16 */
     int32_t use_graph;   /* replay chunks as hipGraphs */
 } fv_server_opts;
