@@ -113,6 +113,7 @@ _SIGS = {
     "fv_mesh_get": [_vp, C.c_int, _vp],
     "fv_mesh_set_gather": [_vp, C.c_int],
     "fv_mesh_get_gather": [_vp, C.POINTER(C.c_int)],
+    "fv_mesh_gather_stats": [_vp, C.c_int, _vp],
     "fv_bc_create": [_vp, _vp, _pp],
     "fv_bc_destroy": [_vp],
     "fv_interpolate": [_vp, _vp, C.c_int, C.c_int, _vp, _vp, _vp, _vp],

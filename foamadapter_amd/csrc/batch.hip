@@ -249,7 +249,7 @@ void start_members(fv_batch* b) {
         BMember* m = b->members[i];
         if (m->needStart) {
             member_tile_list<S>(b, i, slot, st);
-            prm[i] = CgStartParam{m->tol > 0.0 ? m->tol : b->tol, b->delta, 0.0,
+            prm[i] = CgStartParam{m->tol > 0.0 ? m->tol : b->tol, b->delta, b->atol,
                                   m->maxit > 0 ? m->maxit : b->maxit, 0};
             m->needStart = false;
         }
@@ -592,8 +592,10 @@ int fv_batch_autotune(fv_batch* b, const fv_solve_opts* o, const char* cache_pat
 int fv_batch_solve(fv_batch* b, const fv_solve_opts* o, int use_graph, fv_solver_stats* stats) {
     return guarded([&] {
         FV_REQUIRE(b && o, "NULL argument");
-        FV_REQUIRE(o->rel_tol >= 0.0 && o->max_iter >= 0, "bad solver options");
+        FV_REQUIRE(o->rel_tol >= 0.0 && o->max_iter >= 0 && o->abs_tol >= 0.0,
+                   "bad solver options");
         b->tol = o->rel_tol;
+        b->atol = o->abs_tol;
         b->maxit = o->max_iter;
         b->delta = b->prec == FV_SOLVE_F32_IR ? (o->inner_rel_tol > 0.0 ? o->inner_rel_tol : kIrDelta)
                                               : 0.0;
@@ -620,7 +622,11 @@ int fv_batch_solve(fv_batch* b, const fv_solve_opts* o, int use_graph, fv_solver
             m->stats.iters = hs[i].it;
             m->stats.outer_iters = chunks;
             if (stats) stats[i] = m->stats;
-            if (!(hs[i].res <= o->rel_tol)) ++bad;
+            // converged on the relative or (when set) the absolute residual, as
+            // fv_solve_cg judges it
+            if (!(hs[i].res <= o->rel_tol) &&
+                !(o->abs_tol > 0.0 && hs[i].res * hs[i].bnorm <= o->abs_tol))
+                ++bad;
         }
         FV_HIP(hipStreamSynchronize(b->ctx->stream));
         if (bad)

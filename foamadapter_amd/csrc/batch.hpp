@@ -28,6 +28,7 @@ struct fv_batch {
     int tileRows = 1024;         // rows per tile (multiple of 64)
     int chunk = 16;              // CG iterations per FP64 residual replacement
     double tol = 1e-8, delta = fvhip::kIrDelta;
+    double atol = 0.0;           // absolute residual target of fv_batch_solve (0: off)
     int maxit = 1000;
     int nextId = 0;
     std::vector<BMember*> members;

@@ -179,7 +179,7 @@ __global__ __launch_bounds__(kBlock) void k_laplacian(
 // LDS face slots of a CELLS-cell workgroup: 3 owned faces per hex cell plus slack;
 // faces beyond it (unusual meshes) are recomputed in phase 2
 template <int CELLS>
-constexpr int max_up() { return CELLS * 3 + CELLS / 4; }
+constexpr int max_up() { return gather_max_up(CELLS); }
 
 struct Split {
     const int32_t *own, *nbr, *ownStart, *lowOff, *lowFace, *lowOwn, *bOff, *bFace;
@@ -353,7 +353,7 @@ struct Lists {
     const uint16_t *extCell, *bndCell;
 };
 
-constexpr int kMaxLow = 8;  // lower faces per cell from inside the workgroup, in LDS
+constexpr int kMaxLow = kGatherMaxLow;  // lower faces per cell from inside the workgroup, in LDS
 
 // Sorts 8 keys ascending in registers (the 19-comparator network for 8 inputs:
 // layers (0,2)(1,3)(4,6)(5,7) / (0,4)(1,5)(2,6)(3,7) / (0,1)(2,3)(4,5)(6,7) /
@@ -391,8 +391,8 @@ __global__ __launch_bounds__(THREADS) void k_gather(int N, int nBlocks, Split t,
     constexpr int NC = Op::NC;
     constexpr int kUL = Op::kFacesPerPass;
     constexpr int kMaxUp = max_up<CELLS>();
-    constexpr int kMaxExt = CELLS / 2;
-    constexpr int kMaxBnd = CELLS / 4;
+    constexpr int kMaxExt = gather_max_ext(CELLS);
+    constexpr int kMaxBnd = gather_max_bnd(CELLS);
     __shared__ S fluxU[NC][kMaxUp];
     __shared__ S fluxE[NC][kMaxExt];
     __shared__ S fluxB[NC][kMaxBnd];

@@ -299,18 +299,24 @@ const GatherLists& gather_lists(fv_mesh* m, int cells, bool f32) {
     m->lowOwn.download(lowOwn.data());
     m->bOff.download(bOff.data());
     m->bFace.download(bFace.data());
+    std::vector<int32_t> ownStart(N + 1);
+    m->ownStart.download(ownStart.data());
     const int64_t nBlocks = (N + cells - 1) / cells;
     std::vector<int32_t> eo(nBlocks + 1, 0), ef, ew, bo(nBlocks + 1, 0), bf;
     std::vector<uint16_t> ec, bc;
+    int64_t nOverflow = 0;
     for (int64_t blk = 0; blk < nBlocks; ++blk) {
         const int64_t c0 = blk * cells, c1 = std::min<int64_t>(c0 + cells, N);
+        int innerMax = 0;
         for (int64_t c = c0; c < c1; ++c) {
             // a cell's lower faces ascend; those owned before the block come first
-            for (int32_t k = lowOff[c]; k < lowOff[c + 1] && lowOwn[k] < c0; ++k) {
+            int32_t k = lowOff[c];
+            for (; k < lowOff[c + 1] && lowOwn[k] < c0; ++k) {
                 ef.push_back(lowFace[k]);
                 ew.push_back(lowOwn[k]);
                 ec.push_back((uint16_t)(c - c0));
             }
+            innerMax = std::max(innerMax, lowOff[c + 1] - k);
             for (int32_t k = bOff[c]; k < bOff[c + 1]; ++k) {
                 bf.push_back(bFace[k]);
                 bc.push_back((uint16_t)(c - c0));
@@ -318,7 +324,14 @@ const GatherLists& gather_lists(fv_mesh* m, int cells, bool f32) {
         }
         eo[blk + 1] = (int32_t)ef.size();
         bo[blk + 1] = (int32_t)bf.size();
+        const bool fits = ownStart[c1] - ownStart[c0] <= gather_max_up(cells) &&
+                          eo[blk + 1] - eo[blk] <= gather_max_ext(cells) &&
+                          bo[blk + 1] - bo[blk] <= gather_max_bnd(cells) &&
+                          innerMax <= kGatherMaxLow;
+        nOverflow += fits ? 0 : 1;
     }
+    L.nBlocks = nBlocks;
+    L.nOverflow = nOverflow;
     fv_ctx* ctx = m->ctx;
     L.extOff.upload(ctx, eo.data(), eo.size());
     L.extFace.upload(ctx, ef.data(), ef.size());
@@ -578,6 +591,17 @@ int fv_mesh_get_gather(const fv_mesh* m, int* mode) {
     return guarded([&] {
         FV_REQUIRE(m && mode, "NULL argument");
         *mode = gather_cells(m) == 0 ? 1 : gather_cells(m) == 256 ? 2 : gather_cells(m) == 512 ? 3 : 4;
+    });
+}
+
+int fv_mesh_gather_stats(fv_mesh* m, int cells, int64_t* counts2) {
+    return guarded([&] {
+        FV_REQUIRE(m && counts2, "NULL argument");
+        FV_REQUIRE(cells == 256 || cells == 512 || cells == 1024, "cells must be 256, 512 or 1024");
+        FV_REQUIRE(m->upperTri, "the staged gather needs upper-triangular face order");
+        const GatherLists& L = gather_lists(m, cells);
+        counts2[0] = L.nBlocks;
+        counts2[1] = L.nOverflow;
     });
 }
 

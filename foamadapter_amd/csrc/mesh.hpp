@@ -39,7 +39,19 @@ struct GatherLists {
     fvhip::DBuf<double> exW, exMagSf, exNodc, exSf;
     fvhip::DBuf<float> exW32, exMagSf32, exNodc32, exSf32;
     bool ready32 = false;
+    // workgroups whose faces do not fit the kernel's LDS lists (more owned,
+    // external lower or boundary faces than it holds, or a cell with more
+    // than kGatherMaxLow lower faces from inside the workgroup): they take the
+    // per-cell path, same bits, slower
+    int64_t nBlocks = 0, nOverflow = 0;
 };
+
+// LDS capacities of the staged gather for a CELLS-cell workgroup (faces owned,
+// external lower faces, boundary faces, and inner lower faces per cell)
+constexpr int gather_max_up(int cells) { return cells * 3 + cells / 4; }
+constexpr int gather_max_ext(int cells) { return cells / 2; }
+constexpr int gather_max_bnd(int cells) { return cells / 4; }
+constexpr int kGatherMaxLow = 8;
 
 struct fv_mesh {
     fv_ctx* ctx = nullptr;

@@ -17,6 +17,7 @@
 #include "batch.hpp"
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <condition_variable>
@@ -51,7 +52,8 @@ struct Req {
     fv_request d{};
     std::vector<double> wall, T0;  // owned copies of the caller's arrays
     int state = FV_REQ_QUEUED;
-    bool cancel = false;
+    std::atomic<bool> cancel{false};  // set by fv_server_cancel, read by the worker
+    bool waited = false;              // a fv_server_wait holds the request
     int status = FV_OK;
     Clock::time_point tSubmit, tStart, tDone;
     size_t estimate = 0;
@@ -610,6 +612,9 @@ int fv_server_wait(fv_server* s, int64_t id, double* h_result, fv_request_stats*
         auto it = s->reqs.find(id);
         FV_REQUIRE(it != s->reqs.end(), "unknown request id");
         Req* r = it->second;
+        // one waiter per request: the waiter frees it
+        FV_REQUIRE(!r->waited, "request is already being waited on");
+        r->waited = true;
         s->qcv.wait(lk, [&] { return r->state == FV_REQ_DONE; });
         if (stats) {
             stats->status = r->status;

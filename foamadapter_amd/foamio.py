@@ -11,6 +11,7 @@ with 17 significant digits, so a write/read round trip is exact in FP64.
 """
 from __future__ import annotations
 
+import re
 from pathlib import Path
 
 import numpy as np
@@ -88,20 +89,50 @@ def write_field(case_dir, time: str, name: str, values, patches: list[Patch],
     return path
 
 
+def _subdicts(text: str, where) -> dict:
+    """{keyword: body} of the top-level `keyword { body }` entries of a dictionary
+    body, bracketed by matching braces (keywords compared exactly, quotes
+    removed), so a patch whose name is part of another's is never confused with
+    it.  Entries that are not sub-dictionaries are skipped."""
+    out, depth, start, key = {}, 0, 0, None
+    tok = re.compile(r'[{};]|"[^"]*"|[^\s{};"]+')
+    last = None
+    for m in tok.finditer(text):
+        t = m.group(0)
+        if t == "{":
+            if depth == 0:
+                if last is None:
+                    raise ValueError(f"sub-dictionary without a keyword in {where}")
+                key, start = last.strip('"'), m.end()
+            depth += 1
+        elif t == "}":
+            depth -= 1
+            if depth < 0:
+                raise ValueError(f"unbalanced braces in {where}")
+            if depth == 0:
+                out[key] = text[start:m.start()]
+            last = None
+        elif depth == 0:
+            last = None if t == ";" else t
+    if depth:
+        raise ValueError(f"unbalanced braces in {where}")
+    return out
+
+
 def read_field(path, patches: list[Patch], ncomp: int = 1):
     """(internal [N] or [N, 3], {patch name: boundary values}) of an ASCII field file."""
     text = _strip_header(Path(path).read_text())
     internal = _field_values(text, "internalField", None, ncomp)
-    bf = text[text.index("boundaryField"):]
+    top = _subdicts(text[text.index("boundaryField"):], path)
+    if "boundaryField" not in top:
+        raise ValueError(f"no boundaryField in {path}")
+    bf = _subdicts(top["boundaryField"], path)
     out = {}
     for p in patches:
         if p.empty:
             continue
-        i = bf.find(p.name)
-        if i < 0:
+        if p.name not in bf:
             raise ValueError(f"patch {p.name} missing in {path}")
-        seg = bf[i:]
-        seg = seg[:seg.index("}") + 1] if "nonuniform" not in seg.split("}")[0] else seg
-        vals = _field_values(seg, "value", p.size, ncomp)
+        vals = _field_values(bf[p.name], "value", p.size, ncomp)
         out[p.name] = None if vals is None else vals.reshape(-1, ncomp) if ncomp > 1 else vals
     return (internal.reshape(-1, ncomp) if ncomp > 1 else internal), out

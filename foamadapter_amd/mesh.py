@@ -309,6 +309,13 @@ class Mesh:
         call("fv_mesh_get_gather", self.handle, C.byref(m))
         return {1: "stencil-walk", 2: "staged-256", 3: "staged", 4: "staged-1024"}[m.value]
 
+    def gather_stats(self, cells: int) -> tuple[int, int]:
+        """(workgroups, workgroups taking the per-cell path) of the staged gather
+        with `cells` cells per workgroup on this mesh."""
+        c = np.zeros(2, np.int64)
+        call("fv_mesh_gather_stats", self.handle, int(cells), _ptr(c))
+        return int(c[0]), int(c[1])
+
     def to_device(self, src, kind: str = "cell", oriented: bool = False):
         """Caller-numbered device array -> this mesh's numbering (identity if not renumbered).
         kind: 'cell' ([N,...]) or 'face' ([F+B], oriented=True for fluxes)."""

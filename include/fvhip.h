@@ -213,6 +213,11 @@ int fv_mesh_get(const fv_mesh* mesh, fv_geom which, void* h_out);
  * counterpart in the reference: a tuning knob. */
 int fv_mesh_set_gather(fv_mesh* mesh, int mode);
 int fv_mesh_get_gather(const fv_mesh* mesh, int* mode);
+/* How the staged gather with `cells` cells per workgroup (256, 512 or 1024)
+ * runs on this mesh (its lists are built if needed): h_counts2 = {workgroups,
+ * workgroups whose faces exceed its LDS lists and take the per-cell path --
+ * same results, slower}.  No counterpart in the reference: a diagnostic. */
+int fv_mesh_gather_stats(fv_mesh* mesh, int cells, int64_t* h_counts2);
 
 /* Boundary conditions of a field: one kind per patch
  * (fvcc::VolumeBoundary list built by readVolBoundaryConditions, readers.hpp:31-108). */
@@ -422,7 +427,9 @@ int fv_batch_size(const fv_batch* b, int* n);
 int fv_batch_set_shape(fv_batch* b, int tile_rows, int chunk);
 /* stats: array of fv_batch_size entries (in insertion order) or NULL.
  * opts->inner_rel_tol: FP32 residual drop that ends a chunk early (mixed;
- * 0: 1e-5). */
+ * 0: 1e-5).  A member has converged when its FP64 relative residual is at most
+ * opts->rel_tol or, when opts->abs_tol > 0, its absolute residual at most
+ * opts->abs_tol (as fv_solve_cg). */
 int fv_batch_solve(fv_batch* b, const fv_solve_opts* opts, int use_graph,
                    fv_solver_stats* stats);
 /* Autotuner: times full batch solves for tile_rows in {256..4096} x chunk in
@@ -503,11 +510,15 @@ int fv_server_request_bytes(fv_server* s, int mesh_id, int kind, size_t* bytes);
 int fv_server_submit(fv_server* s, const fv_request* req, int64_t* req_id);
 int fv_server_poll(fv_server* s, int64_t req_id, int* state);
 /* Waits for a request, copies its result field [N] (caller numbering; may be
- * NULL) and statistics, and releases it.  Returns the request's status. */
+ * NULL) and statistics, and releases it.  Returns the request's status.  One
+ * waiter per request: a second wait on a request that is already waited on
+ * returns FV_EINVAL (and a wait on a released request "unknown request id"). */
 int fv_server_wait(fv_server* s, int64_t req_id, double* h_result, fv_request_stats* stats);
 /* Cancels a queued or running request (its wait returns FV_ECANCELLED). */
 int fv_server_cancel(fv_server* s, int64_t req_id);
 int fv_server_get_stats(fv_server* s, fv_server_stats* out);
+/* Stops the worker and frees every request; no other call on the server may
+ * run concurrently with (or after) it. */
 int fv_server_destroy(fv_server* s);
 
 #ifdef __cplusplus

@@ -14,10 +14,16 @@ AFFINE_B = np.array([0.2, -0.1, 0.05])
 TAPER_ALPHA = 0.5
 
 
-def affine_hex(n, perm_seed=0):
+# a strong shear in the x-y plane: the x- and y-faces meet their cell deltas at
+# about 88 degrees (cos = 1/sqrt(1 + 30^2) < 0.05), so their non-orthogonal delta
+# coefficients take the 0.05|d| clamp; the z-faces stay orthogonal
+SHEAR_A = np.array([[1.0, 30.0, 0.0], [0.0, 1.0, 0.0], [0.0, 0.0, 1.0]])
+
+
+def affine_hex(n, perm_seed=0, A=AFFINE_A, b=AFFINE_B):
     """(uniform mesh, its image under x -> A x + b): same topology and numbering."""
     poly0 = PolyMesh.hex(n, perm_seed=perm_seed)
-    return poly0, dataclasses.replace(poly0, points=poly0.points @ AFFINE_A.T + AFFINE_B)
+    return poly0, dataclasses.replace(poly0, points=poly0.points @ A.T + b)
 
 
 def tapered_hex(n, alpha=TAPER_ALPHA, perm_seed=0):

@@ -27,6 +27,29 @@ def test_scalar_and_vector_round_trip(tmp_path):
     assert "volVectorField" in pu.read_text() and "calculated" in pu.read_text()
 
 
+def test_patch_names_that_contain_each_other(tmp_path):
+    """Each boundaryField entry is found by its exact name: 'inlet' after
+    'inletWall', and 'wall' after 'inletWall', read their own values."""
+    import dataclasses
+    from foamadapter_amd.mesh import Patch
+    poly = PolyMesh.hex(2)
+    names = ["inletWall", "inlet", "wall", "outlet_wall", "outlet", "x"]
+    patches = [dataclasses.replace(p, name=n) for p, n in zip(poly.patches, names)]
+    assert all(isinstance(p, Patch) for p in patches)
+    B = sum(p.size for p in patches)
+    Tb = np.arange(B, dtype=np.float64) + 0.25
+    p = write_field(tmp_path, "0", "T", np.zeros(poly.n_cells), patches, Tb)
+    _, b2 = read_field(p, patches)
+    off = 0
+    for q in patches:
+        assert np.array_equal(b2[q.name], Tb[off:off + q.size]), q.name
+        off += q.size
+    # uniform values and quoted names are read as well
+    p.write_text(p.read_text().replace("    inlet\n", '    "inlet"\n'))
+    _, b3 = read_field(p, patches)
+    assert np.array_equal(b3["inlet"], b2["inlet"])
+
+
 def test_empty_patches_and_reference_file(tmp_path, fixtures_dir):
     case = fixtures_dir / "setup_operator"
     poly = read_polymesh(case)

@@ -236,3 +236,27 @@ def test_autotune_picks_and_caches(ctx, systems, tmp_path):
     assert all(st.finalResNorm <= 1e-8 for st in stats)
     b.close()
     b2.close()
+
+
+@pytest.mark.parametrize("precision", ["double", "mixed"])
+def test_batch_absolute_tolerance(ctx, systems, precision):
+    """An fvSolution 'relTol 0; tolerance X' dictionary (absolute_residual_norm,
+    src/compatibility/fvSolution.cpp:139-144) stops each member of a batch on
+    ||b - A x|| <= X, as fv_solve_cg does, and returns OK."""
+    from foamadapter_amd import la
+    idx = [0, 1, 3]
+    b = la.Batch(ctx, precision, 256, 8)
+    xs = []
+    for i in idx:
+        x = ctx.upload(systems[i]["x0"])
+        b.add(systems[i]["ls"], x)
+        xs.append(x)
+    bn = [np.linalg.norm(systems[i]["ls"].rhs()) for i in idx]
+    atol = 1e-9 * min(bn)
+    stats = b.solve({"relTol": 0.0, "tolerance": atol, "maxIter": 3000})
+    b.close()
+    for i, x, st, nb in zip(idx, xs, stats, bn):
+        ls = systems[i]["ls"]
+        r = np.linalg.norm(ls.rhs() - ls.scipy() @ x.numpy())
+        assert st.status == 0 and st.finalResNorm * nb <= atol * 1.0001, (i, st)
+        assert r <= atol * 1.01, (i, r, atol)

@@ -23,7 +23,7 @@ def relmax(a, b):
 
 def _meshes(fixtures_dir):
     from foamadapter_amd.mesh import PolyMesh, read_polymesh
-    from skewed_meshes import affine_hex, refined_block, tapered_hex
+    from skewed_meshes import SHEAR_A, affine_hex, refined_block, tapered_hex
     return {
         "setup_operator": read_polymesh(fixtures_dir / "setup_operator"),
         "setup_stencil3D": read_polymesh(fixtures_dir / "setup_stencil3D"),
@@ -35,6 +35,8 @@ def _meshes(fixtures_dir):
         "hex40_jit_perm": PolyMesh.hex(40, jitter=0.2, jitter_seed=9, perm_seed=10),
         # skewed planar faces with closed-form geometry (pinned in test_oracle_cpu)
         "hex6_affine_perm": affine_hex(6, perm_seed=3)[1],
+        # x- and y-faces at ~88 degrees to their deltas: clamped nonOrthDeltaCoeffs
+        "hex6_shear_perm": affine_hex(6, perm_seed=3, A=SHEAR_A)[1],
         "hex4_tapered": tapered_hex(4)[1],
         # polyhedral cells: hanging points, ten-entry rows, a cell with nine lower
         # faces in one workgroup (more than the staged gather's list holds)

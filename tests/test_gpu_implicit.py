@@ -134,12 +134,16 @@ def _heat(ctx, m, g, dt=1e-3, kappa=1e-2, seed=1001):
     return T, kap, T0, Tb
 
 
-def test_heat_transfer_steps_match_oracle(ctx):
+@pytest.mark.parametrize("n,precision", [(16, "double"), (32, "double"), (32, "mixed")])
+def test_heat_transfer_steps_match_oracle(ctx, n, precision):
     """Implicit heat equation ddt(T) - laplacian(kappa,T), Euler, Jacobi-CG to 1e-10
-    (examples/heatTransfer/heatTransfer.cpp:60-86) on a 16^3 cube, 10 steps."""
+    (examples/heatTransfer/heatTransfer.cpp:60-86), 10 steps, on a 16^3 cube and
+    at the heat-transfer case's own size (32^3), in FP64 and with the FP32 CG
+    inside FP64 refinement."""
     import foamadapter_amd as fa
     from foamadapter_amd import dsl
-    poly = fa.PolyMesh.hex(16)
+    print(f"[case] heat {n}^3 {precision}", flush=True)
+    poly = fa.PolyMesh.hex(n)
     m = fa.Mesh.from_poly(ctx, poly)
     g = orc.geometry(poly)
     T, kap, T0, Tb = _heat(ctx, m, g)
@@ -151,7 +155,7 @@ def test_heat_transfer_steps_match_oracle(ctx):
     for step in range(10):
         before = T.numpy()
         st = dsl.solve(dsl.imp.ddt(T) - dsl.imp.laplacian(kap, T), T, step * dt, dt,
-                       solver={"relTol": 1e-10, "maxIter": 1000})
+                       solver={"relTol": 1e-10, "maxIter": 1000, "precision": precision})
         # energy: heat gained by the cells = net flux through the walls (to the
         # solver tolerance; internal fluxes cancel)
         after = T.numpy()
@@ -162,7 +166,9 @@ def test_heat_transfer_steps_match_oracle(ctx):
         ls.ddt(Tref, dt)
         ls.laplacian(gam, Tb, [0] * 6, coeff=-1.0)
         Tref, ost = ls.cg(Tref, 1e-10, 1000)
-        assert st.finalResNorm <= 1e-10 and abs(st.numIter - ost["iters"]) <= 2, (st, ost)
+        assert st.finalResNorm <= 1e-10, (st, ost)
+        if precision == "double":                 # same recurrence as the oracle's CG
+            assert abs(st.numIter - ost["iters"]) <= 2, (st, ost)
     got = T.numpy()
     assert relmax(got, Tref) < 1e-9
     assert got.min() >= 300.0 - 1e-9 and got.max() <= 400.0 + 1e-9   # discrete maximum principle

@@ -177,6 +177,104 @@ def test_affine_geometry_closed_form():
     assert np.max(np.abs(Tf[:len(g1["owner"])] - (g1["Cf"] @ a + c))) < 1e-13
 
 
+def _affine_delta_coeff_errors(g0, g1, A):
+    """Largest relative deviation of each non-orthogonal coefficient of the image
+    g1 = A g0 (+ b) of a uniform mesh g0 from its closed form.
+
+    On the uniform mesh a face's unit normal n0 is a coordinate axis and the
+    owner->neighbour delta d0 (and the owner->face-centre vector of a boundary
+    face, d0/2) is parallel to it; both are exact on a dyadic spacing.  Under x
+    -> A x + b: d = A d0 and the unit normal is n = A^-T n0 / |A^-T n0|.  OpenFOAM's
+    definitions (surfaceInterpolation::makeNonOrthDeltaCoeffs, fvPatch::delta,
+    fvPatch::nf) then give
+      internal  nonOrthDeltaCoeffs = 1 / max(n.d, 0.05 |d|)
+      boundary  nf = n,  delta = n (n.d),  deltaCoeffs = 1/|delta|,
+                nonOrthDeltaCoeffs = 1 / max(n.delta, 0.05 |delta|)."""
+    AinvT = np.linalg.inv(A).T
+
+    def unit(v):
+        return v / np.linalg.norm(v, axis=1, keepdims=True)
+
+    def rel(a, b):
+        return float(np.max(np.abs(a - b) / np.maximum(np.abs(b), 1e-300)))
+
+    d0 = g0["C"][g0["neighbour"]] - g0["C"][g0["owner"]]
+    n = unit(unit(g0["Sf"]) @ AinvT.T)
+    d = d0 @ A.T
+    nd = np.einsum("ij,ij->i", n, d)
+    nodc = 1.0 / np.maximum(nd, 0.05 * np.linalg.norm(d, axis=1))
+    db0 = g0["Cf_b"] - g0["C"][g0["faceCells"]]
+    nb = unit(unit(g0["Sf_b"]) @ AinvT.T)
+    db = db0 @ A.T
+    delta = nb * np.einsum("ij,ij->i", nb, db)[:, None]
+    mag = np.linalg.norm(delta, axis=1)
+    nodcb = 1.0 / np.maximum(np.einsum("ij,ij->i", nb, delta), 0.05 * mag)
+    err = {
+        "nonOrthDeltaCoeffs": rel(g1["nonOrthDeltaCoeffs"], nodc),
+        "nf_b": float(np.max(np.abs(g1["nf_b"] - nb))),
+        "delta_b": float(np.max(np.abs(g1["delta_b"] - delta)) / np.max(mag)),
+        "deltaCoeffs_b": rel(g1["deltaCoeffs_b"], 1.0 / mag),
+        "nonOrthDeltaCoeffs_b": rel(g1["nonOrthDeltaCoeffs_b"], nodcb),
+    }
+    clamped = int(np.count_nonzero(nd < 0.05 * np.linalg.norm(d, axis=1)))
+    return err, clamped
+
+
+@pytest.mark.parametrize("shape", ["affine", "shear"])
+def test_affine_nonorth_delta_coeffs_closed_form(shape):
+    """Non-orthogonal and boundary delta coefficients on skewed planar meshes in
+    closed form (the quantities test/test_unstructuredMesh.cpp:142-158 compares
+    with OpenFOAM: boundary nf, delta, deltaCoeffs).  The general affine mesh
+    has no clamped face; the sheared one clamps its x- and y-faces (n.d <
+    0.05|d|) and keeps its z-faces orthogonal."""
+    from skewed_meshes import AFFINE_A, SHEAR_A, affine_hex
+    A = AFFINE_A if shape == "affine" else SHEAR_A
+    poly0, poly1 = affine_hex(6, perm_seed=3, A=A)
+    g0, g1 = orc.geometry(poly0), orc.geometry(poly1)
+    err, clamped = _affine_delta_coeff_errors(g0, g1, A)
+    # round-off of coordinates up to 30 (shear) against coefficients ~1/h
+    assert max(err.values()) < 1e-12, err
+    F = len(g1["owner"])
+    if shape == "affine":
+        assert clamped == 0
+        # non-orthogonal: nonOrthDeltaCoeffs differs from 1/|d| on most faces
+        assert np.mean(np.abs(g1["nonOrthDeltaCoeffs"] * g1["deltaCoeffs"] ** -1 - 1) > 1e-3) > 0.5
+    else:
+        assert clamped == 2 * F // 3                    # the x- and y-faces
+        d = np.linalg.norm(g1["C"][g1["neighbour"]] - g1["C"][g1["owner"]], axis=1)
+        hit = np.isclose(g1["nonOrthDeltaCoeffs"], 1.0 / (0.05 * d), rtol=1e-14, atol=0)
+        assert np.count_nonzero(hit) == clamped
+
+
+@pytest.mark.parametrize("mistake", ["nodc_is_inverse_distance", "nodc_unclamped",
+                                     "delta_b_not_projected", "nf_b_not_normalised"])
+def test_nonorth_closed_form_rejects_planted_mistakes(mistake):
+    """Each plausible restatement mistake, applied to the oracle's output on the
+    sheared mesh, fails the closed-form check above."""
+    from skewed_meshes import SHEAR_A, affine_hex
+    poly0, poly1 = affine_hex(6, perm_seed=3, A=SHEAR_A)
+    g0, g1 = orc.geometry(poly0), orc.geometry(poly1)
+    bad = {k: v.copy() for k, v in g1.items()}
+    own, nbr, fc = g1["owner"], g1["neighbour"], g1["faceCells"]
+    d = g1["C"][nbr] - g1["C"][own]
+    n = g1["Sf"] / g1["magSf"][:, None]
+    if mistake == "nodc_is_inverse_distance":        # nodc = 1/|d|
+        bad["nonOrthDeltaCoeffs"] = 1.0 / np.linalg.norm(d, axis=1)
+    elif mistake == "nodc_unclamped":                 # nodc = 1/(n.d) without the clamp
+        bad["nonOrthDeltaCoeffs"] = 1.0 / np.einsum("ij,ij->i", n, d)
+    elif mistake == "delta_b_not_projected":          # delta_b = Cf - C_P
+        db = g1["Cf_b"] - g1["C"][fc]
+        mag = np.linalg.norm(db, axis=1)
+        bad["delta_b"] = db
+        bad["deltaCoeffs_b"] = 1.0 / mag
+        bad["nonOrthDeltaCoeffs_b"] = 1.0 / np.maximum(np.einsum("ij,ij->i", g1["nf_b"], db),
+                                                       0.05 * mag)
+    else:                                             # nf_b = Sf_b (not unit)
+        bad["nf_b"] = g1["Sf_b"].copy()
+    err, _ = _affine_delta_coeff_errors(g0, bad, SHEAR_A)
+    assert max(err.values()) > 1e-3, (mistake, err)
+
+
 def test_refined_polyhedral_geometry():
     """Non-conforming refinement: coarse cells whose side faces carry hanging
     points ((m + 3)-gons, geometrically squares) and whose rows hold m*m + 1
