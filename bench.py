@@ -88,6 +88,33 @@ def measured_traffic(kernel, n):
     return None
 
 
+def stream_ceiling(ctx, n_doubles=1 << 28, runs=10):
+    """The hand-written streaming kernels (fv_stream: copy a = b, triad a = b + s c,
+    16-byte loads and stores) on 2 GiB arrays: the HBM rate a kernel that only
+    streams reaches on this part, the practical ceiling of the face operators."""
+    from foamadapter_amd import _lib
+    a, b, c = (ctx.empty((n_doubles,)) for _ in range(3))
+    b.fill(1.0)
+    c.fill(2.0)
+    out = {}
+    for mode, name, moved in ((0, "copy", 16), (1, "triad", 24)):
+        for _ in range(3):
+            _lib.call("fv_stream", ctx.handle, mode, n_doubles, b.ptr, c.ptr, a.ptr, 0.5)
+        ctx.sync()
+        ctx.record(56)
+        for _ in range(runs):
+            _lib.call("fv_stream", ctx.handle, mode, n_doubles, b.ptr, c.ptr, a.ptr, 0.5)
+        ctx.record(57)
+        ms = ctx.elapsed_ms(56, 57) / runs
+        gbs = moved * n_doubles / 2 / (ms * 1e-3) / 1e9
+        out[name] = {"ms": round(ms, 4), "GB/s": round(gbs, 1),
+                     "frac_peak": round(gbs / HBM_PEAK_GBS, 4)}
+    for x in (a, b, c):
+        x.close()
+    out["bytes"] = f"{8 * n_doubles >> 30} GiB per array; copy moves 2, triad 3 arrays"
+    return out
+
+
 def alg_bytes(op, prec, N, F, B):
     f, c, b = BYTES[(op, prec)]
     return f * F + c * N + b * B
@@ -200,12 +227,16 @@ def time_sweep(ctx, case, steps, warmup):
             "per_op": per_op, "dominant": (dom, dom_ms, dom_bytes), "ops": ops}
 
 
-def cpu_baseline(case, budget_s=24.0):
+def cpu_baseline(case, sweep, budget_s=24.0):
     """The CPU restatement on the same mesh and fields, FP64 sweep of the 6
     operators: the multi-core form (OpenMP cell walks in ascending face order,
     the role of NeoN's OpenMP executor; OMP_NUM_THREADS threads) and the serial
     form (1 thread, NeoN's serial executor), each op repeated until about
-    budget_s/12 seconds have passed.  Both return the serial bits."""
+    budget_s/12 seconds have passed.  Both return the serial bits.
+
+    The same outputs check the GPU at full size: every operator of the sweep is
+    run once more on the GPU and compared with the multi-core restatement (FP64)
+    and the serial FP32 restatement (FP32), bit for bit."""
     sys.path.insert(0, str(ROOT / "tests"))
     import oracle_cpu as orc
 
@@ -234,11 +265,13 @@ def cpu_baseline(case, budget_s=24.0):
             "sngrad": lambda: orc.sngrad(g, Th, Tbh, kinds),
         },
     }
-    res = {}
+    res, ref64 = {}, {}
     for form, calls in forms.items():
         tot_b, tot_t, runs = 0.0, 0.0, {}
         for op, fn in calls.items():
-            fn()                                   # warm (first touch, stencil in cache)
+            out = fn()                             # warm (first touch, stencil in cache)
+            if form == "multicore":
+                ref64[op] = out
             n, t = 0, 0.0
             while t < budget_s / 2 / len(calls) or n == 0:
                 t0 = time.perf_counter()
@@ -250,6 +283,38 @@ def cpu_baseline(case, budget_s=24.0):
             runs[op] = n
         res[form] = (round(tot_b / tot_t / 1e9, 3), runs, tot_t)
     omp.close()
+    # full-size parity of the GPU sweep (the checker: the oracle outputs above)
+    log("comparing the 256^3 GPU sweep with the oracle (FP64 multi-core, FP32 serial)")
+    ops = sweep["ops"]
+    T32, phi32, gam32 = case.fields["f32"]
+    t32, tb32, p32, g32 = (T32.internal.numpy(), T32.boundary.numpy(), phi32.numpy(),
+                           gam32.numpy())
+    ref32 = {
+        "interp_linear": lambda: orc.interpolate(g, t32, tb32, 0, None, np.float32),
+        "grad": lambda: orc.grad(g, t32, tb32, np.float32),
+        "div_linear": lambda: orc.div(g, p32, t32, tb32, 0, np.float32),
+        "div_upwind": lambda: orc.div(g, p32, t32, tb32, 1, np.float32),
+        "laplacian": lambda: orc.laplacian(g, g32, t32, tb32, kinds, None, np.float32),
+        "sngrad": lambda: orc.sngrad(g, t32, tb32, kinds, None, np.float32),
+    }
+    parity = {}
+    for prec in ("f64", "f32"):
+        same, worst = [], 0.0
+        for op in OPS:
+            fn, out = ops[(op, prec)]
+            fn()
+            got = out.numpy()
+            want = ref64[op] if prec == "f64" else ref32[op]()
+            want = np.asarray(want).reshape(got.shape)
+            if np.array_equal(got, want):
+                same.append(op)
+            else:
+                d = float(np.max(np.abs(got.astype(np.float64) - want)) /
+                          max(float(np.max(np.abs(want))), 1e-300))
+                worst = max(worst, d)
+        parity[prec] = {"ops": len(OPS), "bitwise": len(same),
+                        "max_rel_diff": worst,
+                        "against": "oracle multi-core FP64" if prec == "f64" else "oracle serial FP32"}
     threads = orc.OmpOps.threads()
     v, runs, t = res["multicore"]
     sv, sruns, st = res["serial"]
@@ -259,7 +324,7 @@ def cpu_baseline(case, budget_s=24.0):
                       f"on the same {N}-cell renumbered mesh and fields (results bitwise the "
                       f"serial oracle's), runs per op {runs}, {t:.1f}s",
             "serial": {"value": sv, "unit": "GB/s", "cores": 1,
-                       "sample": f"serial oracle, runs per op {sruns}, {st:.1f}s"}}
+                       "sample": f"serial oracle, runs per op {sruns}, {st:.1f}s"}}, parity
 
 
 def bench_heat(ctx, n=32, steps=10):
@@ -673,7 +738,7 @@ def main():
     del poly
     if not args.no_cpu:
         log("timing the CPU baseline (oracle, 1 thread)")
-        rec["cpu_baseline"] = cpu_baseline(case)
+        rec["cpu_baseline"], rec["parity_full_size"] = cpu_baseline(case, sweep)
     case.mesh.close()
     if not args.no_implicit:
         log("implicit heat transfer 32^3")

@@ -47,10 +47,14 @@ struct GatherLists {
 };
 
 // LDS capacities of the staged gather for a CELLS-cell workgroup (faces owned,
-// external lower faces, boundary faces, and inner lower faces per cell)
+// external lower faces, boundary faces, and inner lower faces per cell).  A
+// Z-order brick of 8x8x4 / 8x8x8 / 16x8x8 cells has 128 / 192 / 320 faces on
+// its three lower sides, and as many boundary faces at a corner of the domain:
+// both lists hold CELLS/2 records, so corner and edge bricks stay on the fast
+// path.
 constexpr int gather_max_up(int cells) { return cells * 3 + cells / 4; }
 constexpr int gather_max_ext(int cells) { return cells / 2; }
-constexpr int gather_max_bnd(int cells) { return cells / 4; }
+constexpr int gather_max_bnd(int cells) { return cells / 2; }
 constexpr int kGatherMaxLow = 8;
 
 struct fv_mesh {

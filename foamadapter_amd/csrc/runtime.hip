@@ -142,6 +142,39 @@ __global__ void k_permute(int64_t n, int nc, int mode, int oriented,
     }
 }
 
+// Streaming reference kernels: what a kernel that only streams HBM reaches on
+// this part (the ceiling the face operators are judged against).  Each thread
+// moves kStreamU 16-byte elements, lanes consecutive (coalesced), one pass:
+//   mode 0 copy   a = b          (16 bytes read, 16 written per element)
+//   mode 1 triad  a = b + s * c  (32 read, 16 written)
+constexpr int kStreamU = 4;
+
+template <int MODE>
+__global__ __launch_bounds__(256) void k_stream(int64_t n2, const double2* __restrict__ b,
+                                                const double2* __restrict__ c,
+                                                double2* __restrict__ a, double s) {
+    const int64_t base = (int64_t)blockIdx.x * (256 * kStreamU) + threadIdx.x;
+    double2 vb[kStreamU], vc[kStreamU];
+#pragma unroll
+    for (int u = 0; u < kStreamU; ++u) {
+        const int64_t i = base + u * 256;
+        if (i < n2) {
+            vb[u] = b[i];
+            if (MODE == 1) vc[u] = c[i];
+        }
+    }
+#pragma unroll
+    for (int u = 0; u < kStreamU; ++u) {
+        const int64_t i = base + u * 256;
+        if (i < n2) {
+            if (MODE == 1)
+                a[i] = make_double2(vb[u].x + s * vc[u].x, vb[u].y + s * vc[u].y);
+            else
+                a[i] = vb[u];
+        }
+    }
+}
+
 }  // namespace fvhip
 
 using namespace fvhip;
@@ -318,6 +351,28 @@ int fv_permute(fv_ctx* ctx, fv_prec prec, int64_t n, int ncomp, int mode, int or
         else
             k_permute<float><<<g, 256, 0, ctx->stream>>>(n, ncomp, mode, oriented, map,
                                                          (const float*)src, (float*)dst);
+        FV_LAUNCH_CHECK();
+    });
+}
+
+int fv_stream(fv_ctx* ctx, int mode, int64_t n, const double* d_b, const double* d_c, double* d_a,
+              double s) {
+    return guarded([&] {
+        FV_REQUIRE(ctx && n >= 0 && (mode == 0 || mode == 1), "bad argument");
+        FV_REQUIRE(n % 2 == 0, "n must be even (16-byte elements)");
+        if (!n) return;
+        FV_REQUIRE(d_a && d_b && (mode == 0 || d_c), "NULL pointer");
+        FV_REQUIRE(((uintptr_t)d_a | (uintptr_t)d_b | (uintptr_t)(mode ? d_c : d_b)) % 16 == 0,
+                   "arrays must be 16-byte aligned");
+        const int64_t n2 = n / 2;
+        const int64_t g = (n2 + 256 * kStreamU - 1) / (256 * kStreamU);
+        FV_REQUIRE(g < (int64_t(1) << 31), "array too long");
+        if (mode == 0)
+            k_stream<0><<<(unsigned)g, 256, 0, ctx->stream>>>(n2, (const double2*)d_b, nullptr,
+                                                           (double2*)d_a, s);
+        else
+            k_stream<1><<<(unsigned)g, 256, 0, ctx->stream>>>(n2, (const double2*)d_b,
+                                                           (const double2*)d_c, (double2*)d_a, s);
         FV_LAUNCH_CHECK();
     });
 }

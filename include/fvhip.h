@@ -96,6 +96,13 @@ int fv_convert(fv_ctx* ctx, fv_prec to, void* d_dst, fv_prec from, const void* d
 int fv_fill_uniform(fv_ctx* ctx, fv_prec prec, void* d_ptr, int64_t n, double lo, double hi,
                     uint64_t seed, uint64_t stream);
 
+/* Streaming reference kernels (n doubles, 16-byte aligned arrays, n even):
+ * mode 0 copy d_a = d_b, mode 1 triad d_a = d_b + s * d_c.  No counterpart in
+ * the reference: the measured HBM ceiling of a kernel that only streams, which
+ * the benchmark reports beside the face operators. */
+int fv_stream(fv_ctx* ctx, int mode, int64_t n, const double* d_b, const double* d_c, double* d_a,
+              double s);
+
 /* ---- mesh ---------------------------------------------------------------- */
 
 /* OpenFOAM polyMesh primitives (host arrays): replaces the fvMesh that

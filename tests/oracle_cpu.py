@@ -61,6 +61,12 @@ lib.orc_laplacian_bcoeffs.argtypes = [C.c_int] * 3 + [C.c_void_p] * 7 + [C.c_dou
 lib.orc_set_reference.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_double, C.c_void_p, C.c_void_p]
 lib.orc_cg.argtypes = [C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                        C.c_double, C.c_int, C.c_void_p]
+lib.orc_omp_cg.argtypes = lib.orc_cg.argtypes
+
+
+def omp_set_threads(n: int) -> int:
+    """OpenMP threads of the multi-core restatement (n <= 0: unchanged); returns the count."""
+    return lib.orc_omp_set_threads(int(n))
 
 
 def geometry(poly) -> dict:
@@ -269,12 +275,15 @@ class LinSys:
         N = len(self.rhs)
         return sps.csr_matrix((self.values, self.sp["colIdxs"], self.sp["rowOffs"]), shape=(N, N))
 
-    def cg(self, x0, rel_tol, max_iter=1000):
+    def cg(self, x0, rel_tol, max_iter=1000, omp=False):
+        """Jacobi-PCG from x0: the serial oracle, or (omp=True) its multi-core
+        form (same recurrence, block-ordered dot products)."""
         x = np.array(x0, dtype=np.float64, copy=True)
         stats = np.zeros(3)
         sp = self.sp
-        rc = lib.orc_cg(len(self.rhs), P(sp["rowOffs"]), P(sp["colIdxs"]), P(self.values),
-                        P(self.rhs), P(x), rel_tol, max_iter, P(stats))
+        fn = lib.orc_omp_cg if omp else lib.orc_cg
+        rc = fn(len(self.rhs), P(sp["rowOffs"]), P(sp["colIdxs"]), P(self.values),
+                P(self.rhs), P(x), rel_tol, max_iter, P(stats))
         return x, {"init_res": stats[0], "final_res": stats[1], "iters": int(stats[2]), "rc": rc}
 
 

@@ -162,7 +162,9 @@ def test_forward_euler_advection_march(ctx, scheme, steps):
         odts.append((dt, co))
         t, dt = t + new_dt, new_dt
     assert [d for d, _ in dts] == [d for d, _ in odts]          # same time steps, bit for bit
-    assert max(co for _, co in dts) > 0.3                        # the Courant limit was reached
+    assert dts[-1][0] > 5 * dts[0][0]                            # the step grew (factor 1.2)
+    if steps >= 40:
+        assert max(co for _, co in dts) > 0.3                    # ... up to the Courant limit
     assert relmax(got, Tr) <= 1e-12, relmax(got, Tr)
     assert np.max(np.abs(got - T0)) > 1e-2                       # the field really moved
     bc.close()

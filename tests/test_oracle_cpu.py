@@ -694,3 +694,27 @@ def test_heat_time_march_dense_solve_and_energy_balance():
         assert abs(gained - wall) <= 1e-10 * max(abs(gained), abs(wall)), (step, gained, wall)
         assert Td.min() >= 300.0 and Td.max() <= 400.0          # discrete maximum principle
         T = Td
+
+
+def test_multicore_cg_matches_serial_and_is_thread_count_independent():
+    """The implicit solves' CPU baseline (orc_omp_cg): the serial CG's recurrence
+    with block-ordered dot products -- the same bits at 1, 3 and all threads,
+    the serial solution within the tolerance, the iteration count within 1."""
+    # 32^3: four dot-product blocks of 8192 rows
+    g, sp, ls, Told, Tb, gamma = _heat_system(32, perm=5, jitter=0.2, kappa=1.0, dt=1e-2)
+    xs, st_s = ls.cg(Told, 1e-10, 1000)
+    n0 = orc.omp_set_threads(0)
+    outs = []
+    for t in (1, 3, n0):
+        orc.omp_set_threads(t)
+        x, st = ls.cg(Told, 1e-10, 1000, omp=True)
+        outs.append((x, st))
+    orc.omp_set_threads(n0)
+    for x, st in outs[1:]:
+        assert np.array_equal(x, outs[0][0]) and st == outs[0][1]
+    x, st = outs[0]
+    assert st["rc"] == 0 and st["final_res"] <= 1e-10 and abs(st["iters"] - st_s["iters"]) <= 1
+    assert st_s["iters"] > 20                      # a real solve, not a single step
+    assert relmax(x, xs) < 1e-8
+    r = ls.rhs - ls.scipy() @ x
+    assert np.linalg.norm(r) / np.linalg.norm(ls.rhs) < 2e-10
