@@ -149,6 +149,7 @@ _SIGS = {
     "fv_constrain_hbya": [_vp, _vp, _vp],
     "fv_linsys_boundary_coeffs": [_vp, C.POINTER(_vp), C.POINTER(_vp)],
     "fv_courant_number": [_vp, _vp, _vp, _d, C.POINTER(_d)],
+    "fv_courant_number_noalloc": [_vp, _vp, _vp, _d, _vp],
     "fv_set_delta_t": [_d, _d, _d, _d, C.POINTER(_d)],
     "fv_solve_cg": [_vp, _vp, C.POINTER(FvSolveOpts), C.POINTER(FvSolverStats)],
     "fv_solve_bicgstab": [_vp, _vp, C.POINTER(FvSolveOpts), C.POINTER(FvSolverStats)],

@@ -16,9 +16,18 @@
 
 #include <algorithm>
 #include <cmath>
+#include <mutex>
 #include <vector>
 
 using namespace fvhip;
+
+// Block maxima of the Courant kernel for fv_courant_number_noalloc: one double
+// per 256-cell block, so meshes of up to kCoScratchBlocks * 256 = 2^25 cells
+// (1 MiB; every GPU the library runs on holds its own copy until the process
+// ends).  Host code never names this array in a launch or a copy: it uses the
+// device address hipGetSymbolAddress returns for the current device.
+constexpr int kCoScratchBlocks = 1 << 17;
+static __device__ __align__(256) double scratch[kCoScratchBlocks];
 
 namespace {
 
@@ -155,6 +164,45 @@ __global__ __launch_bounds__(kBlock) void k_conum(int N, int F, const int32_t* _
     if (threadIdx.x == 0) blockMax[blockIdx.x] = sh[0];
 }
 
+// one block: the maximum of the n block maxima, times dt/2, into *co (device)
+__global__ __launch_bounds__(kBlock) void k_conum_final(int n, const double* __restrict__ blockMax,
+                                                        double dt, double* __restrict__ co) {
+    __shared__ double sh[kBlock];
+    double m = 0.0;
+    for (int i = threadIdx.x; i < n; i += kBlock) m = fmax(m, blockMax[i]);
+    sh[threadIdx.x] = m;
+    __syncthreads();
+    for (int w = kBlock / 2; w > 0; w >>= 1) {
+        if (threadIdx.x < w) sh[threadIdx.x] = fmax(sh[threadIdx.x], sh[threadIdx.x + w]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *co = 0.5 * sh[0] * dt;
+}
+
+// Device address of `scratch` on each device, looked up once per device with
+// that device current (a fixed table: the lookup allocates nothing).
+constexpr int kScratchDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[kScratchDevices] = {};
+
+double* scratch_on(int device) {
+    FV_REQUIRE(device >= 0 && device < kScratchDevices, "device index out of range");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[device]) {
+        int cur = 0;
+        FV_HIP(hipGetDevice(&cur));
+        if (cur != device) FV_HIP(hipSetDevice(device));
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (cur != device) FV_HIP(hipSetDevice(cur));
+        if (e != hipSuccess || !p)
+            throw Error{FV_EDEVICE, std::string("hipGetSymbolAddress(scratch): ") +
+                                        hipGetErrorString(e)};
+        g_scratch_ptr[device] = static_cast<double*>(p);
+    }
+    return g_scratch_ptr[device];
+}
+
 void check_ls(const fv_linsys* ls) { FV_REQUIRE(ls && ls->sp && ls->sp->mesh, "NULL linear system"); }
 
 }  // namespace
@@ -245,6 +293,23 @@ int fv_courant_number(fv_ctx* ctx, const fv_mesh* m, const void* d_phi, double d
         double mx = 0.0;
         for (double v : h) mx = std::max(mx, v);
         *co = 0.5 * mx * dt;
+    });
+}
+
+int fv_courant_number_noalloc(fv_ctx* ctx, const fv_mesh* m, const void* d_phi, double dt,
+                              double* d_co) {
+    return guarded([&] {
+        FV_REQUIRE(ctx && m && m->ctx == ctx && d_phi && d_co, "bad arguments");
+        const int G = grid_for(m->nCells, kBlock);
+        FV_REQUIRE(G <= kCoScratchBlocks,
+                   "mesh too large for fv_courant_number_noalloc: at most 33554432 cells "
+                   "(use fv_courant_number)");
+        double* bm = scratch_on(ctx->device);
+        // every block writes its slot, so nothing in the scratch needs clearing
+        k_conum<<<G, kBlock, 0, ctx->stream>>>(m->nCells, m->nInt, m->cfOff.p, m->cfIdx.p,
+                                                (const double*)d_phi, m->V.p, bm);
+        k_conum_final<<<1, kBlock, 0, ctx->stream>>>(G, bm, dt, (double*)d_co);
+        FV_LAUNCH_CHECK();
     });
 }
 

@@ -404,6 +404,17 @@ int fv_constrain_hbya(const fv_bc* u_bc, const void* d_Ub, void* d_hbya_b);
  * OpenFOAM CourantNo.H): co = 0.5 * max_c(sum_faces |phi| / V_c) * dt over the
  * internal and boundary faces of each cell, d_phi [F+B] FP64. */
 int fv_courant_number(fv_ctx* ctx, const fv_mesh* mesh, const void* d_phi, double dt, double* co);
+/* fv_courant_number for callers that cannot allocate (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid allocation).  No
+ * counterpart in the reference.  Allocates nothing: the per-block maxima live in
+ * a fixed 1 MiB device array of the library (one per GPU, held until the
+ * process ends), so meshes of at most 2^25 = 33,554,432 cells (FV_EINVAL
+ * beyond); the result co = 0.5 * max_c(sum |phi| / V) * dt is written to the
+ * DEVICE double d_co, stream-ordered and asynchronous (no host wait), with the
+ * bits of fv_courant_number.  Calls that overlap on one GPU (from several
+ * streams or threads) share that array and race. */
+int fv_courant_number_noalloc(fv_ctx* ctx, const fv_mesh* mesh, const void* d_phi, double dt,
+                              double* d_co);
 /* setDeltaT (src/auxiliary/setup.cpp:13-22): new_dt = min(f * dt, max_dt) with
  * f = min(min(maxCo/(co + SMALL), 1 + 0.1 maxCo/(co + SMALL)), 1.2).  Host only. */
 int fv_set_delta_t(double dt, double co, double max_co, double max_dt, double* new_dt);
