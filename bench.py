@@ -71,21 +71,39 @@ KERNEL_OF = {
     ("laplacian", "f32"): "k_gather<float, LapOp<float>, 1024, 512>",
 }
 TRAFFIC_PROFILE = ROOT / "profiles" / "faceops_sweep_256cube.md"
+# the sources whose kernels the traffic profile measured; the profile records
+# their hash, and a profile of other sources is not used
+KERNEL_SOURCES = ("faceops.hip", "mesh.hip", "mesh.hpp", "common.hpp")
+
+
+def kernel_stamp():
+    import hashlib
+    h = hashlib.sha256()
+    for f in KERNEL_SOURCES:
+        h.update((ROOT / "foamadapter_amd" / "csrc" / f).read_bytes())
+    return h.hexdigest()[:16]
 
 
 def measured_traffic(kernel, n):
-    """HBM bytes per launch of `kernel` from the committed rocprofv3 FETCH_SIZE /
-    WRITE_SIZE summary of this sweep (256^3 only), or None."""
+    """(HBM bytes per launch of `kernel`, source note) from the committed rocprofv3
+    FETCH_SIZE / WRITE_SIZE summary of this sweep (256^3 only), provided the
+    summary was measured on the current kernel sources (its recorded hash);
+    otherwise (None, why)."""
+    import re
     if n != 256 or not TRAFFIC_PROFILE.exists():
-        return None
-    for line in TRAFFIC_PROFILE.read_text().splitlines():
+        return None, "no traffic profile for this size"
+    text = TRAFFIC_PROFILE.read_text()
+    m = re.search(r"kernel sources sha256\[:16\]: `([0-9a-f]{16})`", text)
+    if not m or m.group(1) != kernel_stamp():
+        return None, "the committed traffic profile measured other kernel sources"
+    for line in text.splitlines():
         cells = [c.strip() for c in line.strip().strip("|").split("|")]
-        if len(cells) == 4 and cells[0] == f"`{kernel}`":
+        if len(cells) >= 4 and cells[0] == f"`{kernel}`":
             try:
-                return int(float(cells[3]) * 1e9)
+                return int(float(cells[3]) * 1e9), f"{TRAFFIC_PROFILE.relative_to(ROOT)} ({m.group(1)})"
             except ValueError:
-                return None
-    return None
+                return None, "unreadable profile row"
+    return None, "kernel not in the traffic profile"
 
 
 def stream_ceiling(ctx, n_doubles=1 << 28, runs=10):
@@ -97,7 +115,9 @@ def stream_ceiling(ctx, n_doubles=1 << 28, runs=10):
     b.fill(1.0)
     c.fill(2.0)
     out = {}
-    for mode, name, moved in ((0, "copy", 16), (1, "triad", 24)):
+    # bytes moved per double of the arrays: copy 8 read + 8 written, triad 16 + 8,
+    # read 8 (its block sums are 1/2048 of that)
+    for mode, name, moved in ((0, "copy", 16), (1, "triad", 24), (2, "read", 8)):
         for _ in range(3):
             _lib.call("fv_stream", ctx.handle, mode, n_doubles, b.ptr, c.ptr, a.ptr, 0.5)
         ctx.sync()
@@ -106,18 +126,78 @@ def stream_ceiling(ctx, n_doubles=1 << 28, runs=10):
             _lib.call("fv_stream", ctx.handle, mode, n_doubles, b.ptr, c.ptr, a.ptr, 0.5)
         ctx.record(57)
         ms = ctx.elapsed_ms(56, 57) / runs
-        gbs = moved * n_doubles / 2 / (ms * 1e-3) / 1e9
+        gbs = moved * n_doubles / (ms * 1e-3) / 1e9
         out[name] = {"ms": round(ms, 4), "GB/s": round(gbs, 1),
                      "frac_peak": round(gbs / HBM_PEAK_GBS, 4)}
     for x in (a, b, c):
         x.close()
-    out["bytes"] = f"{8 * n_doubles >> 30} GiB per array; copy moves 2, triad 3 arrays"
+    out["bytes"] = (f"{8 * n_doubles >> 30} GiB per array; copy moves 2 arrays, triad 3, "
+                    f"read 1")
     return out
+
+
+def mapback_cost(ctx, case, sweep, runs=10):
+    """Per op of the sweep: the time to map its output back to the caller's
+    numbering (the scatter of fv_permute through the renumbered mesh's map,
+    oriented for snGrad) and the op's algorithmic GB/s with that mapping
+    included."""
+    from foamadapter_amd import _lib
+    from foamadapter_amd.runtime import prec_of
+    mesh = case.mesh
+    N, F, B = mesh.n_cells, mesh.n_internal_faces, mesh.n_boundary_faces
+    out, tot_ms = {}, 0.0
+    for (op, prec), (fn, res) in sweep["ops"].items():
+        face = op in ("interp_linear", "sngrad")
+        mp = mesh.face_map if face else mesh.cell_map
+        ncomp = 3 if op == "grad" else 1
+        dst = ctx.empty(res.shape, res.dtype)
+        args = (ctx.handle, prec_of(res.dtype), res.shape[0], ncomp, 1, 1 if op == "sngrad" else 0,
+                mp.ptr, res.ptr, dst.ptr)
+        _lib.call("fv_permute", *args)
+        ctx.sync()
+        ctx.record(54)
+        for _ in range(runs):
+            _lib.call("fv_permute", *args)
+        ctx.record(55)
+        ms = ctx.elapsed_ms(54, 55) / runs
+        op_ms = sweep["per_op"][f"{op}_{prec}"]["ms"]
+        tot_ms += op_ms + ms
+        out[f"{op}_{prec}"] = {"map_ms": round(ms, 4),
+                               "GB/s_with_map": round(alg_bytes(op, prec, N, F, B) /
+                                                      ((op_ms + ms) * 1e-3) / 1e9, 1)}
+        dst.close()
+    step_b = sum(alg_bytes(op, p, N, F, B) for op, p in sweep["ops"])
+    return {"per_op": out, "sweep_GB/s_with_map": round(step_b / (tot_ms * 1e-3) / 1e9, 1),
+            "what": "each output scattered back to the caller's cell / face numbering after "
+                    "the op (fv_permute), timed with HIP events"}
 
 
 def alg_bytes(op, prec, N, F, B):
     f, c, b = BYTES[(op, prec)]
     return f * F + c * N + b * B
+
+
+def cg_bytes(N, nnz, iters, reps, mixed, starts=1):
+    """Algorithmic bytes of Jacobi-PCG on an N-row CSR system with nnz entries
+    (int32 columns and row offsets): each FP64 residual r = b - A x (the start
+    and every replacement of the mixed solver) reads the matrix, x, b and writes r
+    (12 nnz + 4(N+1) + 24 N); each iteration reads the matrix and the vectors
+    it needs once (FP64 12 nnz + 4(N+1) + 104 N, FP32 inner 8 nnz + 4(N+1) + 52 N)."""
+    res64 = 12 * nnz + 4 * (N + 1) + 24 * N
+    it = (8 * nnz + 4 * (N + 1) + 52 * N) if mixed else (12 * nnz + 4 * (N + 1) + 104 * N)
+    return (starts + reps) * res64 + iters * it
+
+
+def heat_assembly_bytes(N, F, B):
+    """ddt + laplacian assembly (implicit): 56 / 40 / 36 bytes per internal face /
+    cell / boundary face."""
+    return 56 * F + 40 * N + 36 * B
+
+
+def roof(bytes_, ms):
+    gbs = bytes_ / (ms * 1e-3) / 1e9 if ms > 0 else 0.0
+    return {"algorithmic_GB": round(bytes_ / 1e9, 4), "effective_GB/s": round(gbs, 1),
+            "frac_peak": round(gbs / HBM_PEAK_GBS, 4)}
 
 
 def log(msg):
@@ -327,14 +407,16 @@ def cpu_baseline(case, sweep, budget_s=24.0):
                        "sample": f"serial oracle, runs per op {sruns}, {st:.1f}s"}}, parity
 
 
-def bench_heat(ctx, n=32, steps=10):
+def bench_heat(ctx, n=32, steps=10, cpu=True):
     """Implicit heat transfer: ddt(T) - laplacian(1e-2, T), Euler dt=1e-3, 10 steps,
-    Jacobi-CG to 1e-10; walls T=300 (x0), 400 (x1), 350 elsewhere."""
+    Jacobi-CG to 1e-10; walls T=300 (x0), 400 (x1), 350 elsewhere.  Bytes: per step
+    the assembly and the CG (iterations, FP64 residuals)."""
     import foamadapter_amd as fa
     from foamadapter_amd import dsl, fvcc
     poly = fa.PolyMesh.hex(n)
     m = fa.Mesh.from_poly(ctx, poly)
     N, F, B = m.n_cells, m.n_internal_faces, m.n_boundary_faces
+    nnz = N + 2 * F
     po = m.get("patchOffsets")
     Tb = np.full(B, 350.0)
     Tb[po[0]:po[1]] = 300.0
@@ -350,28 +432,71 @@ def bench_heat(ctx, n=32, steps=10):
         for run in range(2):
             T = fvcc.VolumeField(m, ctx.uniform(N, 300.0, 400.0, 1001, 0), ctx.upload(Tb), bc, "T")
             eqn = dsl.imp.ddt(T) - dsl.imp.laplacian(kap, T)
-            iters = []
+            stats = []
             ctx.sync()
             t0 = time.perf_counter()
             for s in range(steps):
                 st = dsl.solve(eqn, T, s * 1e-3, 1e-3, solver={"relTol": 1e-10, "maxIter": 1000,
                                                                  "precision": mode})
-                iters.append(st.numIter)
+                stats.append(st)
             ctx.sync()
-            runs.append((time.perf_counter() - t0, iters, T.numpy()))
-        wall, iters, Tf = runs[1]
+            runs.append((time.perf_counter() - t0, stats, T.numpy()))
+        wall, stats, Tf = runs[1]
         assert np.array_equal(Tf, runs[0][2]), "heat run is not reproducible"
+        mixed = mode == "mixed"
+        nbytes = sum(heat_assembly_bytes(N, F, B) +
+                     cg_bytes(N, nnz, st.numIter, st.outerIter if mixed else 0, mixed)
+                     for st in stats)
         res[mode] = {"wall_ms": round(wall * 1e3, 2), "ms_per_step": round(wall * 1e3 / steps, 3),
                      "first_run_ms_per_step": round(runs[0][0] * 1e3 / steps, 3),
-                     "cg_iters": iters, "Tmin": float(Tf.min()), "Tmax": float(Tf.max())}
+                     "cg_iters": [st.numIter for st in stats],
+                     "fp64_residual_replacements": [st.outerIter for st in stats] if mixed else None,
+                     "Tmin": float(Tf.min()), "Tmax": float(Tf.max()),
+                     "roofline": roof(nbytes, wall * 1e3)}
+        res[mode]["_T"] = Tf
+    out = {"mesh": f"{n}^3", "cells": N}
+    if cpu:
+        out["cpu_baseline"], Tcpu = cpu_heat(poly, Tb, steps)
+        for mode in ("double", "mixed"):
+            Tg = res[mode]["_T"]
+            res[mode]["rel_diff_vs_oracle"] = float(np.max(np.abs(Tg - Tcpu)) / np.max(np.abs(Tcpu)))
+    for mode in ("double", "mixed"):
+        del res[mode]["_T"]
     m.close()
-    return {"mesh": f"{n}^3", "cells": N, **res}
+    return {**out, **res}
 
 
-def bench_poisson(ctx, n=128, seed=1003, repeats=3):
+def cpu_heat(poly, Tb, steps):
+    """The heat case on the oracle: serial assembly per step and the multi-core
+    Jacobi-PCG (orc_omp_cg, OMP_NUM_THREADS threads) to 1e-10; the run is the
+    CPU baseline of the heat leg and its end state the parity check."""
+    sys.path.insert(0, str(ROOT / "tests"))
+    import oracle_cpu as orc
+    g = orc.geometry(poly)
+    sp = orc.sparsity(g)
+    N, F, B = len(g["V"]), len(g["owner"]), len(g["faceCells"])
+    gam = np.full(F + B, 1e-2)
+    T = orc.uniform(N, 300.0, 400.0, 1001, 0)
+    iters = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        ls = orc.LinSys(g, sp)
+        ls.ddt(T, 1e-3)
+        ls.laplacian(gam, Tb, [0] * 6, coeff=-1.0)
+        T, st = ls.cg(T, 1e-10, 1000, omp=True)
+        iters.append(st["iters"])
+    wall = time.perf_counter() - t0
+    threads = orc.OmpOps.threads()
+    return {"value": round(wall * 1e3 / steps, 3), "unit": "ms/step", "cores": threads,
+            "kind": "port", "cg_iters": iters,
+            "sample": f"oracle: serial assembly + multi-core Jacobi-PCG ({threads} threads) per "
+                      f"step, all {steps} steps of the same case"}, T
+
+
+def bench_poisson(ctx, n=128, seed=1003, repeats=3, cpu=True):
     """laplacian(1,p) = div(phi(U)), p=0 on x0, zeroGradient elsewhere, 128^3
     jittered + permuted (renumbered at import); FP64 CG vs FP32 CG in FP64
-    refinement, both to 1e-8 relative residual."""
+    refinement, both to 1e-8 relative residual.  Bytes: cg_bytes."""
     import foamadapter_amd as fa
     from foamadapter_amd import dsl, fvcc, la
     t0 = time.time()
@@ -388,6 +513,7 @@ def bench_poisson(ctx, n=128, seed=1003, repeats=3):
     ls = dsl.assemble(dsl.imp.laplacian(one, p) - dsl.exp.div(phi), p, 0.0, 1.0)
     out = {}
     xs = {}
+    nnz = N + 2 * F
     for mode in ("double", "mixed"):
         best = None
         for _ in range(repeats):
@@ -402,22 +528,56 @@ def bench_poisson(ctx, n=128, seed=1003, repeats=3):
                 best = (ms, st)
             xs[mode] = x
         ms, st = best
-        # algorithmic bytes: per CG iteration 12 nnz + 4(N+1) + 104 N (FP64) or
-        # 8 nnz + 4(N+1) + 52 N (FP32 inner), per FP64 residual replacement
-        # 12 nnz + 4(N+1) + 24 N (CSR, x, b, r)
-        nnz = N + 2 * F
-        it_b = (12 * nnz + 4 * (N + 1) + 104 * N) if mode == "double" else \
-            (8 * nnz + 4 * (N + 1) + 52 * N)
-        rep_b = 0 if mode == "double" else st.outerIter * (12 * nnz + 4 * (N + 1) + 24 * N)
+        mixed = mode == "mixed"
+        nbytes = cg_bytes(N, nnz, st.numIter, st.outerIter if mixed else 0, mixed,
+                          starts=1 if mixed else 2)   # FP64: start and the explicit residual at exit
         out[mode] = {"ms": round(ms, 3), "solves_per_s": round(1e3 / ms, 2), "iters": st.numIter,
                      "outer": st.outerIter, "final_rel_res": st.finalResNorm,
-                     "true_rel_res": st.trueResNorm,
-                     "effective_GB/s": round((st.numIter * it_b + rep_b) / ms / 1e6, 1)}
+                     "true_rel_res": st.trueResNorm, "roofline": roof(nbytes, ms)}
     a, b = xs["mixed"].numpy(), xs["double"].numpy()
     out["mixed_vs_fp64_rel_err"] = float(np.linalg.norm(a - b) / np.linalg.norm(b))
     out["speedup_mixed_over_fp64"] = round(out["double"]["ms"] / out["mixed"]["ms"], 3)
+    if cpu:
+        out["cpu_baseline"] = cpu_poisson(ls, b)
     m.close()
     return {"mesh": f"{n}^3 jittered, permuted, renumbered", "cells": N, **out}
+
+
+def cpu_poisson(ls, x_gpu, serial_iters=40):
+    """The same assembled system (values and right-hand side downloaded from the
+    GPU) solved by the oracle's multi-core Jacobi-PCG (orc_omp_cg) from x = 0 to
+    1e-8: the CPU baseline of the Poisson leg, and its solution against the GPU's.
+    The serial CG runs a bounded sample of serial_iters iterations beside it."""
+    sys.path.insert(0, str(ROOT / "tests"))
+    import oracle_cpu as orc
+    A = ls.scipy()
+    rhs = ls.rhs()
+    N = A.shape[0]
+    sp = {"rowOffs": np.ascontiguousarray(A.indptr, np.int32),
+          "colIdxs": np.ascontiguousarray(A.indices, np.int32)}
+    vals = np.ascontiguousarray(A.data)
+    stats = np.zeros(3)
+    x = np.zeros(N)
+    t0 = time.perf_counter()
+    orc.lib.orc_omp_cg(N, orc.P(sp["rowOffs"]), orc.P(sp["colIdxs"]), orc.P(vals), orc.P(rhs),
+                       orc.P(x), 1e-8, 20000, orc.P(stats))
+    wall = time.perf_counter() - t0
+    xs = np.zeros(N)
+    st2 = np.zeros(3)
+    t1 = time.perf_counter()
+    orc.lib.orc_cg(N, orc.P(sp["rowOffs"]), orc.P(sp["colIdxs"]), orc.P(vals), orc.P(rhs),
+                   orc.P(xs), 0.0, serial_iters, orc.P(st2))
+    ts = time.perf_counter() - t1
+    threads = orc.OmpOps.threads()
+    iters = int(stats[2])
+    return {"value": round(1.0 / wall, 4), "unit": "solves/s", "ms": round(wall * 1e3, 1),
+            "cores": threads, "kind": "port", "iters": iters, "final_rel_res": float(stats[1]),
+            "gpu_fp64_rel_diff": float(np.linalg.norm(x - x_gpu) / np.linalg.norm(x_gpu)),
+            "sample": f"oracle multi-core Jacobi-PCG ({threads} threads), the whole solve of the "
+                      f"same {N}-row system from x = 0 to 1e-8 ({iters} iterations)",
+            "serial": {"ms_per_iter": round(ts * 1e3 / serial_iters, 2), "cores": 1,
+                       "sample": f"serial oracle CG, {serial_iters} iterations of the same solve",
+                       "solve_ms_estimate": round(ts * 1e3 / serial_iters * iters, 1)}}
 
 
 def _loguniform(u, lo, hi):
@@ -459,7 +619,8 @@ def bench_batched(ctx, P=512, seed=1004, steps=5):
         T0 = ctx.uniform(N, 300.0, 400.0, seed + 1 + i, 0)
         probs.append({"m": m, "bc": bc, "Tb": ctx.upload(Tb), "T0": T0,
                       "kap": ctx.upload(np.full(F + B, kappas[i])),
-                      "ls": la.LinearSystem(sps[ns[i]]), "N": N})
+                      "ls": la.LinearSystem(sps[ns[i]]), "N": N, "nnz": N + 2 * F,
+                      "asm_b": heat_assembly_bytes(N, F, B)})
     ctx.sync()
     total_cells = int(sum(p["N"] for p in probs))
     log(f"batched: {P} problems, {total_cells} cells, set up in {time.time() - t0:.1f}s")
@@ -468,7 +629,7 @@ def bench_batched(ctx, P=512, seed=1004, steps=5):
     def run_plain():
         o = solver_opts(solver)
         st = _lib.FvSolverStats()
-        iters = 0
+        iters, nbytes = 0, 0
         for p in probs:
             x = p["x"]
             _lib.call("fv_memcpy_d2d", ctx.handle, x.ptr, p["T0"].ptr, x.nbytes)
@@ -478,20 +639,23 @@ def bench_batched(ctx, P=512, seed=1004, steps=5):
                           p["kap"].ptr, p["bc"].handle, p["Tb"].ptr, None, -1.0)
                 _lib.call("fv_solve_cg", p["ls"].handle, x.ptr, C.byref(o), C.byref(st))
                 iters += st.iters
+                nbytes += p["asm_b"] + cg_bytes(p["N"], p["nnz"], st.iters, 0, False, starts=2)
         ctx.sync()
-        return iters
+        return iters, nbytes
 
     def run_batched(batch, heat):
         for p in probs:
             _lib.call("fv_memcpy_d2d", ctx.handle, p["x"].ptr, p["T0"].ptr, p["x"].nbytes)
             _lib.call("fv_memcpy_d2d", ctx.handle, p["xo"].ptr, p["T0"].ptr, p["x"].nbytes)
-        iters = 0
+        iters, nbytes = 0, 0
         for s in range(steps):
             heat.assemble(-1.0)                           # all 512 systems, one launch
             stats = batch.solve(solver, use_graph=True)   # x_old <- x after the solve
             iters += sum(st.numIter for st in stats)
+            nbytes += sum(p["asm_b"] + cg_bytes(p["N"], p["nnz"], st.numIter, st.outerIter, True)
+                          for p, st in zip(probs, stats))
         ctx.sync()
-        return iters
+        return iters, nbytes
 
     for p in probs:
         p["x"] = ctx.empty((p["N"],))
@@ -500,11 +664,11 @@ def bench_batched(ctx, P=512, seed=1004, steps=5):
            "mesh_edges": [int(ns.min()), int(ns.max())], "solves": P * steps}
     # plain FP64 per-problem drop-in
     t = time.perf_counter()
-    it_plain = run_plain()
+    it_plain, b_plain = run_plain()
     wall_plain = time.perf_counter() - t
-    x_plain = [p["x"].numpy() for p in probs[:8]]
+    x_plain = [p["x"].numpy() for p in probs]
     out["plain_fp64"] = {"wall_s": round(wall_plain, 3), "solves_per_s": round(P * steps / wall_plain, 1),
-                         "cg_iters": it_plain}
+                         "cg_iters": it_plain, "roofline": roof(b_plain, wall_plain * 1e3)}
     # batched, mixed precision, autotuned shape
     batch = la.Batch(ctx, "mixed")
     for p in probs:
@@ -522,12 +686,15 @@ def bench_batched(ctx, P=512, seed=1004, steps=5):
                             for p in probs])
     run_batched(batch, heat)                              # warm (graph capture)
     t = time.perf_counter()
-    it_b = run_batched(batch, heat)
+    it_b, b_b = run_batched(batch, heat)
     wall_b = time.perf_counter() - t
-    err = max(float(np.max(np.abs(p["x"].numpy() - xp)) / np.max(np.abs(xp)))
-              for p, xp in zip(probs[:8], x_plain))
+    errs = np.array([float(np.max(np.abs(p["x"].numpy() - xp)) / np.max(np.abs(xp)))
+                     for p, xp in zip(probs, x_plain)])
     out["batched_mixed"] = {"wall_s": round(wall_b, 3), "solves_per_s": round(P * steps / wall_b, 1),
-                            "cg_iters": it_b, "max_rel_diff_vs_plain_first8": err}
+                            "cg_iters": it_b, "roofline": roof(b_b, wall_b * 1e3),
+                            "rel_diff_vs_plain_fp64": {"members": len(errs),
+                                                       "max": float(errs.max()),
+                                                       "median": float(np.median(errs))}}
     out["speedup"] = round(wall_plain / wall_b, 2)
     batch.close()
     for p in probs:
@@ -557,12 +724,14 @@ def _uniform_host(n, lo, hi, seed, stream):
     return lo + (hi - lo) * v
 
 
-def bench_served(n_req=96, seed=1005, rate=200.0, tile_rows=4096, chunk=16):
+def bench_served(n_req=96, seed=1005, rate=200.0, tile_rows=4096, chunk=16, rate_low=40.0):
     """Served request stream: Poisson arrivals, 70% heat (5 Euler steps, kappa ~
     logU[1e-4,1e-1]) / 30% pressure Poisson, on jittered permuted hex cubes with
     edge n ~ logU[16,160] (snapped to a catalogue of resident meshes), 200 GB
     HBM budget, 1e-8 relative residual.  The same stream is run through the
-    plain FP64 per-request path and through continuous batching (mixed)."""
+    plain FP64 per-request path and through continuous batching (mixed), at the
+    saturating rate and at a lower one; every mixed result is compared with the
+    plain FP64 result of the same request."""
     import foamadapter_amd as fa
     from foamadapter_amd.serve import Server
 
@@ -576,10 +745,9 @@ def bench_served(n_req=96, seed=1005, rate=200.0, tile_rows=4096, chunk=16):
     t0 = time.time()
     polys = [fa.PolyMesh.hex(n, jitter=0.2, jitter_seed=seed + n, perm_seed=seed + n) for n in cat]
     log(f"served: catalogue of {len(cat)} meshes generated in {time.time() - t0:.1f}s")
-    results = {}
     budget = 200 << 30
 
-    def run(label, **opts):
+    def run(label, rate_, keep, **opts):
         srv = Server(0, hbm_budget=budget, **opts)
         t1 = time.time()
         ids = [srv.add_mesh(p, renumber=True) for p in polys]
@@ -592,8 +760,8 @@ def bench_served(n_req=96, seed=1005, rate=200.0, tile_rows=4096, chunk=16):
         start = time.perf_counter()
         tnext = start
         for i in range(n_req):
-            if rate:
-                tnext += gaps[i] / rate
+            if rate_:
+                tnext += gaps[i] / rate_
                 d = tnext - time.perf_counter()
                 if d > 0:
                     time.sleep(d)
@@ -603,11 +771,18 @@ def bench_served(n_req=96, seed=1005, rate=200.0, tile_rows=4096, chunk=16):
             else:
                 rids.append(srv.submit_heat(mid, kappa=float(kappa[i]), dt=1e-3, steps=5,
                                             seed=seed + 10 * i, rel_tol=1e-8))
-        lat, bad = [], 0
-        for r in rids:
-            _, st = srv.wait(r)
+        lat, bad, results, iters = [], 0, [], 0
+        it_bytes = 0
+        for i, r in enumerate(rids):
+            x, st = srv.wait(r, ids[mesh_idx[i]] if keep else None)
+            results.append(x)
+            n = cat[mesh_idx[i]]
+            N, F = n ** 3, 3 * n * n * (n - 1)
             if st.status == 0:
                 lat.append(st.latency_ms)
+                iters += st.iters
+                it_bytes += cg_bytes(N, N + 2 * F, st.iters, 0, opts["precision"] == "mixed",
+                                     starts=0)
             else:
                 bad += 1
         wall = time.perf_counter() - start
@@ -619,18 +794,34 @@ def bench_served(n_req=96, seed=1005, rate=200.0, tile_rows=4096, chunk=16):
                 "p99_ms": round(float(np.percentile(lat, 99)), 2),
                 "mean_ms": round(float(lat.mean()), 2), "failed": bad,
                 "peak_active": s["peak_active"], "peak_gb": round(s["peak_bytes"] / 1e9, 2),
+                "cg_iters": iters,
+                "cg_iteration_roofline": roof(it_bytes, wall * 1e3),
                 "warmup_completed": srv_stats0["completed"],
                 "worker_ms": {k: round(s[k] - srv_stats0[k], 1)
                               for k in ("admit_ms", "gpu_ms", "advance_ms")},
                 "chunks": s["chunks"] - srv_stats0["chunks"],
                 "table_builds": s["table_builds"] - srv_stats0["table_builds"],
-                "graph_builds": s["graph_builds"] - srv_stats0["graph_builds"]}
+                "graph_builds": s["graph_builds"] - srv_stats0["graph_builds"]}, results
 
-    results["plain_fp64"] = run("plain", precision="double", batched=False)
-    results["batched_mixed"] = run("batched", precision="mixed", batched=True,
-                                   tile_rows=tile_rows, chunk=chunk)
-    results["speedup_solves_per_s"] = round(results["batched_mixed"]["solves_per_s"] /
-                                            results["plain_fp64"]["solves_per_s"], 2)
+    results = {}
+    plain, xp = run("plain", rate, True, precision="double", batched=False)
+    mixed, xm = run("batched", rate, True, precision="mixed", batched=True,
+                    tile_rows=tile_rows, chunk=chunk)
+    errs = np.array([float(np.linalg.norm(a - b) / np.linalg.norm(b))
+                     for a, b in zip(xm, xp) if a is not None and b is not None])
+    results["plain_fp64"] = plain
+    results["batched_mixed"] = mixed
+    results["batched_mixed"]["rel_err_vs_fp64_per_request"] = {
+        "requests": int(len(errs)), "max": float(errs.max()), "median": float(np.median(errs))}
+    results["speedup_solves_per_s"] = round(mixed["solves_per_s"] / plain["solves_per_s"], 2)
+    if rate_low:
+        lo_p, _ = run("plain-low", rate_low, False, precision="double", batched=False)
+        lo_m, _ = run("batched-low", rate_low, False, precision="mixed", batched=True,
+                      tile_rows=tile_rows, chunk=chunk)
+        results["below_saturation"] = {
+            "rate_per_s": rate_low,
+            "plain_fp64": {k: lo_p[k] for k in ("solves_per_s", "p50_ms", "p99_ms", "mean_ms")},
+            "batched_mixed": {k: lo_m[k] for k in ("solves_per_s", "p50_ms", "p99_ms", "mean_ms")}}
     results["stream"] = {"requests": n_req, "heat": int((kind == 0).sum()),
                          "poisson": int(kind.sum()), "rate_per_s": rate,
                          "mesh_edges": cat}
@@ -664,6 +855,8 @@ def main():
                     help="Poisson arrival rate (requests/s) of the served stream (0: all at once); "
                          "the default offers more than the plain path sustains, so both paths "
                          "are measured at saturation")
+    ap.add_argument("--rate-low", type=float, default=40.0,
+                    help="a second, lower arrival rate of the served stream (0: skip)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="first copy 2 GiB device-to-device 3 times (a known-size reference "
                          "for hardware traffic counters)")
@@ -694,6 +887,7 @@ def main():
     N, F, B = case.mesh.n_cells, case.mesh.n_internal_faces, case.mesh.n_boundary_faces
     (dop, dprec), dom_ms, dom_bytes = sweep["dominant"]
     achieved = dom_bytes / (dom_ms * 1e-3) / 1e9
+    traffic, traffic_note = measured_traffic(KERNEL_OF[(dop, dprec)], args.n)
     rec = {
         "metric": "face-op GB/s vs HBM peak (explicit operator sweep, FP64+FP32)",
         "value": round(sweep["GB/s"], 1),
@@ -716,13 +910,16 @@ def main():
         "per_op": sweep["per_op"],
         "roofline": {"bound": "hbm", "kernel": f"{dop}_{dprec}", "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": measured_traffic(KERNEL_OF[(dop, dprec)], args.n),
+                     "traffic": traffic, "traffic_source": traffic_note,
                      "bytes_per_launch": dom_bytes, "avg_launch_ms": round(dom_ms, 4),
                      "kernel_name": KERNEL_OF[(dop, dprec)]},
         "boundary_mapping": {"ms": round(case.mapping_ms, 3),
                              "what": "permuting the caller's T, U (cells) and gamma (faces) into "
                                      "the renumbered order once at upload"},
     }
+    rec["caller_numbering"] = mapback_cost(ctx, case, sweep)
+    log("streaming ceiling (copy, triad)")
+    rec["ceiling"] = stream_ceiling(ctx)
     if not args.no_as_given:
         g = Case(ctx, poly, args.seed, renumber=False)
         ag = time_sweep(ctx, g, max(3, args.steps // 4), 1)
@@ -742,16 +939,17 @@ def main():
     case.mesh.close()
     if not args.no_implicit:
         log("implicit heat transfer 32^3")
-        rec["implicit_heat"] = bench_heat(ctx)
+        rec["implicit_heat"] = bench_heat(ctx, cpu=not args.no_cpu)
         log("pressure Poisson 128^3, FP64 vs mixed")
-        rec["implicit_poisson"] = bench_poisson(ctx)
+        rec["implicit_poisson"] = bench_poisson(ctx, cpu=not args.no_cpu)
     if args.batched:
         log("batched heat solves")
         rec["batched_heat"] = bench_batched(ctx, P=args.batch_problems)
     if args.served:
         log("served request stream")
         rec["served"] = bench_served(n_req=args.requests, rate=args.rate,
-                                     tile_rows=args.served_tile_rows, chunk=args.served_chunk)
+                                     tile_rows=args.served_tile_rows, chunk=args.served_chunk,
+                                     rate_low=args.rate_low)
     print(json.dumps(rec), flush=True)
     ctx.close()
     return 0

@@ -603,10 +603,8 @@ int fv_batch_solve(fv_batch* b, const fv_solve_opts* o, int use_graph, fv_solver
         b->useGraph = use_graph != 0;
         for (BMember* m : b->members) m->needStart = true;
         std::vector<CgState> hs;
-        int chunks = 0;
         while (true) {
             batch_step(b, hs);
-            ++chunks;
             bool running = false;
             for (const CgState& st : hs)
                 if (st.stop < 2 && st.it < st.maxit) running = true;
@@ -620,7 +618,7 @@ int fv_batch_solve(fv_batch* b, const fv_solve_opts* o, int use_graph, fv_solver
             m->stats.final_res = hs[i].res;
             m->stats.true_res = hs[i].res;
             m->stats.iters = hs[i].it;
-            m->stats.outer_iters = chunks;
+            m->stats.outer_iters = hs[i].nrep;  // FP64 residual replacements, as fv_solve_cg
             if (stats) stats[i] = m->stats;
             // converged on the relative or (when set) the absolute residual, as
             // fv_solve_cg judges it

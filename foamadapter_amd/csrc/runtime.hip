@@ -147,6 +147,8 @@ __global__ void k_permute(int64_t n, int nc, int mode, int oriented,
 // moves kStreamU 16-byte elements, lanes consecutive (coalesced), one pass:
 //   mode 0 copy   a = b          (16 bytes read, 16 written per element)
 //   mode 1 triad  a = b + s * c  (32 read, 16 written)
+//   mode 2 read   a[block] = sum of b over the block (16 read; one double
+//                 written per 1024 elements)
 constexpr int kStreamU = 4;
 
 template <int MODE>
@@ -158,10 +160,25 @@ __global__ __launch_bounds__(256) void k_stream(int64_t n2, const double2* __res
 #pragma unroll
     for (int u = 0; u < kStreamU; ++u) {
         const int64_t i = base + u * 256;
+        vb[u] = make_double2(0.0, 0.0);
         if (i < n2) {
             vb[u] = b[i];
             if (MODE == 1) vc[u] = c[i];
         }
+    }
+    if (MODE == 2) {
+        __shared__ double sh[256];
+        double t = 0.0;
+#pragma unroll
+        for (int u = 0; u < kStreamU; ++u) t += vb[u].x + vb[u].y;
+        sh[threadIdx.x] = t;
+        __syncthreads();
+        for (int w = 128; w > 0; w >>= 1) {
+            if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) reinterpret_cast<double*>(a)[blockIdx.x] = sh[0];
+        return;
     }
 #pragma unroll
     for (int u = 0; u < kStreamU; ++u) {
@@ -358,7 +375,7 @@ int fv_permute(fv_ctx* ctx, fv_prec prec, int64_t n, int ncomp, int mode, int or
 int fv_stream(fv_ctx* ctx, int mode, int64_t n, const double* d_b, const double* d_c, double* d_a,
               double s) {
     return guarded([&] {
-        FV_REQUIRE(ctx && n >= 0 && (mode == 0 || mode == 1), "bad argument");
+        FV_REQUIRE(ctx && n >= 0 && mode >= 0 && mode <= 2, "bad argument");
         FV_REQUIRE(n % 2 == 0, "n must be even (16-byte elements)");
         if (!n) return;
         FV_REQUIRE(d_a && d_b && (mode == 0 || d_c), "NULL pointer");
@@ -369,6 +386,9 @@ int fv_stream(fv_ctx* ctx, int mode, int64_t n, const double* d_b, const double*
         FV_REQUIRE(g < (int64_t(1) << 31), "array too long");
         if (mode == 0)
             k_stream<0><<<(unsigned)g, 256, 0, ctx->stream>>>(n2, (const double2*)d_b, nullptr,
+                                                           (double2*)d_a, s);
+        else if (mode == 2)
+            k_stream<2><<<(unsigned)g, 256, 0, ctx->stream>>>(n2, (const double2*)d_b, nullptr,
                                                            (double2*)d_a, s);
         else
             k_stream<1><<<(unsigned)g, 256, 0, ctx->stream>>>(n2, (const double2*)d_b,

@@ -97,7 +97,8 @@ int fv_fill_uniform(fv_ctx* ctx, fv_prec prec, void* d_ptr, int64_t n, double lo
                     uint64_t seed, uint64_t stream);
 
 /* Streaming reference kernels (n doubles, 16-byte aligned arrays, n even):
- * mode 0 copy d_a = d_b, mode 1 triad d_a = d_b + s * d_c.  No counterpart in
+ * mode 0 copy d_a = d_b, mode 1 triad d_a = d_b + s * d_c, mode 2 read (d_a
+ * receives one partial sum of d_b per 2048 doubles).  No counterpart in
  * the reference: the measured HBM ceiling of a kernel that only streams, which
  * the benchmark reports beside the face operators. */
 int fv_stream(fv_ctx* ctx, int mode, int64_t n, const double* d_b, const double* d_c, double* d_a,
@@ -443,7 +444,8 @@ int fv_batch_add(fv_batch* b, fv_linsys* ls, void* d_x, void* d_xold, int* id);
 int fv_batch_clear(fv_batch* b);
 int fv_batch_size(const fv_batch* b, int* n);
 int fv_batch_set_shape(fv_batch* b, int tile_rows, int chunk);
-/* stats: array of fv_batch_size entries (in insertion order) or NULL.
+/* stats: array of fv_batch_size entries (in insertion order) or NULL;
+ * stats.outer_iters counts the member's FP64 residual replacements.
  * opts->inner_rel_tol: FP32 residual drop that ends a chunk early (mixed;
  * 0: 1e-5).  A member has converged when its FP64 relative residual is at most
  * opts->rel_tol or, when opts->abs_tol > 0, its absolute residual at most
