@@ -6,13 +6,15 @@ Nothing in the product package imports this module.
 from __future__ import annotations
 
 import ctypes as C
+import os
 import subprocess
 from pathlib import Path
 
 import numpy as np
 
 ROOT = Path(__file__).resolve().parents[1]
-ORACLE_SO = ROOT / "oracle" / "_build" / "liboracle.so"
+# FV_ORACLE_SO selects another build of the same oracle (the sanitizer build)
+ORACLE_SO = Path(os.environ.get("FV_ORACLE_SO", ROOT / "oracle" / "_build" / "liboracle.so"))
 
 
 def _load():
