@@ -102,6 +102,7 @@ _SIGS = {
     "fv_convert": [_vp, C.c_int, _vp, C.c_int, _vp, _i64],
     "fv_fill_uniform": [_vp, C.c_int, _vp, _i64, _d, _d, _u64, _u64],
     "fv_stream": [_vp, C.c_int, _i64, _vp, _vp, _vp, _d],
+    "fv_component": [_vp, C.c_int, _i64, C.c_int, C.c_int, C.c_int, _vp, _vp],
     "fv_hexmesh_sizes": [_i32, _i32, _i32, _vp],
     "fv_hexmesh_generate": [_i32, _i32, _i32, _d, _d, _d, _d, _u64, _u64,
                             _vp, _vp, _vp, _vp, _vp, _vp, _vp],

@@ -142,6 +142,21 @@ __global__ void k_permute(int64_t n, int nc, int mode, int oriented,
     }
 }
 
+// one component of an interleaved field: mode 0 dst[i] = src[i*nc + k] (out),
+// mode 1 dst[i*nc + k] = src[i] (in)
+template <typename S>
+__global__ void k_component(int64_t n, int nc, int k, int mode, const S* __restrict__ src,
+                            S* __restrict__ dst) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t st = (int64_t)gridDim.x * blockDim.x;
+    for (; i < n; i += st) {
+        if (mode == 0)
+            dst[i] = src[i * nc + k];
+        else
+            dst[i * nc + k] = src[i];
+    }
+}
+
 // Streaming reference kernels: what a kernel that only streams HBM reaches on
 // this part (the ceiling the face operators are judged against).  Each thread
 // moves kStreamU 16-byte elements, lanes consecutive (coalesced), one pass:
@@ -368,6 +383,25 @@ int fv_permute(fv_ctx* ctx, fv_prec prec, int64_t n, int ncomp, int mode, int or
         else
             k_permute<float><<<g, 256, 0, ctx->stream>>>(n, ncomp, mode, oriented, map,
                                                          (const float*)src, (float*)dst);
+        FV_LAUNCH_CHECK();
+    });
+}
+
+int fv_component(fv_ctx* ctx, fv_prec prec, int64_t n, int ncomp, int comp, int mode,
+                 const void* d_src, void* d_dst) {
+    return guarded([&] {
+        FV_REQUIRE(ctx && n >= 0 && ncomp >= 1 && comp >= 0 && comp < ncomp &&
+                       (mode == 0 || mode == 1),
+                   "bad argument");
+        if (!n) return;
+        FV_REQUIRE(d_src && d_dst && d_src != d_dst, "NULL or aliased pointer");
+        const int g = grid_for(n, 256, 8192);
+        if (prec == FV_F64)
+            k_component<double><<<g, 256, 0, ctx->stream>>>(n, ncomp, comp, mode,
+                                                            (const double*)d_src, (double*)d_dst);
+        else
+            k_component<float><<<g, 256, 0, ctx->stream>>>(n, ncomp, comp, mode,
+                                                           (const float*)d_src, (float*)d_dst);
         FV_LAUNCH_CHECK();
     });
 }

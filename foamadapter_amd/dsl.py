@@ -15,7 +15,7 @@ from dataclasses import dataclass, field
 
 import numpy as np
 
-from ._lib import FV_LINEAR, FV_UPWIND, call
+from ._lib import FV_LINEAR, FV_UPWIND, call, lib
 from . import fvcc
 from .la import LinearSystem, SparsityPattern, SolverStats, solve_cg
 from .runtime import DeviceArray
@@ -120,9 +120,26 @@ def _linsys(mesh) -> LinearSystem:
     return ls
 
 
+def _on_component(expr: Expression, psi, psi_k) -> Expression:
+    """The expression with the field psi replaced by its component psi_k."""
+    def swap(a):
+        return psi_k if a is psi else a
+    return Expression([Term(t.kind, t.implicit, tuple(swap(a) for a in t.args), t.coeff, t.scheme)
+                       for t in expr.terms])
+
+
 def assemble(expr: Expression, psi: fvcc.VolumeField, t: float, dt: float,
              ls: LinearSystem | None = None) -> LinearSystem:
-    """Expression::assemble(t, dt, sparsity, ls) (expression.hpp:69-73)."""
+    """Expression::assemble(t, dt, sparsity, ls) (expression.hpp:69-73).
+
+    A Vec3 field (PDESolver<Vec3>, the momentum predictor of
+    examples/neoIcoFoam/neoIcoFoam.cpp:94-96) is assembled component by
+    component: the matrix, which the boundary-condition kinds shared by the
+    components make the same for all three, is that of component 0, and the
+    three right-hand sides are kept as ``ls.rhs3`` ([N, 3] device array); the
+    system's own right-hand side holds component 0."""
+    if psi.ncomp > 1:
+        return _assemble_vec(expr, psi, t, dt, ls)
     mesh = psi.mesh
     ctx = mesh.ctx
     ls = ls or _linsys(mesh)
@@ -162,6 +179,36 @@ def assemble(expr: Expression, psi: fvcc.VolumeField, t: float, dt: float,
     return ls
 
 
+def _assemble_vec(expr, psi, t, dt, ls):
+    mesh = psi.mesh
+    ctx = mesh.ctx
+    ls = ls or _linsys(mesh)
+    if psi.old_time is None and any(tm.kind == "ddt" for tm in expr.terms):
+        psi.old_time = fvcc.VolumeField(mesh, _copy(psi.internal), None)
+    rhs3 = ctx.empty((mesh.n_cells, psi.ncomp))
+    rhs = ctx.empty((mesh.n_cells,))
+    for k in reversed(range(psi.ncomp)):          # component 0 last: its system stays in ls
+        pk = psi.component(k)
+        assemble(_on_component(expr, psi, pk), pk, t, dt, ls)
+        call("fv_memcpy_d2d", ctx.handle, rhs.ptr, lib.fv_linsys_rhs(ls.handle), rhs.nbytes)
+        fvcc.set_component(rhs3, k, rhs)
+    ls.rhs3 = rhs3
+    return ls
+
+
+def _solve_vec(ls: LinearSystem, psi: fvcc.VolumeField, solver) -> list[SolverStats]:
+    """Each component of psi solved on the shared matrix with its right-hand side."""
+    ctx = psi.mesh.ctx
+    stats = []
+    for k in range(psi.ncomp):
+        b = fvcc.component(ls.rhs3, k)
+        call("fv_memcpy_d2d", ctx.handle, lib.fv_linsys_rhs(ls.handle), b.ptr, b.nbytes)
+        x = fvcc.component(psi.internal, k)
+        stats.append(solve_cg(ls, x, solver))
+        fvcc.set_component(psi.internal, k, x)
+    return stats
+
+
 def _copy(a: DeviceArray) -> DeviceArray:
     d = a.ctx.empty(a.shape, a.dtype)
     call("fv_memcpy_d2d", a.ctx.handle, d.ptr, a.ptr, a.nbytes)
@@ -176,7 +223,10 @@ def solve(expr: Expression, psi: fvcc.VolumeField, t: float = 0.0, dt: float = 1
     ls = assemble(expr, psi, t, dt)
     for fn in post_assembly:
         fn(ls)
-    stats = solve_cg(ls, psi.internal, solver)
+    if psi.ncomp > 1:
+        stats = _solve_vec(ls, psi, solver)
+    else:
+        stats = solve_cg(ls, psi.internal, solver)
     psi.correct_boundary_conditions()
     if any(tm.kind == "ddt" for tm in expr.terms):
         call("fv_memcpy_d2d", psi.mesh.ctx.handle, psi.old_time.internal.ptr, psi.internal.ptr,
@@ -207,7 +257,16 @@ class PDESolver:
     def solve(self) -> SolverStats:
         self.assemble()
         if self._ref is not None:
+            if self.psi.ncomp > 1:
+                raise ValueError("setReference applies to scalar equations")
             call("fv_set_reference", self.ls.handle, self._ref[0], self._ref[1])
+        if self.psi.ncomp > 1:
+            all_stats = _solve_vec(self.ls, self.psi, self.solver)
+            self.psi.correct_boundary_conditions()
+            print(f"[fvhip] Solving for {self.psi.name}: Final residuals "
+                  f"{[s.finalResNorm for s in all_stats]} No Iterations "
+                  f"{[s.numIter for s in all_stats]}")
+            return all_stats
         stats = solve_cg(self.ls, self.psi.internal, self.solver)
         self.psi.correct_boundary_conditions()
         print(f"[fvhip] Solving for {self.psi.name}: Initial residual: {stats.initResNorm} "

@@ -136,3 +136,41 @@ def read_field(path, patches: list[Patch], ncomp: int = 1):
         vals = _field_values(bf[p.name], "value", p.size, ncomp)
         out[p.name] = None if vals is None else vals.reshape(-1, ncomp) if ncomp > 1 else vals
     return (internal.reshape(-1, ncomp) if ncomp > 1 else internal), out
+
+
+def read_boundary_conditions(path, patches: list[Patch], ncomp: int = 1) -> dict:
+    """{patch: (kind, boundary values [size] / [size, 3])} of an ASCII field file,
+    mapped as FoamAdapter's readVolBoundaryConditions maps them
+    (include/FoamAdapter/auxiliary/readers.hpp:43-95): fixedValue keeps its value,
+    noSlip is fixedValue 0, zeroGradient is fixedGradient 0, fixedGradient keeps
+    its gradient, calculated / extrapolatedCalculated are calculated (value or
+    0).  Empty patches are skipped."""
+    text = _strip_header(Path(path).read_text())
+    top = _subdicts(text[text.index("boundaryField"):], path)
+    bf = _subdicts(top["boundaryField"], path)
+    out = {}
+    for p in patches:
+        if p.empty:
+            continue
+        if p.name not in bf:
+            raise ValueError(f"patch {p.name} missing in {path}")
+        body = bf[p.name]
+        m = re.search(r"\btype\s+([A-Za-z]+)\s*;", body)
+        if not m:
+            raise ValueError(f"patch {p.name} of {path} has no type")
+        t = m.group(1)
+        shape = (p.size,) if ncomp == 1 else (p.size, ncomp)
+        if t == "noSlip":
+            kind, vals = "fixedValue", np.zeros(shape)
+        elif t == "zeroGradient":
+            kind, vals = "fixedGradient", np.zeros(shape)
+        elif t == "fixedGradient":
+            kind, vals = t, _field_values(body, "gradient", p.size, ncomp)
+        elif t in ("fixedValue", "calculated", "extrapolatedCalculated"):
+            kind = "fixedValue" if t == "fixedValue" else "calculated"
+            vals = _field_values(body, "value", p.size, ncomp)
+            vals = np.zeros(shape) if vals is None else vals
+        else:
+            raise ValueError(f"boundary condition {t} of patch {p.name} is not supported")
+        out[p.name] = (kind, np.asarray(vals, np.float64).reshape(shape))
+    return out

@@ -51,9 +51,32 @@ class VolumeField:
             b = ctx.upload(np.asarray(boundary, dtype))
         return cls(mesh, internal, b, bcs, name)
 
+    @property
+    def ncomp(self) -> int:
+        """1 for a scalar field, 3 for a Vec3 field ([N, 3], xyz interleaved)."""
+        return 1 if len(self.internal.shape) == 1 else int(self.internal.shape[1])
+
+    def component(self, k: int) -> "VolumeField":
+        """Scalar copy of component k (internal, boundary, fixed gradient and
+        old-time values) with the same boundary conditions."""
+        def comp(a):
+            return None if a is None else component(a, k)
+        f = VolumeField(self.mesh, comp(self.internal), comp(self.boundary), self.bcs,
+                        f"{self.name}[{k}]", comp(self.boundary_gradient))
+        if self.old_time is not None:
+            f.old_time = VolumeField(self.mesh, comp(self.old_time.internal), None)
+        return f
+
     def correct_boundary_conditions(self):
-        """fixedGradient/zeroGradient faces: Tb = T_P + g/deltaCoeffs."""
+        """fixedGradient/zeroGradient faces: Tb = T_P + g/deltaCoeffs (each
+        component of a Vec3 field)."""
         if self.bcs is None or self.mesh.n_boundary_faces == 0:
+            return
+        if self.ncomp > 1:
+            for k in range(self.ncomp):
+                c = self.component(k)
+                c.correct_boundary_conditions()
+                set_component(self.boundary, k, c.boundary)
             return
         call("fv_correct_boundary", self.mesh.ctx.handle, self.mesh.handle, prec_of(self.dtype),
              self.internal.ptr, self.bcs.handle,
@@ -65,6 +88,20 @@ class VolumeField:
 
     def numpy(self) -> np.ndarray:
         return self.internal.numpy()
+
+
+def component(a: DeviceArray, k: int) -> DeviceArray:
+    """Component k of an interleaved [n, ncomp] device array, as a new [n] array."""
+    n, nc = a.shape[0], a.shape[1]
+    out = a.ctx.empty((n,), a.dtype)
+    call("fv_component", a.ctx.handle, prec_of(a.dtype), n, nc, k, 0, a.ptr, out.ptr)
+    return out
+
+
+def set_component(a: DeviceArray, k: int, src: DeviceArray):
+    """Component k of the interleaved [n, ncomp] device array a = src [n]."""
+    call("fv_component", a.ctx.handle, prec_of(a.dtype), a.shape[0], a.shape[1], k, 1, src.ptr,
+         a.ptr)
 
 
 class SurfaceField:

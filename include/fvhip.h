@@ -90,6 +90,13 @@ int fv_memcpy_d2d(fv_ctx* ctx, void* d_dst, const void* d_src, size_t bytes);
 int fv_fill(fv_ctx* ctx, fv_prec prec, void* d_ptr, int64_t n, double value);
 /* Convert between precisions on the device (n elements). */
 int fv_convert(fv_ctx* ctx, fv_prec to, void* d_dst, fv_prec from, const void* d_src, int64_t n);
+/* One component of an interleaved field of n items with ncomp components:
+ * mode 0 copies it out (dst[i] = src[i*ncomp + comp]), mode 1 in (dst[i*ncomp +
+ * comp] = src[i]).  A Vec3 equation (PDESolver<Vec3>, the momentum predictor of
+ * examples/neoIcoFoam/neoIcoFoam.cpp:94-96) is assembled and solved component
+ * by component on the shared matrix through it. */
+int fv_component(fv_ctx* ctx, fv_prec prec, int64_t n, int ncomp, int comp, int mode,
+                 const void* d_src, void* d_dst);
 /* Counter-based uniform random numbers in [lo,hi): element i gets
  * lo + (hi-lo)*u(seed, stream, i) with u = splitmix64-hash(...) >> 11 * 2^-53,
  * identical on host and device, so CPU checkers can regenerate any input. */

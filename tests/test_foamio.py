@@ -80,3 +80,21 @@ def test_write_from_renumbered_device_mesh(ctx, tmp_path):
     t2, b2 = read_field(p, poly.patches)
     assert np.array_equal(t2, T)
     assert np.array_equal(np.concatenate([b2[q.name] for q in poly.patches]), Tb)
+
+
+def test_boundary_conditions_of_the_coupling_case(fixtures_dir):
+    """readVolBoundaryConditions' mapping (readers.hpp:43-95) on the reference's
+    pressure-velocity case: noSlip -> fixedValue 0, fixedValue keeps its
+    (vector) value, zeroGradient -> fixedGradient 0."""
+    from foamadapter_amd.foamio import read_boundary_conditions
+    case = fixtures_dir / "setup_pressureVelocityCoupling"
+    poly = read_polymesh(case)
+    assert [p.name for p in poly.patches] == ["fixedWalls", "inlet", "outlet"]
+    u = read_boundary_conditions(case / "0" / "ofU", poly.patches, 3)
+    assert u["fixedWalls"][0] == "fixedValue" and not u["fixedWalls"][1].any()
+    assert u["inlet"][0] == "fixedValue"
+    assert np.array_equal(u["inlet"][1], np.tile([1.0, 0.0, 0.0], (9, 1)))
+    assert u["outlet"][0] == "fixedGradient" and u["outlet"][1].shape == (9, 3)
+    p = read_boundary_conditions(case / "0" / "ofp", poly.patches)
+    assert p["outlet"] == ("fixedValue", p["outlet"][1]) and np.all(p["outlet"][1] == 0.1)
+    assert p["fixedWalls"][0] == p["inlet"][0] == "fixedGradient"
