@@ -647,14 +647,17 @@ def bench_batched(ctx, P=512, seed=1004, steps=5):
         for p in probs:
             _lib.call("fv_memcpy_d2d", ctx.handle, p["x"].ptr, p["T0"].ptr, p["x"].nbytes)
             _lib.call("fv_memcpy_d2d", ctx.handle, p["xo"].ptr, p["T0"].ptr, p["x"].nbytes)
-        iters, nbytes = 0, 0
+        iters, nbytes, cgb = 0, 0, 0
         for s in range(steps):
             heat.assemble(-1.0)                           # all 512 systems, one launch
             stats = batch.solve(solver, use_graph=True)   # x_old <- x after the solve
             iters += sum(st.numIter for st in stats)
-            nbytes += sum(p["asm_b"] + cg_bytes(p["N"], p["nnz"], st.numIter, st.outerIter, True)
-                          for p, st in zip(probs, stats))
+            c = sum(cg_bytes(p["N"], p["nnz"], st.numIter, st.outerIter, True)
+                    for p, st in zip(probs, stats))
+            cgb += c
+            nbytes += c + sum(p["asm_b"] for p in probs)
         ctx.sync()
+        run_batched.cg_bytes = cgb
         return iters, nbytes
 
     for p in probs:
@@ -685,13 +688,23 @@ def bench_batched(ctx, P=512, seed=1004, steps=5):
     heat = la.HeatAssembly([(p["ls"], p["xo"], 1e-3, p["kap"], p["bc"], p["Tb"], None)
                             for p in probs])
     run_batched(batch, heat)                              # warm (graph capture)
+    ev0 = batch.timing()
     t = time.perf_counter()
     it_b, b_b = run_batched(batch, heat)
     wall_b = time.perf_counter() - t
+    ev1 = batch.timing()
+    ev_ms = (ev1["chunk_ms"] - ev0["chunk_ms"]) + (ev1["start_ms"] - ev0["start_ms"])
     errs = np.array([float(np.max(np.abs(p["x"].numpy() - xp)) / np.max(np.abs(xp)))
                      for p, xp in zip(probs, x_plain)])
     out["batched_mixed"] = {"wall_s": round(wall_b, 3), "solves_per_s": round(P * steps / wall_b, 1),
                             "cg_iters": it_b, "roofline": roof(b_b, wall_b * 1e3),
+                            "gpu_events": {"chunks": ev1["chunks"] - ev0["chunks"],
+                                           "chunk_ms": round(ev1["chunk_ms"] - ev0["chunk_ms"], 3),
+                                           "start_ms": round(ev1["start_ms"] - ev0["start_ms"], 3),
+                                           "solve_roofline": roof(run_batched.cg_bytes, ev_ms),
+                                           "what": "HIP events around every chunk graph launch "
+                                                   "and batched start (assembly and host work "
+                                                   "excluded)"},
                             "rel_diff_vs_plain_fp64": {"members": len(errs),
                                                        "max": float(errs.max()),
                                                        "median": float(np.median(errs))}}
@@ -801,7 +814,9 @@ def bench_served(n_req=96, seed=1005, rate=200.0, tile_rows=4096, chunk=16, rate
                               for k in ("admit_ms", "gpu_ms", "advance_ms")},
                 "chunks": s["chunks"] - srv_stats0["chunks"],
                 "table_builds": s["table_builds"] - srv_stats0["table_builds"],
-                "graph_builds": s["graph_builds"] - srv_stats0["graph_builds"]}, results
+                "graph_builds": s["graph_builds"] - srv_stats0["graph_builds"],
+                "gpu_event_ms": {k: round(s[k] - srv_stats0[k], 1)
+                                 for k in ("chunk_event_ms", "start_event_ms")}}, results
 
     results = {}
     plain, xp = run("plain", rate, True, precision="double", batched=False)

@@ -160,6 +160,7 @@ _SIGS = {
     "fv_batch_clear": [_vp],
     "fv_batch_size": [_vp, C.POINTER(C.c_int)],
     "fv_batch_set_shape": [_vp, C.c_int, C.c_int],
+    "fv_batch_timing": [_vp, _vp, _vp],
     "fv_batch_solve": [_vp, C.POINTER(FvSolveOpts), C.c_int, C.POINTER(FvSolverStats)],
     "fv_batch_autotune": [_vp, C.POINTER(FvSolveOpts), C.c_char_p, C.POINTER(C.c_int),
                           C.POINTER(C.c_int)],

@@ -41,6 +41,8 @@ constexpr int kB = kCgBlock;
 fv_batch::~fv_batch() {
     if (graphExec) (void)hipGraphExecDestroy(graphExec);
     if (graph) (void)hipGraphDestroy(graph);
+    for (hipEvent_t e : ev)
+        if (e) (void)hipEventDestroy(e);
     for (auto* m : members) delete m;
 }
 
@@ -271,13 +273,24 @@ void start_members(fv_batch* b) {
     const auto* g = (const CgGrid<S>*)b->startGridDev.p;
     const CgPartials red = partials_of<S>(b);
     const int G = cg_grid((int)st.size());
+    FV_HIP(hipEventRecord(b->ev[2], s));
     kc_start_tile<S><<<G, kB, 0, s>>>(g, (const CgStartParam*)b->startParams.p);
     kc_resid<S, 1><<<G, kB, 0, s>>>(g, red);
     kc_replace<S, 1><<<G, kB, 0, s>>>(g, red);
     kc_direction<S, 2><<<G, kB, 0, s>>>(g, red, 0);
     FV_LAUNCH_CHECK();
+    FV_HIP(hipEventRecord(b->ev[3], s));
     // the upload buffer must outlive the launches
     FV_HIP(hipStreamSynchronize(s));
+    float ms = 0.0f;
+    FV_HIP(hipEventElapsedTime(&ms, b->ev[2], b->ev[3]));
+    b->startMs += ms;
+    ++b->starts;
+}
+
+void make_events(fv_batch* b) {
+    for (hipEvent_t& e : b->ev)
+        if (!e) FV_HIP(hipEventCreate(&e));
 }
 
 template <typename S>
@@ -285,13 +298,20 @@ void step_t(fv_batch* b, std::vector<CgState>& out) {
     if (b->dirty) build_tables<S>(b);
     out.clear();
     if (b->members.empty()) return;
+    make_events(b);
     start_members<S>(b);
-    run_chunk<S>(b);
-    std::vector<CgState> all(b->slotCap);
     hipStream_t s = b->ctx->stream;
+    FV_HIP(hipEventRecord(b->ev[0], s));
+    run_chunk<S>(b);
+    FV_HIP(hipEventRecord(b->ev[1], s));
+    std::vector<CgState> all(b->slotCap);
     FV_HIP(hipMemcpyAsync(all.data(), b->states.p, b->slotCap * sizeof(CgState),
                           hipMemcpyDeviceToHost, s));
     FV_HIP(hipStreamSynchronize(s));
+    float ms = 0.0f;
+    FV_HIP(hipEventElapsedTime(&ms, b->ev[0], b->ev[1]));
+    b->chunkMs += ms;
+    ++b->chunks;
     for (BMember* m : b->members) out.push_back(all[m->slot]);
 }
 
@@ -425,6 +445,16 @@ int fv_batch_size(const fv_batch* b, int* n) {
     return guarded([&] {
         FV_REQUIRE(b && n, "NULL argument");
         *n = (int)b->members.size();
+    });
+}
+
+int fv_batch_timing(const fv_batch* b, double* h_ms2, int64_t* h_counts2) {
+    return guarded([&] {
+        FV_REQUIRE(b && h_ms2 && h_counts2, "NULL argument");
+        h_ms2[0] = b->chunkMs;
+        h_ms2[1] = b->startMs;
+        h_counts2[0] = b->chunks;
+        h_counts2[1] = b->starts;
     });
 }
 

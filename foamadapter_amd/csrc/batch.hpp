@@ -49,6 +49,11 @@ struct fv_batch {
     hipGraphExec_t graphExec = nullptr;
     int64_t graphBuilds = 0, tableBuilds = 0;  // counters (server statistics)
     bool useGraph = true;
+    // GPU time of the chunks (graph launch or kernel sequence) and of the
+    // batched starts, from HIP events on the batch's stream around each
+    int64_t chunks = 0, starts = 0;
+    double chunkMs = 0.0, startMs = 0.0;
+    hipEvent_t ev[4] = {};
     ~fv_batch();
 };
 

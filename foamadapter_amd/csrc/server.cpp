@@ -92,6 +92,7 @@ struct fv_server {
     Clock::time_point tFirst, tLast;
     double admitMs = 0, gpuMs = 0, advanceMs = 0;  // worker time split (under qmu when read)
     int64_t chunks = 0, tableBuilds = 0, graphBuilds = 0;  // copies of the batch's counters
+    double chunkEventMs = 0, startEventMs = 0;             // and of its event timings
     size_t peakBytes = 0;
     int peakActive = 0;
     size_t residentBytes = 0;  // meshes, their sparsity patterns and boundary conditions
@@ -397,6 +398,8 @@ void worker_loop(fv_server* s) {
             ++s->chunks;
             s->tableBuilds = s->batch->tableBuilds;
             s->graphBuilds = s->batch->graphBuilds;
+            s->chunkEventMs = s->batch->chunkMs;
+            s->startEventMs = s->batch->startMs;
         } catch (const Error& e) {
             // a device error fails everything in flight
             set_error(e.msg);
@@ -673,6 +676,8 @@ int fv_server_get_stats(fv_server* s, fv_server_stats* o) {
         o->chunks = s->chunks;
         o->table_builds = s->tableBuilds;
         o->graph_builds = s->graphBuilds;
+        o->chunk_event_ms = s->chunkEventMs;
+        o->start_event_ms = s->startEventMs;
         if (s->haveFirst && s->completed) o->wall_s = ms_between(s->tFirst, s->tLast) * 1e-3;
         if (o->wall_s > 0) o->solves_per_s = s->completed / o->wall_s;
         if (!s->latencies.empty()) {

@@ -12,6 +12,7 @@
 #include "cgkernels.hpp"
 
 #include <cmath>
+#include <vector>
 
 using namespace fvhip;
 
@@ -32,6 +33,7 @@ struct CgWork : fv_solver_cache {
     CgPartials red{};
     int G = 0;  // launch grid (multiple of 8)
     int lastIters = 0;  // iterations of the previous solve (time steps repeat them)
+    std::vector<int> lastRounds;  // FP32 iterations per round of the previous mixed solve
 
     CgWork(fv_ctx* c, const fv_linsys* ls, bool separateCorrection) {
         const fv_sparsity* sp = ls->sp;
@@ -169,84 +171,136 @@ CgWork<S>& work_of(fv_linsys* ls, bool mixed) {
     return *static_cast<CgWork<S>*>(slot.get());
 }
 
+// Iterations a CG run still needs for its relative residual to fall from r to
+// a target, at the rate (log-residual decrease per iteration) observed between
+// the two latest polls; -1 while no rate is known.
+struct ConvRate {
+    int it = -1;
+    double r = 0.0, rate = 0.0;
+    void observe(int it1, double r1) {
+        if (it >= 0 && it1 > it && r1 > 0.0 && r > r1) rate = std::log(r / r1) / (it1 - it);
+        it = it1;
+        r = r1;
+    }
+    int remaining(double r1, double target) const {
+        if (!(rate > 0.0) || !(r1 > 0.0)) return -1;
+        if (r1 <= target) return 0;
+        const double n = std::ceil(std::log(r1 / target) / rate);
+        return n > 1e6 ? 1000000 : (int)n;
+    }
+};
+
+// Extra iterations launched beyond a predicted stop (CG residuals are not
+// monotone, so the prediction errs both ways).
+constexpr int kPredictMargin = 2;
+
+// Chunks of iterations are queued with their state copied to pinned memory
+// behind an event; at most two such polls are in flight, so the device works on
+// one chunk while the host reads the state of the one before.  Once two polls
+// have given a convergence rate, each chunk is sized to end just after the
+// predicted stop (the iterations still missing minus those already queued), so
+// a solve overshoots its stop by a few no-op iterations, not by a whole chunk;
+// the first chunks ramp up 4, 8, 16, ...  Kernels of a stopped system are
+// no-ops (about 2 us each).
+struct PollPipe {
+    CgState* mirror;
+    fv_ctx* ctx;
+    int pend[2] = {-1, -1}, npend = 0, slot = 0;
+    void push(hipStream_t s, const CgState* dev) {
+        FV_HIP(hipMemcpyAsync(&mirror[slot], dev, sizeof(CgState), hipMemcpyDeviceToHost, s));
+        FV_HIP(hipEventRecord(ctx->pollEv[slot], s));
+        pend[npend++] = slot;
+        slot ^= 1;
+    }
+    CgState pop() {
+        const int q = pend[0];
+        FV_HIP(hipEventSynchronize(ctx->pollEv[q]));
+        pend[0] = pend[1];
+        --npend;
+        return mirror[q];
+    }
+};
+
 // Plain FP64 CG from x; stops on the recurrence residual estimate.  *true_res
-// receives the explicit FP64 residual of the result.
-//
-// Iterations follow the start without a round trip (a system converged at the
-// start turns them into no-ops).  Polls do not stall the stream: each chunk's
-// state is copied to pinned memory behind an event and read only after the
-// next chunk is queued, so the device never waits for the host; a solve ends
-// at most one chunk of no-op launches after the state reports a stop.  Where
-// the previous solve of this system (a time step before) says the solve should
-// be done, the poll blocks instead and brings the explicit residual with it,
-// so a solve that repeats its iteration count costs one synchronisation.
+// receives the explicit FP64 residual of the result.  A solve of a system whose
+// previous solve (a time step before) took k iterations first launches k at
+// once and reads the state with the explicit residual: one synchronisation when
+// the iteration count repeats.
 CgState run_cg(fv_ctx* ctx, CgWork<double>& w, double tol, int maxit, int check, double atol,
                double* true_res) {
     hipStream_t s = ctx->stream;
-    CgState* mirror = static_cast<CgState*>(ctx_poll_host(ctx));
     static_assert(2 * sizeof(CgState) <= fv_ctx::kPollBytes, "poll area too small");
+    PollPipe pp{static_cast<CgState*>(ctx_poll_host(ctx)), ctx};
     w.start(s, tol, maxit, 0.0, atol);
     FV_LAUNCH_CHECK();
     CgState h{}, t{};
-    int launched = 0, batch = std::min(w.lastIters > 0 ? w.lastIters : 4, std::max(check, 1));
-    int slot = 0, pending = -1;  // mirror slot of the poll in flight
-    bool stopped = false, haveTrue = false;
-    while (launched < maxit) {
-        const int n = std::min(batch, maxit - launched);
-        batch = std::min(std::max(2 * batch, 4), check);
+    int launched = 0;
+    bool haveTrue = false;
+    check = std::max(check, 1);
+    if (w.lastIters > 0) {
+        const int n = std::min(w.lastIters, maxit);
         w.chunk(s, n);
         FV_LAUNCH_CHECK();
-        launched += n;
-        if ((w.lastIters > 0 && launched >= w.lastIters) || launched >= maxit) {
-            const bool withTrue = w.lastIters > 0 && launched >= w.lastIters;
-            if (withTrue) w.enqueue_true_residual(s);
-            h = w.state2(s, &t);
-            haveTrue = withTrue;
-            pending = -1;
-            if (h.stop) {
-                stopped = true;
-                break;
-            }
-            continue;
-        }
-        FV_HIP(hipMemcpyAsync(&mirror[slot], w.st.p, sizeof(CgState), hipMemcpyDeviceToHost, s));
-        FV_HIP(hipEventRecord(ctx->pollEv[slot], s));
-        if (pending >= 0) {
-            FV_HIP(hipEventSynchronize(ctx->pollEv[pending]));
-            if (mirror[pending].stop) {  // the chunk queued since ran as no-ops
-                FV_HIP(hipEventSynchronize(ctx->pollEv[slot]));
-                h = mirror[slot];
-                haveTrue = false;
-                stopped = true;
-                break;
-            }
-        }
-        pending = slot;
-        slot ^= 1;
+        launched = n;
+        w.enqueue_true_residual(s);
+        h = w.state2(s, &t);
+        haveTrue = true;
     }
-    if (!stopped) h = w.state(s);
+    ConvRate cr;
+    cr.observe(h.it, h.res);
+    int ramp = 4;
+    while (!h.stop) {
+        bool queued = false;
+        if (pp.npend < 2 && launched < maxit) {
+            const double target = std::max(tol, atol > 0.0 ? atol / h.bnorm : 0.0);
+            const int R = h.it > 0 ? cr.remaining(h.res, target) : -1;
+            int need;
+            if (R < 0) {
+                need = ramp;
+                ramp = std::min(2 * ramp, check);
+            } else {
+                need = R + kPredictMargin - (launched - h.it);
+            }
+            if (pp.npend == 0) need = std::max(need, 1);
+            need = std::min(std::min(need, check), maxit - launched);
+            if (need > 0) {
+                w.chunk(s, need);
+                FV_LAUNCH_CHECK();
+                launched += need;
+                pp.push(s, w.st.p);
+                queued = true;
+                haveTrue = false;
+            }
+        }
+        if (pp.npend == 0) break;               // the iteration limit is launched and read
+        if (queued && pp.npend < 2) continue;   // keep a second chunk queued before waiting
+        h = pp.pop();
+        cr.observe(h.it, h.res);
+    }
+    while (pp.npend) h = pp.pop();  // chunks queued behind the stop ran as no-ops
     // a system that needed no iteration polls (blocking) after the first
-    w.lastIters = stopped ? std::max(h.it, 1) : h.it;
+    w.lastIters = h.stop ? std::max(h.it, 1) : h.it;
     *true_res = haveTrue ? t.res : w.true_residual(s);
     return h;
 }
 
 // ---- mixed precision: FP32 CG with FP64 residual replacement ---------------
 //
-// The FP32 CG accumulates its correction d (FP32) for a chunk of iterations; a
-// chunk ends after `chunk` iterations, or earlier when the FP32 residual has
-// dropped by `delta` since the last replacement or its estimate reaches the
-// tolerance.  Then, in FP64: x += d, r = b - A x, ||r||/||b|| tested against the
-// tolerance; the FP32 residual is replaced by the true one and CG continues
-// with its search direction (beta from the replaced residual), so the Krylov
-// space is not restarted.
+// The FP32 CG accumulates its correction d (FP32); it stops when the FP32
+// residual has dropped by `delta` since the last replacement, its estimate
+// reaches the tolerance, or the iteration limit.  A round is a chunk of FP32
+// iterations followed by the replacement kernels, which act only on a stopped
+// CG: in FP64 x += d, r = b - A x, the convergence test on ||r||/||b||; the
+// FP32 residual is replaced by the true one and CG continues with its search
+// direction (beta from the replaced residual), so the Krylov space is not
+// restarted.
 //
-// A round is a chunk of FP32 iterations followed by the replacement (a no-op
-// unless the FP32 CG stopped in the chunk); the host reads the state after
-// each round.  (Queuing the next round before reading, as run_cg does, was
-// measured slower on short solves: on the 32^3 heat step, 5 iterations to
-// 1e-10, the no-op round behind the last one cost more than the read it hid,
-// 0.65 vs 0.49 ms, and gained 1 % at 128^3.)
+// Rounds are queued and read like run_cg's chunks (two polls in flight), and
+// once a rate is known each round is sized to end just after the predicted
+// stop of the FP32 CG (drop trigger or tolerance), where its replacement
+// follows.  The rounds of a solve are recorded: the next solve of the same
+// system (a time step) queues the same rounds at once and reads the state
+// once, continuing adaptively only if that did not converge.
 struct IrOut {
     double init_res, res, bnorm;
     int iters, outer;  // outer: FP64 residual replacements
@@ -255,21 +309,78 @@ struct IrOut {
 IrOut run_mixed(fv_ctx* ctx, CgWork<float>& w, double tol, int maxit, int chunk, double delta,
                 int max_outer, double atol) {
     hipStream_t s = ctx->stream;
+    PollPipe pp{static_cast<CgState*>(ctx_poll_host(ctx)), ctx};
     w.start(s, tol, maxit, delta, atol);
     FV_LAUNCH_CHECK();
-    CgState h{};  // the first chunk follows the start without a round trip
-    int batch = std::min(8, chunk), rounds = 0;
-    while (h.stop < 2 && h.it < maxit && rounds < max_outer) {
-        // short chunks first (short solves end early), then `chunk` iterations
-        const int n = std::min(batch, maxit - h.it);
-        batch = std::min(2 * batch, chunk);
+    CgState h{};
+    int launched = 0, rounds = 0;
+    // iterations that ran in each round (no-op launches behind a stop do not
+    // count): what the next solve replays
+    std::vector<int> sched;
+    int done = 0;  // iterations accounted in sched
+    auto round = [&](int n) {
         w.chunk(s, n);
         w.replacement(s);  // only once the FP32 CG stopped (trigger, estimate, limit)
         FV_LAUNCH_CHECK();
-        h = w.state(s);
+        launched += n;
         ++rounds;
+    };
+    chunk = std::max(chunk, 1);
+    if (!w.lastRounds.empty()) {
+        int replayed = 0;
+        for (int n : w.lastRounds) {
+            if (launched >= maxit || rounds >= max_outer) break;
+            round(std::min(n, maxit - launched));
+            ++replayed;
+        }
+        h = w.state(s);
+        // the replayed rounds keep their sizes; the last one ends where the state is
+        for (int i = 0; i + 1 < replayed; ++i) {
+            sched.push_back(w.lastRounds[i]);
+            done += w.lastRounds[i];
+        }
+        if (h.it > done) sched.push_back(h.it - done);
+        done = h.it;
     }
+    ConvRate cr;
+    cr.observe(h.it, h.res);
+    int ramp = 8;
+    while (h.stop < 2 && h.it < maxit) {
+        bool queued = false;
+        if (pp.npend < 2 && launched < maxit && rounds < max_outer) {
+            // where the FP32 CG will stop: the drop trigger or the tolerance
+            double target = std::max(tol, atol > 0.0 && h.bnorm > 0.0 ? atol / h.bnorm : 0.0);
+            if (h.bnorm > 0.0 && h.rrep > 0.0) target = std::max(target, delta * h.rrep / h.bnorm);
+            const int R = h.it > 0 ? cr.remaining(h.res, target) : -1;
+            int need;
+            if (R < 0) {
+                need = ramp;
+                ramp = std::min(2 * ramp, chunk);
+            } else {
+                need = R + kPredictMargin - (launched - h.it);
+            }
+            if (pp.npend == 0) need = std::max(need, 1);
+            need = std::min(std::min(need, chunk), maxit - launched);
+            if (need > 0) {
+                round(need);
+                pp.push(s, w.st.p);
+                queued = true;
+            }
+        }
+        if (pp.npend == 0) {
+            if (launched >= maxit || rounds >= max_outer) break;
+            continue;
+        }
+        if (queued && pp.npend < 2) continue;
+        h = pp.pop();
+        cr.observe(h.it, h.res);
+        if (h.it > done) sched.push_back(h.it - done);  // one poll per round, in order
+        done = std::max(done, h.it);
+    }
+    while (pp.npend) h = pp.pop();
     if (rounds == 0) h = w.state(s);
+    // the schedule of a converged solve is what the next solve replays
+    if (h.stop == 2) w.lastRounds = sched;
     return IrOut{h.res0, h.res, h.bnorm, h.it, h.nrep};
 }
 

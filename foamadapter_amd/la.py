@@ -168,8 +168,8 @@ def mapFvSolution(solver: dict) -> dict:
     pre = d["preconditioner"]
     if isinstance(pre, dict):
         if isinstance(pre.get("type"), dict):
-            raise ValueError("GAMG is not supported in FoamAdapter, please use a different "
-                             "preconditioner.")
+            raise ValueError("a multigrid (GAMG) preconditioner cannot be mapped: choose DIC "
+                             "(point Jacobi) for this solver")
     elif pre in PRECONDITIONER_MAP:
         d["preconditioner"] = dict(PRECONDITIONER_MAP[pre])
     crit = dict(d.get("criteria", {}))
@@ -252,6 +252,14 @@ class Batch:
 
     def set_shape(self, tile_rows: int, chunk: int):
         call("fv_batch_set_shape", self.handle, int(tile_rows), int(chunk))
+
+    def timing(self) -> dict:
+        """GPU milliseconds and counts of the chunks and batched starts so far
+        (HIP events on the batch's stream)."""
+        ms, n = np.zeros(2), np.zeros(2, np.int64)
+        call("fv_batch_timing", self.handle, C.c_void_p(ms.ctypes.data), C.c_void_p(n.ctypes.data))
+        return {"chunk_ms": float(ms[0]), "start_ms": float(ms[1]), "chunks": int(n[0]),
+                "starts": int(n[1])}
 
     def autotune(self, solver: dict | None = None, cache_path: str | None = None) -> tuple[int, int]:
         """Pick tile rows and chunk length by timing solves (cached per size bucket)."""

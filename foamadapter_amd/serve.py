@@ -47,7 +47,8 @@ class FvServerStats(C.Structure):
                 ("mean_ms", C.c_double), ("max_ms", C.c_double), ("peak_bytes", C.c_size_t),
                 ("peak_active", C.c_int32), ("resident_bytes", C.c_size_t),
                 ("admit_ms", C.c_double), ("gpu_ms", C.c_double), ("advance_ms", C.c_double),
-                ("chunks", C.c_int64), ("table_builds", C.c_int64), ("graph_builds", C.c_int64)]
+                ("chunks", C.c_int64), ("table_builds", C.c_int64), ("graph_builds", C.c_int64),
+                ("chunk_event_ms", C.c_double), ("start_event_ms", C.c_double)]
 
 
 HEAT, POISSON = 0, 1

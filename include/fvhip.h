@@ -459,6 +459,12 @@ int fv_batch_set_shape(fv_batch* b, int tile_rows, int chunk);
  * opts->abs_tol (as fv_solve_cg). */
 int fv_batch_solve(fv_batch* b, const fv_solve_opts* opts, int use_graph,
                    fv_solver_stats* stats);
+/* GPU time the batch has spent so far, from HIP events on its stream:
+ * h_ms2 = {chunks (the replayed chunk graphs or kernel sequences, each with
+ * its FP64 residual replacement), batched starts}, h_counts2 = {chunks run,
+ * starts run}.  No counterpart in the reference: kernel-level timing of the
+ * batched and served solves without a profiler. */
+int fv_batch_timing(const fv_batch* b, double* h_ms2, int64_t* h_counts2);
 /* Autotuner: times full batch solves for tile_rows in {256..4096} x chunk in
  * {8,16,32} from the batch's current x (restored after every trial), applies
  * the fastest shape and records it in the JSON file cache_path (may be NULL)
@@ -524,6 +530,9 @@ typedef struct {
      * built and chunk graphs captured (batched) */
     double admit_ms, gpu_ms, advance_ms;
     int64_t chunks, table_builds, graph_builds;
+    /* batched: GPU time of the chunks and of the batched starts from HIP events
+     * around each (the part of gpu_ms the device was working) */
+    double chunk_event_ms, start_event_ms;
 } fv_server_stats;
 int fv_server_create(int device, const fv_server_opts* opts, fv_server** out);
 /* Adds a resident mesh (renumber != 0: Z-order renumbered internally). */
