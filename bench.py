@@ -584,13 +584,15 @@ def _loguniform(u, lo, hi):
     return np.exp(np.log(lo) + u * (np.log(hi) - np.log(lo)))
 
 
-def bench_batched(ctx, P=512, seed=1004, steps=5):
+def bench_batched(ctx, P=512, seed=1004, steps=5, use_graph=True):
     """512 independent heat problems on cubic meshes n ~ logU[12, 96], kappa ~
     logU[1e-4, 1e-1], dt = 1e-3, T0 ~ U[300,400], walls ~ U[300,400], 5 Euler
     steps, to 1e-8.  Plain: one problem at a time, FP64 fv_solve_cg per step.
     New: per step, the 512 assemblies in one launch (fv_assemble_ddt_laplacian_many)
     and all solves in one batched launch sequence (FP32 CG with FP64 residual
-    replacement, hipGraph chunks, autotuned tile shape)."""
+    replacement, hipGraph chunks, autotuned tile shape).  use_graph=False launches
+    the chunks' kernels one by one with the default shape (no autotuning, which
+    replays graphs): the same kernels, for kernel-level profiling."""
     import foamadapter_amd as fa
     from foamadapter_amd import _lib, la
     from foamadapter_amd.la import solver_opts
@@ -650,7 +652,7 @@ def bench_batched(ctx, P=512, seed=1004, steps=5):
         iters, nbytes, cgb = 0, 0, 0
         for s in range(steps):
             heat.assemble(-1.0)                           # all 512 systems, one launch
-            stats = batch.solve(solver, use_graph=True)   # x_old <- x after the solve
+            stats = batch.solve(solver, use_graph=use_graph)   # x_old <- x after the solve
             iters += sum(st.numIter for st in stats)
             c = sum(cg_bytes(p["N"], p["nnz"], st.numIter, st.outerIter, True)
                     for p, st in zip(probs, stats))
@@ -682,8 +684,12 @@ def bench_batched(ctx, P=512, seed=1004, steps=5):
         _lib.call("fv_assemble_ddt_laplacian", p["ls"].handle, p["xo"].ptr, 1e-3, p["kap"].ptr,
                   p["bc"].handle, p["Tb"].ptr, None, -1.0)
     t = time.perf_counter()
-    shape = batch.autotune(solver, str(ROOT / "foamadapter_amd" / "batch_tuning.json"))
-    out["autotune"] = {"tile_rows": shape[0], "chunk": shape[1],
+    if use_graph:
+        shape = batch.autotune(solver, str(ROOT / "foamadapter_amd" / "batch_tuning.json"))
+    else:
+        shape = (4096, 16)
+        batch.set_shape(*shape)
+    out["autotune"] = {"tile_rows": shape[0], "chunk": shape[1], "graphs": use_graph,
                        "seconds": round(time.perf_counter() - t, 2)}
     heat = la.HeatAssembly([(p["ls"], p["xo"], 1e-3, p["kap"], p["bc"], p["Tb"], None)
                             for p in probs])
@@ -858,6 +864,9 @@ def main():
     ap.add_argument("--batched", dest="batched", action="store_true",
                     help="run the batched heat workload (the default)")
     ap.add_argument("--batch-problems", type=int, default=512)
+    ap.add_argument("--batch-no-graph", action="store_true",
+                    help="batched leg without hipGraph replay and autotuning (for kernel "
+                         "profiling)")
     ap.add_argument("--no-served", dest="served", action="store_false",
                     help="skip the served request stream")
     ap.add_argument("--served", dest="served", action="store_true",
@@ -959,7 +968,8 @@ def main():
         rec["implicit_poisson"] = bench_poisson(ctx, cpu=not args.no_cpu)
     if args.batched:
         log("batched heat solves")
-        rec["batched_heat"] = bench_batched(ctx, P=args.batch_problems)
+        rec["batched_heat"] = bench_batched(ctx, P=args.batch_problems,
+                                            use_graph=not args.batch_no_graph)
     if args.served:
         log("served request stream")
         rec["served"] = bench_served(n_req=args.requests, rate=args.rate,
