@@ -96,13 +96,17 @@ def measured_traffic(kernel, n):
     m = re.search(r"kernel sources sha256\[:16\]: `([0-9a-f]{16})`", text)
     if not m or m.group(1) != kernel_stamp():
         return None, "the committed traffic profile measured other kernel sources"
+    # rows: | `kernel symbol` | op prec | mean us | alg GB | alg TB/s | fetch GB | write GB | ...
+    op = kernel.rsplit("_", 1)
+    want = f"{op[0]} {op[1]}"
     for line in text.splitlines():
         cells = [c.strip() for c in line.strip().strip("|").split("|")]
-        if len(cells) >= 4 and cells[0] == f"`{kernel}`":
+        if len(cells) >= 7 and cells[1] == want:
             try:
-                return int(float(cells[3]) * 1e9), f"{TRAFFIC_PROFILE.relative_to(ROOT)} ({m.group(1)})"
+                hbm = float(cells[5]) + float(cells[6])
             except ValueError:
                 return None, "unreadable profile row"
+            return int(hbm * 1e9), f"{TRAFFIC_PROFILE.relative_to(ROOT)} ({m.group(1)})"
     return None, "kernel not in the traffic profile"
 
 
