@@ -885,6 +885,8 @@ def main():
                          "are measured at saturation")
     ap.add_argument("--rate-low", type=float, default=40.0,
                     help="a second, lower arrival rate of the served stream (0: skip)")
+    ap.add_argument("--gather-run", type=int, default=0,
+                    help="bricks per XCD run of the staged gathers (0: the library default)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="first copy 2 GiB device-to-device 3 times (a known-size reference "
                          "for hardware traffic counters)")
@@ -910,6 +912,8 @@ def main():
     poly = fa.PolyMesh.hex(args.n, jitter=0.2, jitter_seed=args.seed, perm_seed=args.seed)
     log(f"hex mesh {args.n}^3 generated in {time.time() - t0:.1f}s")
     case = Case(ctx, poly, args.seed, renumber=True)
+    if args.gather_run:
+        case.mesh.set_gather_run(args.gather_run)
     log(f"renumbered device mesh + fields ready (mesh setup {case.setup_s:.1f}s)")
     sweep = time_sweep(ctx, case, args.steps, args.warmup)
     N, F, B = case.mesh.n_cells, case.mesh.n_internal_faces, case.mesh.n_boundary_faces

@@ -114,6 +114,7 @@ _SIGS = {
     "fv_mesh_sizes": [_vp, _vp],
     "fv_mesh_get": [_vp, C.c_int, _vp],
     "fv_mesh_set_gather": [_vp, C.c_int],
+    "fv_mesh_set_gather_run": [_vp, C.c_int],
     "fv_mesh_get_gather": [_vp, C.POINTER(C.c_int)],
     "fv_mesh_gather_stats": [_vp, C.c_int, _vp],
     "fv_bc_create": [_vp, _vp, _pp],

@@ -172,9 +172,24 @@ __global__ __launch_bounds__(kBlock) void k_laplacian(
 // result is the same bit for bit.  On a Z-order renumbered mesh a workgroup is a
 // compact brick (8x8x4, 8x8x8 or 16x8x8 cells), and only the faces on its lower
 // sides are external.
-// Workgroups run in their natural order, which the hardware deals round-robin
-// over the 8 XCDs: measured at 256^3 this is as fast as or faster (FP64
-// gradient +7%) than remapping each XCD onto one contiguous range of bricks.
+// The hardware deals workgroups round-robin over the 8 XCDs (block h runs on
+// XCD h % 8), each with its own L2.  With `run` > 1 the block index is mapped
+// so that each XCD takes runs of `run` consecutive bricks (XCD x's s-th block
+// gets brick ((s / run) * 8 + x) * run + s % run): a brick's lower neighbours
+// in Z-order (bricks b-1, b-2, b-4, ...), whose faces and cells it reads
+// again, then ran shortly before on the same XCD and are still in its L2.
+// Measured at 256^3, runs of 8 cut the L2 misses of the gathers by 6-18% at
+// the same speed; run = 1 is the natural order.
+__device__ __forceinline__ int brick_of(int h, int run) {
+    if (run <= 1) return h;
+    const int x = h & 7, s = h >> 3;
+    return ((s / run) * 8 + x) * run + s % run;
+}
+inline int gather_grid(int nBlocks, int run) {
+    if (run <= 1) return nBlocks;
+    const int g = 8 * run;
+    return (nBlocks + g - 1) / g * g;
+}
 
 // LDS face slots of a CELLS-cell workgroup: 3 owned faces per hex cell plus slack;
 // faces beyond it (unusual meshes) are recomputed in phase 2
@@ -384,7 +399,7 @@ static_assert(kMaxLow == 8, "sort8 sorts the per-cell lower-face lists");
 // workgroup whose faces do not fit the LDS arrays takes the per-cell path of
 // gather_cell instead; both give the same bits.
 template <typename S, typename Op, int CELLS, int THREADS>
-__global__ __launch_bounds__(THREADS) void k_gather(int N, int nBlocks, Split t,
+__global__ __launch_bounds__(THREADS) void k_gather(int N, int nBlocks, int run, Split t,
                                                     Lists L, ExGeo<S> xg, Op op,
                                                     const S* __restrict__ V, S coeff,
                                                     S* __restrict__ out) {
@@ -400,7 +415,7 @@ __global__ __launch_bounds__(THREADS) void k_gather(int N, int nBlocks, Split t,
     __shared__ uint16_t lowL[kMaxLow][CELLS];  // slot-major: lanes read consecutive cells
     __shared__ int lowN[CELLS];
     __shared__ int overflow;
-    const int lb = (int)blockIdx.x;
+    const int lb = brick_of((int)blockIdx.x, run);
     if (lb >= nBlocks) return;
     const int c0 = lb * CELLS, c1 = min(c0 + CELLS, N);
     const int fs = t.ownStart[c0], fe = t.ownStart[c1];
@@ -576,8 +591,8 @@ void launch_gather_t(fv_ctx* ctx, const fv_mesh* m, const Op& op, const S* V, S 
         xg = {gl.exW.p, gl.exMagSf.p, gl.exNodc.p, gl.exSf.p};
     else
         xg = {gl.exW32.p, gl.exMagSf32.p, gl.exNodc32.p, gl.exSf32.p};
-    k_gather<S, Op, CELLS, THREADS><<<nBlocks, THREADS, 0, ctx->stream>>>(
-        m->nCells, nBlocks, split_of(m), L, xg, op, V, coeff, out);
+    k_gather<S, Op, CELLS, THREADS><<<gather_grid(nBlocks, m->gatherRun), THREADS, 0, ctx->stream>>>(
+        m->nCells, nBlocks, m->gatherRun, split_of(m), L, xg, op, V, coeff, out);
     FV_LAUNCH_CHECK();
 }
 

@@ -587,6 +587,13 @@ int fv_mesh_set_gather(fv_mesh* m, int mode) {
     });
 }
 
+int fv_mesh_set_gather_run(fv_mesh* m, int run) {
+    return guarded([&] {
+        FV_REQUIRE(m && run >= 1 && run <= 4096, "run must be in [1, 4096]");
+        m->gatherRun = run;
+    });
+}
+
 int fv_mesh_get_gather(const fv_mesh* m, int* mode) {
     return guarded([&] {
         FV_REQUIRE(m && mode, "NULL argument");

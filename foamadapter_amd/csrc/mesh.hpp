@@ -76,6 +76,7 @@ struct fv_mesh {
     //   bOff[N+1], bFace[B]: boundary faces of c, ascending
     bool upperTri = false;
     int gatherMode = 0;  // cell-result kernels: 0 default, 1 stencil walk, 2/3/4 staged 256/512/1024
+    int gatherRun = 8;   // staged gather: consecutive bricks per XCD (1: natural order)
     fvhip::DBuf<int32_t> ownStart, lowOff, lowFace, lowOwn, bOff, bFace;
     GatherLists glists[3];  // workgroups of 256, 512, 1024 cells (built on first use)
     MeshF32 f32;
@@ -99,7 +100,7 @@ const GatherLists& gather_lists(fv_mesh* m, int cells, bool f32 = false);
 // Cells per workgroup of the LDS-staged gather for the cell-result operators,
 // 0 when the per-cell stencil walk runs instead (faces not in upper-triangular
 // order, or forced by fv_mesh_set_gather); kDefaultGatherCells stands for the
-// per-operator default (the gradient 512, divergence and laplacian 256).
+// per-operator default of the staged gather (its Op::kDefaultCells).
 constexpr int kDefaultGatherCells = 512;
 inline int gather_cells(const fv_mesh* m) {
     if (!m->upperTri) return 0;

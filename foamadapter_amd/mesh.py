@@ -299,9 +299,15 @@ class Mesh:
         return dst
 
     def set_gather(self, mode: int):
-        """0: LDS-staged face blocks when possible (default); 1: per-cell stencil walk;
-        2/3/4: staged with 256/512 (the default)/1024 cells per workgroup."""
+        """0: LDS-staged face blocks when possible, each operator's default workgroup
+        size (default); 1: per-cell stencil walk; 2/3/4: staged with 256/512/1024
+        cells per workgroup."""
         call("fv_mesh_set_gather", self.handle, int(mode))
+
+    def set_gather_run(self, run: int):
+        """Consecutive bricks each XCD takes in the staged gather (default 8; 1: the
+        natural order).  The results are the same bits for any run."""
+        call("fv_mesh_set_gather_run", self.handle, int(run))
 
     @property
     def gather_kernel(self) -> str:

@@ -228,6 +228,11 @@ int fv_mesh_get(const fv_mesh* mesh, fv_geom which, void* h_out);
  * counterpart in the reference: a tuning knob. */
 int fv_mesh_set_gather(fv_mesh* mesh, int mode);
 int fv_mesh_get_gather(const fv_mesh* mesh, int* mode);
+/* Order in which the staged gather's workgroups take the mesh's bricks: each
+ * of the 8 XCDs takes runs of `run` consecutive bricks (default 8; 1 is the
+ * natural order, brick b on XCD b % 8).  Results are the same bits for any
+ * run.  No counterpart in the reference: a tuning knob. */
+int fv_mesh_set_gather_run(fv_mesh* mesh, int run);
 /* How the staged gather with `cells` cells per workgroup (256, 512 or 1024)
  * runs on this mesh (its lists are built if needed): h_counts2 = {workgroups,
  * workgroups whose faces exceed its LDS lists and take the per-cell path --
