@@ -62,6 +62,7 @@ def updateVelocity(hByA: fvcc.VolumeField, rAU: fvcc.VolumeField, p: fvcc.Volume
     gradP.close()
 
 
-def flux(U: fvcc.VolumeField) -> fvcc.SurfaceField:
-    """phi = Sf . interpolate(U) with boundary Sf_b . U_b (:213-267)."""
-    return fvcc.flux(U.mesh.ctx, U.mesh, U)
+def flux(U: fvcc.VolumeField, out: fvcc.SurfaceField | None = None) -> fvcc.SurfaceField:
+    """phi = Sf . interpolate(U) with boundary Sf_b . U_b (:213-267), into out
+    when given."""
+    return fvcc.flux(U.mesh.ctx, U.mesh, U, out)
