@@ -22,12 +22,15 @@ Jacobi-CG to 1e-10; the steps run twice and the second run is timed, the first
 reported beside it) and the pressure-Poisson case (128^3 jittered, permuted,
 FP64 CG vs FP32-inner/FP64-refined CG to 1e-8).
 
-Also reported by default: 512 batched heat problems (batched_heat) and the
+Also reported by default: 512 batched heat problems (batched_heat), the
 served request stream (served: solves/s, p50/p99 latency, against plain FP64
-per-request solves); --no-batched / --no-served skip them.
+per-request solves) and the neoIcoFoam PISO loop on a 64^3 lid-driven cavity
+(icofoam_cavity: ms per time step); --no-batched / --no-served / --no-icofoam
+skip them.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 256] [--no-cpu]
                        [--no-implicit] [--no-as-given] [--no-batched] [--no-served]
+                       [--no-icofoam]
 """
 from __future__ import annotations
 
@@ -747,6 +750,57 @@ def _uniform_host(n, lo, hi, seed, stream):
     return lo + (hi - lo) * v
 
 
+def bench_icofoam(ctx, n=64, seed=1006, warmup=2, steps=5):
+    """neoIcoFoam (examples/neoIcoFoam/neoIcoFoam.cpp:80-180) on a lid-driven
+    cavity: n^3 jittered, permuted mesh renumbered at import, lid (patch y1)
+    U = (1, 0, 0), walls no-slip, nu = 0.01, dt = 0.5 h (Co about 0.5), upwind
+    convection, momentum predictor and two pressure correctors, the solver
+    settings of OpenFOAM's cavity tutorial (U: PBiCGStab to an absolute 1e-5;
+    p: PCG to 1e-6 absolute or 0.05 relative).  Timed: `steps` whole time steps
+    after `warmup`, host loop included (per step: three momentum solves, two
+    pressure solves, the coupling kernels, the Courant number and the
+    continuity error read back)."""
+    import contextlib
+    import foamadapter_amd as fa
+    from foamadapter_amd import fvcc
+    from foamadapter_amd.piso import IcoFoam
+    poly = fa.PolyMesh.hex(n, jitter=0.2, jitter_seed=seed, perm_seed=seed)
+    m = fa.Mesh.from_poly(ctx, poly, renumber=True)
+    del poly
+    N, F, B = m.n_cells, m.n_internal_faces, m.n_boundary_faces
+    po = m.get("patchOffsets")
+    Ub = np.zeros((B, 3))
+    Ub[po[3]:po[4], 0] = 1.0
+    U = fvcc.VolumeField(m, ctx.zeros((N, 3)), ctx.upload(Ub),
+                         fa.BoundaryConditions(m, ["fixedValue"] * 6), "U")
+    p = fvcc.VolumeField(m, ctx.zeros((N,)), ctx.zeros((B,)),
+                         fa.BoundaryConditions(m, ["zeroGradient"] * 6), "p")
+    phi = fvcc.flux(ctx, m, U)
+    nu = fvcc.SurfaceField(m, ctx.upload(np.full(F + B, 0.01)), "nu")
+    sol = IcoFoam(U, p, phi, nu, 0.5 / n, fvSchemes={"div": "Gauss upwind"},
+                  fvSolution={"U": {"solver": "PBiCGStab", "tolerance": 1e-5, "relTol": 0.0},
+                              "p": {"solver": "PCG", "tolerance": 1e-6, "relTol": 0.05},
+                              "PISO": {"nCorrectors": 2, "pRefCell": 0, "pRefValue": 0}})
+    with contextlib.redirect_stdout(sys.stderr):      # the solvers' log lines
+        for _ in range(warmup):
+            sol.step()
+        ctx.sync()
+        t0 = time.perf_counter()
+        reps = [sol.step() for _ in range(steps)]
+        ctx.sync()
+        wall = time.perf_counter() - t0
+    last = reps[-1]
+    out = {"mesh": f"{n}^3 jittered, permuted, renumbered", "cells": N, "steps": steps,
+           "ms_per_step": round(wall * 1e3 / steps, 3),
+           "cell_steps_per_s": round(N * steps / wall, 1),
+           "coNum": round(last.coNum, 4), "sumLocalContErr": last.sumLocalContErr,
+           "U_iters": [[s.numIter for s in r.U] for r in reps],
+           "p_iters": [[s.numIter for s in r.p] for r in reps],
+           "max_Ux": float(np.max(U.numpy()[:, 0]))}
+    m.close()
+    return out
+
+
 def bench_served(n_req=96, seed=1005, rate=200.0, tile_rows=4096, chunk=16, rate_low=40.0):
     """Served request stream: Poisson arrivals, 70% heat (5 Euler steps, kappa ~
     logU[1e-4,1e-1]) / 30% pressure Poisson, on jittered permuted hex cubes with
@@ -885,6 +939,8 @@ def main():
                          "are measured at saturation")
     ap.add_argument("--rate-low", type=float, default=40.0,
                     help="a second, lower arrival rate of the served stream (0: skip)")
+    ap.add_argument("--no-icofoam", action="store_true")
+    ap.add_argument("--ico-n", type=int, default=64, help="cells per edge of the icoFoam cavity")
     ap.add_argument("--gather-run", type=int, default=0,
                     help="bricks per XCD run of the staged gathers (0: the library default)")
     ap.add_argument("--copy-probe", action="store_true",
@@ -974,6 +1030,9 @@ def main():
         rec["implicit_heat"] = bench_heat(ctx, cpu=not args.no_cpu)
         log("pressure Poisson 128^3, FP64 vs mixed")
         rec["implicit_poisson"] = bench_poisson(ctx, cpu=not args.no_cpu)
+    if not args.no_icofoam:
+        log("icoFoam lid-driven cavity")
+        rec["icofoam_cavity"] = bench_icofoam(ctx, n=args.ico_n)
     if args.batched:
         log("batched heat solves")
         rec["batched_heat"] = bench_batched(ctx, P=args.batch_problems,
