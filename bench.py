@@ -801,7 +801,8 @@ def bench_icofoam(ctx, n=64, seed=1006, warmup=2, steps=5):
     return out
 
 
-def bench_served(n_req=96, seed=1005, rate=200.0, tile_rows=4096, chunk=16, rate_low=40.0):
+def bench_served(n_req=96, seed=1005, rate=200.0, tile_rows=4096, chunk=16, rate_low=40.0,
+                 solo_rows=0):
     """Served request stream: Poisson arrivals, 70% heat (5 Euler steps, kappa ~
     logU[1e-4,1e-1]) / 30% pressure Poisson, on jittered permuted hex cubes with
     edge n ~ logU[16,160] (snapped to a catalogue of resident meshes), 200 GB
@@ -880,12 +881,13 @@ def bench_served(n_req=96, seed=1005, rate=200.0, tile_rows=4096, chunk=16, rate
                 "table_builds": s["table_builds"] - srv_stats0["table_builds"],
                 "graph_builds": s["graph_builds"] - srv_stats0["graph_builds"],
                 "gpu_event_ms": {k: round(s[k] - srv_stats0[k], 1)
-                                 for k in ("chunk_event_ms", "start_event_ms")}}, results
+                                 for k in ("chunk_event_ms", "start_event_ms")},
+                "solo_runs": s["solo_runs"] - srv_stats0["solo_runs"]}, results
 
     results = {}
     plain, xp = run("plain", rate, True, precision="double", batched=False)
     mixed, xm = run("batched", rate, True, precision="mixed", batched=True,
-                    tile_rows=tile_rows, chunk=chunk)
+                    tile_rows=tile_rows, chunk=chunk, solo_rows=solo_rows)
     errs = np.array([float(np.linalg.norm(a - b) / np.linalg.norm(b))
                      for a, b in zip(xm, xp) if a is not None and b is not None])
     results["plain_fp64"] = plain
@@ -896,7 +898,7 @@ def bench_served(n_req=96, seed=1005, rate=200.0, tile_rows=4096, chunk=16, rate
     if rate_low:
         lo_p, _ = run("plain-low", rate_low, False, precision="double", batched=False)
         lo_m, _ = run("batched-low", rate_low, False, precision="mixed", batched=True,
-                      tile_rows=tile_rows, chunk=chunk)
+                      tile_rows=tile_rows, chunk=chunk, solo_rows=solo_rows)
         results["below_saturation"] = {
             "rate_per_s": rate_low,
             "plain_fp64": {k: lo_p[k] for k in ("solves_per_s", "p50_ms", "p99_ms", "mean_ms")},
@@ -933,6 +935,9 @@ def main():
     ap.add_argument("--served-chunk", type=int, default=16,
                     help="CG iterations per chunk of the served batch")
     ap.add_argument("--served-tile-rows", type=int, default=4096)
+    ap.add_argument("--served-solo-rows", type=int, default=0,
+                    help="cells from which a served request runs alone (0: the library's "
+                         "default, negative: never)")
     ap.add_argument("--rate", type=float, default=200.0,
                     help="Poisson arrival rate (requests/s) of the served stream (0: all at once); "
                          "the default offers more than the plain path sustains, so both paths "
@@ -1041,7 +1046,7 @@ def main():
         log("served request stream")
         rec["served"] = bench_served(n_req=args.requests, rate=args.rate,
                                      tile_rows=args.served_tile_rows, chunk=args.served_chunk,
-                                     rate_low=args.rate_low)
+                                     rate_low=args.rate_low, solo_rows=args.served_solo_rows)
     print(json.dumps(rec), flush=True)
     ctx.close()
     return 0

@@ -81,6 +81,7 @@ struct fv_server {
     std::deque<Req*> pending;
     std::unordered_map<int64_t, Req*> reqs;
     std::vector<Req*> active;
+    std::deque<Req*> solo;  // admitted requests that run alone (batched servers)
     int64_t nextId = 1;
     bool stop = false;
     std::thread worker;
@@ -92,6 +93,7 @@ struct fv_server {
     Clock::time_point tFirst, tLast;
     double admitMs = 0, gpuMs = 0, advanceMs = 0;  // worker time split (under qmu when read)
     int64_t chunks = 0, tableBuilds = 0, graphBuilds = 0;  // copies of the batch's counters
+    int64_t soloRuns = 0;                                   // requests that ran alone
     double chunkEventMs = 0, startEventMs = 0;             // and of its event timings
     size_t peakBytes = 0;
     int peakActive = 0;
@@ -344,7 +346,14 @@ void worker_loop(fv_server* s) {
         for (Req* r : adm) {
             try {
                 setup(s, r);
-                if (batched) {
+                // a system large enough to fill the GPU runs alone through the
+                // single-system solver (its polling and replay beat the batch
+                // tables' per-chunk round trips); the others join the batch
+                const bool alone = batched && s->opts.solo_rows > 0 &&
+                                   s->meshes[r->d.mesh_id]->N >= s->opts.solo_rows;
+                if (alone) {
+                    s->solo.push_back(r);
+                } else if (batched) {
                     r->member = batch_add(s->batch, r->ls, r->x.p, r->xold.p, r);
                     r->member->tol = r->d.rel_tol;
                     r->member->maxit = r->d.max_iter;
@@ -356,16 +365,21 @@ void worker_loop(fv_server* s) {
                 finish(s, r, e.code);
             }
         }
-        const Clock::time_point t1 = Clock::now();
+        Clock::time_point t1 = Clock::now();
         {
             std::lock_guard<std::mutex> lk(s->qmu);
             s->peakActive = std::max(s->peakActive, (int)s->active.size());
             s->peakBytes = std::max(s->peakBytes, s->ctx->in_use);
             s->admitMs += ms_between(t0, t1);
         }
-        if (!batched) {
+        if (!batched || !s->solo.empty()) {
+            // one request alone: the next of a plain server, or the next large
+            // request of a batched one (then one chunk of the batch follows)
             Req* r = nullptr;
-            {
+            if (batched) {
+                r = s->solo.front();
+                s->solo.pop_front();
+            } else {
                 std::lock_guard<std::mutex> lk(s->qmu);
                 if (!s->active.empty()) r = s->active.front();
             }
@@ -379,8 +393,9 @@ void worker_loop(fv_server* s) {
             }
             {
                 std::lock_guard<std::mutex> lk(s->qmu);
-                s->active.erase(s->active.begin());
+                s->active.erase(std::find(s->active.begin(), s->active.end(), r));
                 s->gpuMs += ms_between(t1, Clock::now());
+                if (batched) ++s->soloRuns;
             }
             try {
                 finish(s, r, status);
@@ -388,7 +403,8 @@ void worker_loop(fv_server* s) {
                 set_error(e.msg);
                 finish(s, r, e.code);
             }
-            continue;
+            if (!batched) continue;
+            t1 = Clock::now();  // the batch's chunk below is timed on its own
         }
         if (s->batch->members.empty()) continue;
         try {
@@ -408,6 +424,7 @@ void worker_loop(fv_server* s) {
                 std::lock_guard<std::mutex> lk(s->qmu);
                 act.swap(s->active);
             }
+            s->solo.clear();  // (its requests are in act)
             for (Req* r : act) {
                 if (r->member) batch_remove(s->batch, r->member);
                 r->member = nullptr;
@@ -483,6 +500,7 @@ int fv_server_create(int device, const fv_server_opts* opts, fv_server** out) {
         FV_REQUIRE(opts->mode == FV_SOLVE_F64 || opts->mode == FV_SOLVE_F32_IR, "unknown mode");
         s = new fv_server();
         s->opts = *opts;
+        if (s->opts.solo_rows == 0) s->opts.solo_rows = 1 << 20;
         check(fv_ctx_create(device, 0, &s->ctx));
         // measured on the served stream of jittered cubes, n ~ logU[16,160]:
         // 4096-row tiles 114 solves/s, 2048 112, 1024 107, 8192 112
@@ -678,6 +696,7 @@ int fv_server_get_stats(fv_server* s, fv_server_stats* o) {
         o->graph_builds = s->graphBuilds;
         o->chunk_event_ms = s->chunkEventMs;
         o->start_event_ms = s->startEventMs;
+        o->solo_runs = s->soloRuns;
         if (s->haveFirst && s->completed) o->wall_s = ms_between(s->tFirst, s->tLast) * 1e-3;
         if (o->wall_s > 0) o->solves_per_s = s->completed / o->wall_s;
         if (!s->latencies.empty()) {

@@ -25,7 +25,7 @@ from .mesh import PolyMesh
 class FvServerOpts(C.Structure):
     _fields_ = [("hbm_budget", C.c_size_t), ("mode", C.c_int32), ("batched", C.c_int32),
                 ("max_active", C.c_int32), ("tile_rows", C.c_int32), ("chunk", C.c_int32),
-                ("use_graph", C.c_int32)]
+                ("use_graph", C.c_int32), ("solo_rows", C.c_int32)]
 
 
 class FvRequest(C.Structure):
@@ -48,7 +48,8 @@ class FvServerStats(C.Structure):
                 ("peak_active", C.c_int32), ("resident_bytes", C.c_size_t),
                 ("admit_ms", C.c_double), ("gpu_ms", C.c_double), ("advance_ms", C.c_double),
                 ("chunks", C.c_int64), ("table_builds", C.c_int64), ("graph_builds", C.c_int64),
-                ("chunk_event_ms", C.c_double), ("start_event_ms", C.c_double)]
+                ("chunk_event_ms", C.c_double), ("start_event_ms", C.c_double),
+                ("solo_runs", C.c_int64)]
 
 
 HEAT, POISSON = 0, 1
@@ -82,9 +83,13 @@ class RequestStats:
 class Server:
     def __init__(self, device: int = 0, hbm_budget: int = 0, precision: str = "mixed",
                  batched: bool = True, max_active: int = 0, tile_rows: int = 0, chunk: int = 0,
-                 use_graph: bool = True):
+                 use_graph: bool = True, solo_rows: int = 0):
+        """solo_rows (batched): requests on meshes of at least this many cells run
+        alone through the single-system solver between chunks (0: 1 << 20,
+        negative: never)."""
         o = FvServerOpts(hbm_budget, _lib.FV_SOLVE_F32_IR if precision == "mixed" else _lib.FV_SOLVE_F64,
-                         1 if batched else 0, max_active, tile_rows, chunk, 1 if use_graph else 0)
+                         1 if batched else 0, max_active, tile_rows, chunk, 1 if use_graph else 0,
+                         solo_rows)
         h = C.c_void_p()
         call("fv_server_create", device, C.byref(o), C.byref(h))
         self.handle = h

@@ -499,6 +499,9 @@ typedef struct {
     int32_t tile_rows;   /* batch tile rows (multiple of 64), 0: 4096 */
     int32_t chunk;       /* CG iterations per FP64 residual replacement, 0: 16 */
     int32_t use_graph;   /* replay chunks as hipGraphs */
+    int32_t solo_rows;   /* batched: a request on a mesh of at least this many cells runs
+                            alone through fv_solve_cg between chunks (a system that size
+                            fills the GPU by itself); 0: 1 << 20, negative: never */
 } fv_server_opts;
 typedef struct {
     int32_t mesh_id;
@@ -530,7 +533,8 @@ typedef struct {
     size_t resident_bytes; /* device bytes of the resident meshes */
     /* where the worker's time went (ms): admitting requests (allocation, inputs,
      * assembly), GPU chunks (batched: one chunk of every member, with its
-     * synchronisation; plain: whole solves), and advancing requests between
+     * synchronisation, and the whole solves of requests that run alone; plain:
+     * whole solves), and advancing requests between
      * chunks (next Euler step, results, release); chunks run, launch tables
      * built and chunk graphs captured (batched) */
     double admit_ms, gpu_ms, advance_ms;
@@ -538,6 +542,7 @@ typedef struct {
     /* batched: GPU time of the chunks and of the batched starts from HIP events
      * around each (the part of gpu_ms the device was working) */
     double chunk_event_ms, start_event_ms;
+    int64_t solo_runs;   /* batched: requests that ran alone (solo_rows) */
 } fv_server_stats;
 int fv_server_create(int device, const fv_server_opts* opts, fv_server** out);
 /* Adds a resident mesh (renumber != 0: Z-order renumbered internally). */

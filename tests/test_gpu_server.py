@@ -51,9 +51,13 @@ def _oracle_heat(poly, kappa, dt, steps, walls, T0):
     return T
 
 
-@pytest.mark.parametrize("batched,precision", [(True, "mixed"), (True, "double"), (False, "double")])
-def test_heat_requests_match_oracle(polys, batched, precision):
-    srv, ids = _server(polys, batched=batched, precision=precision, tile_rows=256, chunk=8)
+@pytest.mark.parametrize("batched,precision,solo", [(True, "mixed", -1), (True, "double", -1),
+                                                    (False, "double", 0), (True, "mixed", 500)])
+def test_heat_requests_match_oracle(polys, batched, precision, solo):
+    """(solo 500: the 9^3 and 12^3 requests of the batched server run alone
+    through the single-system solver, the 6^3 one in the batch)"""
+    srv, ids = _server(polys, batched=batched, precision=precision, tile_rows=256, chunk=8,
+                       solo_rows=solo)
     reqs = []
     for k, (mid, poly) in enumerate(zip(ids, polys)):
         N = poly.n_cells
@@ -69,6 +73,7 @@ def test_heat_requests_match_oracle(polys, batched, precision):
         assert np.max(np.abs(T - ref)) / np.max(np.abs(ref)) < 1e-6
     s = srv.stats()
     assert s["completed"] == 3 and s["failed"] == 0 and s["p99_ms"] >= s["p50_ms"] > 0
+    assert s["solo_runs"] == (2 if solo == 500 else 0)
     srv.close()
 
 
