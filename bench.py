@@ -88,10 +88,10 @@ def kernel_stamp():
 
 
 def measured_traffic(kernel, n):
-    """(HBM bytes per launch of `kernel`, source note) from the committed rocprofv3
-    FETCH_SIZE / WRITE_SIZE summary of this sweep (256^3 only), provided the
-    summary was measured on the current kernel sources (its recorded hash);
-    otherwise (None, why)."""
+    """(HBM bytes per launch of the sweep's `kernel`, named by op and precision as
+    "grad_f64", source note) from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE
+    summary of this sweep (256^3 only), provided the summary was measured on the
+    current kernel sources (its recorded hash); otherwise (None, why)."""
     import re
     if n != 256 or not TRAFFIC_PROFILE.exists():
         return None, "no traffic profile for this size"
@@ -111,6 +111,18 @@ def measured_traffic(kernel, n):
                 return None, "unreadable profile row"
             return int(hbm * 1e9), f"{TRAFFIC_PROFILE.relative_to(ROOT)} ({m.group(1)})"
     return None, "kernel not in the traffic profile"
+
+
+def roofline_record(dominant, n):
+    """The bench line's roofline object for the sweep's dominant kernel:
+    dominant = ((op, precision), mean launch ms, algorithmic bytes per launch)."""
+    (op, prec), ms, nbytes = dominant
+    achieved = nbytes / (ms * 1e-3) / 1e9
+    traffic, note = measured_traffic(f"{op}_{prec}", n)
+    return {"bound": "hbm", "kernel": f"{op}_{prec}", "achieved": round(achieved, 1),
+            "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": traffic, "traffic_source": note, "bytes_per_launch": nbytes,
+            "avg_launch_ms": round(ms, 4), "kernel_name": KERNEL_OF[(op, prec)]}
 
 
 def stream_ceiling(ctx, n_doubles=1 << 28, runs=10):
@@ -979,8 +991,6 @@ def main():
     sweep = time_sweep(ctx, case, args.steps, args.warmup)
     N, F, B = case.mesh.n_cells, case.mesh.n_internal_faces, case.mesh.n_boundary_faces
     (dop, dprec), dom_ms, dom_bytes = sweep["dominant"]
-    achieved = dom_bytes / (dom_ms * 1e-3) / 1e9
-    traffic, traffic_note = measured_traffic(KERNEL_OF[(dop, dprec)], args.n)
     rec = {
         "metric": "face-op GB/s vs HBM peak (explicit operator sweep, FP64+FP32)",
         "value": round(sweep["GB/s"], 1),
@@ -1001,11 +1011,7 @@ def main():
                    "cells": N, "internal_faces": F, "boundary_faces": B, "ops": OPS,
                    "precisions": ["f64", "f32"]},
         "per_op": sweep["per_op"],
-        "roofline": {"bound": "hbm", "kernel": f"{dop}_{dprec}", "achieved": round(achieved, 1),
-                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": traffic, "traffic_source": traffic_note,
-                     "bytes_per_launch": dom_bytes, "avg_launch_ms": round(dom_ms, 4),
-                     "kernel_name": KERNEL_OF[(dop, dprec)]},
+        "roofline": roofline_record(sweep["dominant"], args.n),
         "boundary_mapping": {"ms": round(case.mapping_ms, 3),
                              "what": "permuting the caller's T, U (cells) and gamma (faces) into "
                                      "the renumbered order once at upload"},

@@ -28,6 +28,21 @@ def test_traffic_is_read_from_the_committed_profile_of_the_current_kernels():
         assert 0.8 * alg < traffic < 2.0 * alg, (key, traffic, alg)
 
 
+def test_roofline_record_of_the_dominant_kernel_carries_its_traffic():
+    """The bench line's roofline object (what main() emits) for a dominant FP64
+    gradient at 256^3: achieved = bytes / time, and the traffic from the profile."""
+    if bench.kernel_stamp() not in bench.TRAFFIC_PROFILE.read_text():
+        pytest.skip("kernel sources changed since the committed traffic profile")
+    n = 256
+    nbytes = bench.alg_bytes("grad", "f64", n ** 3, 3 * n * n * (n - 1), 6 * n * n)
+    r = bench.roofline_record((("grad", "f64"), 0.5, nbytes), n)
+    assert r["kernel_name"] == bench.KERNEL_OF[("grad", "f64")]
+    assert abs(r["achieved"] - nbytes / 0.5e-3 / 1e9) < 0.1 and r["frac"] == round(r["achieved"] / 8000, 4)
+    assert r["traffic"] is not None and r["traffic"] > nbytes, r["traffic_source"]
+    for key in ("bound", "achieved", "peak", "unit", "frac", "traffic"):
+        assert key in r
+
+
 def test_traffic_is_withheld_for_other_sizes_and_other_sources(monkeypatch):
     assert bench.measured_traffic("grad_f64", 128)[0] is None
     monkeypatch.setattr(bench, "kernel_stamp", lambda: "0" * 16)
