@@ -11,6 +11,8 @@
  *   dsl::Expression eqn(dsl::imp::ddt(T) - dsl::imp::laplacian(kappa, T));
  *   dsl::solve(eqn, T, t, dt, fvSchemes, solverDict);          // examples/heatTransfer/heatTransfer.cpp:67-76
  *   FoamAdapter-style PDESolver: fvhip::PDESolver(eqn, p).setReference(0, 0.0); .solve();
+ *   PDESolver<Vec3> UEqn(dsl::imp::ddt(U) + dsl::imp::div(phi, U) - dsl::imp::laplacian(nu, U), U);
+ *   auto [rAU, HbyA] = fvhip::computeRAUandHByA(UEqn);          // neoIcoFoam.cpp:94-171
  *
  * Every call goes through libfvhip's extern "C" entry points (hand-written HIP
  * kernels); failures throw fvhip::Error carrying the status code and
@@ -133,6 +135,21 @@ class Vector {
     T* p_ = nullptr;
     size_t n_ = 0;
 };
+
+// NeoN::scalar and NeoN::Vec3, the value types of the equations (a Vec3 field is
+// stored xyz interleaved, 3 doubles per cell or face).
+using scalar = double;
+struct Vec3 {
+    double x = 0, y = 0, z = 0;
+};
+
+// Component k of the interleaved [n][3] vector src into dst [n] (mode 0), or
+// dst [n][3] component k = src [n] (mode 1).
+inline void moveComponent(const Vector<double>& src, Vector<double>& dst, int k, size_t n, int mode) {
+    if (n)
+        check(fv_component(src.exec().get(), FV_F64, (int64_t)n, 3, k, mode, src.data(), dst.data()),
+              "fv_component");
+}
 
 // ---- mesh ----------------------------------------------------------------------
 
@@ -298,6 +315,65 @@ class VolumeField {
     std::shared_ptr<BoundaryConditions> bcs_;
     Vector<double> internal_, boundary_;
     std::unique_ptr<Vector<double>> gradient_, old_;
+};
+
+// fvcc::VolumeField<Vec3>: internal [3N] and boundary [3B] values xyz
+// interleaved, one set of boundary conditions for the three components.  The
+// equations of a vector field are solved component by component: component()
+// copies one out into a scalar field (internal, boundary and old-time values),
+// setComponent() copies one back.
+class VolumeVectorField {
+   public:
+    VolumeVectorField(const UnstructuredMesh& mesh, std::string name,
+                      std::shared_ptr<BoundaryConditions> bcs)
+        : name(std::move(name)),
+          mesh_(&mesh),
+          bcs_(std::move(bcs)),
+          internal_(mesh.exec(), 3 * (size_t)mesh.nCells(), 0.0),
+          boundary_(mesh.exec(), 3 * (size_t)mesh.nBoundaryFaces(), 0.0) {}
+    Vector<double>& internalVector() { return internal_; }
+    const Vector<double>& internalVector() const { return internal_; }
+    Vector<double>& boundaryValues() { return boundary_; }
+    const Vector<double>& boundaryValues() const { return boundary_; }
+    const BoundaryConditions& bcs() const { return *bcs_; }
+    const std::shared_ptr<BoundaryConditions>& bcsPtr() const { return bcs_; }
+    const UnstructuredMesh& mesh() const { return *mesh_; }
+    bool hasOldTime() const { return (bool)old_; }
+    // fvcc::oldTime(U): created on first use as a copy of the field
+    Vector<double>& oldTime() {
+        if (!old_) {
+            old_ = std::make_unique<Vector<double>>(mesh_->exec(), internal_.size());
+            old_->copyFrom(internal_);
+        }
+        return *old_;
+    }
+    void component(int k, VolumeField& out) {
+        moveComponent(internal_, out.internalVector(), k, (size_t)mesh_->nCells(), 0);
+        moveComponent(boundary_, out.boundaryValues(), k, (size_t)mesh_->nBoundaryFaces(), 0);
+        if (old_) moveComponent(*old_, out.oldTime(), k, (size_t)mesh_->nCells(), 0);
+    }
+    void setComponent(int k, const VolumeField& in) {
+        moveComponent(in.internalVector(), internal_, k, (size_t)mesh_->nCells(), 1);
+        moveComponent(in.boundaryValues(), boundary_, k, (size_t)mesh_->nBoundaryFaces(), 1);
+    }
+    // each component's fixedGradient / zeroGradient faces from its cell values
+    void correctBoundaryConditions() {
+        if (!mesh_->nBoundaryFaces()) return;
+        for (int k = 0; k < 3; ++k) {
+            VolumeField c(*mesh_, name, bcs_);
+            moveComponent(internal_, c.internalVector(), k, (size_t)mesh_->nCells(), 0);
+            moveComponent(boundary_, c.boundaryValues(), k, (size_t)mesh_->nBoundaryFaces(), 0);
+            c.correctBoundaryConditions();
+            moveComponent(c.boundaryValues(), boundary_, k, (size_t)mesh_->nBoundaryFaces(), 1);
+        }
+    }
+    std::string name;
+
+   private:
+    const UnstructuredMesh* mesh_;
+    std::shared_ptr<BoundaryConditions> bcs_;
+    Vector<double> internal_, boundary_;
+    std::unique_ptr<Vector<double>> old_;
 };
 
 // fvcc::SurfaceField<scalar>: internal faces then boundary faces [F+B]
@@ -518,6 +594,7 @@ struct Term {
     fvcc::VolumeField* psi = nullptr;
     const fvcc::SurfaceField* face = nullptr;  // gamma (laplacian) or phi (div)
     const Vector<double>* cell = nullptr;      // source coefficient field
+    fvcc::VolumeVectorField* vpsi = nullptr;   // the unknown of a vector equation
 };
 
 // dsl::Expression: a sum of implicit and explicit terms
@@ -568,6 +645,17 @@ inline Term div(const fvcc::SurfaceField& phi, fvcc::VolumeField& psi, std::stri
 }
 inline Term source(const Vector<double>& coeffField, fvcc::VolumeField& psi) {
     return Term{Term::Source, 1.0, "linear", &psi, nullptr, &coeffField};
+}
+// the same operators on a vector field (PDESolver<Vec3>, neoIcoFoam.cpp:94-96)
+inline Term ddt(fvcc::VolumeVectorField& U) {
+    return Term{Term::Ddt, 1.0, "linear", nullptr, nullptr, nullptr, &U};
+}
+inline Term laplacian(const fvcc::SurfaceField& gamma, fvcc::VolumeVectorField& U) {
+    return Term{Term::Laplacian, 1.0, "linear", nullptr, &gamma, nullptr, &U};
+}
+inline Term div(const fvcc::SurfaceField& phi, fvcc::VolumeVectorField& U,
+                std::string scheme = "linear") {
+    return Term{Term::Div, 1.0, std::move(scheme), nullptr, &phi, nullptr, &U};
 }
 }  // namespace imp
 
@@ -635,6 +723,33 @@ inline void assemble(const Expression& e, fvcc::VolumeField& psi, double /*t*/, 
     }
 }
 
+// The equation of a vector field U, component by component on one matrix (the
+// components share their boundary-condition kinds, so the matrix is the same
+// for all three): each component's right-hand side goes to rhs3 [3N]
+// (interleaved); ls keeps component 0's system.
+inline void assembleVector(const Expression& e, fvcc::VolumeVectorField& U, double t, double dt,
+                           la::LinearSystem& ls, Vector<double>& rhs3) {
+    const UnstructuredMesh& m = U.mesh();
+    const size_t N = (size_t)m.nCells();
+    for (const Term& term : e.terms())
+        if (term.kind == Term::Ddt && term.vpsi == &U) U.oldTime();
+    for (int k = 2; k >= 0; --k) {
+        fvcc::VolumeField c(m, U.name, U.bcsPtr());
+        U.component(k, c);
+        Expression ek = e;
+        for (Term& term : ek.terms())
+            if (term.vpsi == &U) {
+                term.psi = &c;
+                term.vpsi = nullptr;
+            }
+        assemble(ek, c, t, dt, ls);
+        Vector<double> b(m.exec(), N);
+        if (N) check(fv_memcpy_d2d(m.exec().get(), b.data(), ls.rhs(), N * sizeof(double)),
+                     "fv_memcpy_d2d");
+        moveComponent(b, rhs3, k, N, 1);
+    }
+}
+
 // fvSolution solver entry -> solve options (src/compatibility/fvSolution.cpp:19-159):
 // solver PCG -> CG, PBiCGStab / smoothSolver -> BiCGStab; relTol, maxIter (1000),
 // tolerance; "precision mixed" selects FP32 CG inside FP64 refinement
@@ -697,8 +812,10 @@ inline la::SolverStats solve(Expression expr, fvcc::VolumeField& psi, double t, 
 
 }  // namespace dsl
 
-// FoamAdapter::PDESolver (include/FoamAdapter/datastructures/expression.hpp:23-179):
-// assemble, optional SetReference, solve
+// FoamAdapter::PDESolver<T> (include/FoamAdapter/datastructures/expression.hpp:23-179):
+// assemble, optional SetReference, solve.  PDESolver<scalar> (also written
+// PDESolver, its arguments deciding) and PDESolver<Vec3>.
+template <typename T = scalar>
 class PDESolver {
    public:
     PDESolver(dsl::Expression expr, fvcc::VolumeField& psi, double t = 0.0, double dt = 1.0,
@@ -716,6 +833,12 @@ class PDESolver {
         refValue_ = value;
     }
     la::LinearSystem& linearSystem() { return ls_; }
+    fvcc::VolumeField& psi() { return psi_; }
+    // RunTime::t and RunTime::dt the next assembly uses
+    void setTime(double t, double dt) {
+        t_ = t;
+        dt_ = dt;
+    }
     la::SolverStats solve() {
         assemble();
         if (refCell_ >= 0) check(fv_set_reference(ls_.get(), refCell_, refValue_), "fv_set_reference");
@@ -733,6 +856,126 @@ class PDESolver {
     int refCell_ = -1;
     double refValue_ = 0.0;
 };
+
+// PDESolver<Vec3>: the matrix once, three right-hand sides, the components
+// solved in turn (BiCGStab for the non-symmetric momentum equation when the
+// solver dictionary says PBiCGStab / smoothSolver).
+template <>
+class PDESolver<Vec3> {
+   public:
+    PDESolver(dsl::Expression expr, fvcc::VolumeVectorField& U, double t = 0.0, double dt = 1.0,
+              Dictionary fvSchemes = {}, Dictionary solverDict = {})
+        : expr_(std::move(expr)), U_(U), t_(t), dt_(dt), solver_(std::move(solverDict)),
+          ls_(la::SparsityPattern(U.mesh())), rhs3_(U.mesh().exec(), 3 * (size_t)U.mesh().nCells()) {
+        expr_.read(fvSchemes);
+    }
+    la::LinearSystem& assemble() {
+        dsl::assembleVector(expr_, U_, t_, dt_, ls_, rhs3_);
+        return ls_;
+    }
+    la::LinearSystem& linearSystem() { return ls_; }
+    const Vector<double>& rhs3() const { return rhs3_; }  // the three right-hand sides, xyz
+    fvcc::VolumeVectorField& psi() { return U_; }
+    void setTime(double t, double dt) {
+        t_ = t;
+        dt_ = dt;
+    }
+    std::vector<la::SolverStats> solve() {
+        assemble();
+        const UnstructuredMesh& m = U_.mesh();
+        const size_t N = (size_t)m.nCells();
+        std::vector<la::SolverStats> st;
+        Vector<double> x(m.exec(), N), b(m.exec(), N);
+        for (int k = 0; k < 3; ++k) {
+            moveComponent(rhs3_, b, k, N, 0);
+            if (N) check(fv_memcpy_d2d(m.exec().get(), ls_.rhs(), b.data(), N * sizeof(double)),
+                         "fv_memcpy_d2d");
+            moveComponent(U_.internalVector(), x, k, N, 0);
+            st.push_back(dsl::solveSystem(ls_, x, solver_));
+            moveComponent(x, U_.internalVector(), k, N, 1);
+        }
+        U_.correctBoundaryConditions();
+        return st;
+    }
+
+   private:
+    dsl::Expression expr_;
+    fvcc::VolumeVectorField& U_;
+    double t_, dt_;
+    Dictionary solver_;
+    la::LinearSystem ls_;
+    Vector<double> rhs3_;
+};
+
+// ---- pressure-velocity coupling (src/algorithms/pressureVelocityCoupling.cpp) ----
+
+// computeRAUandHByA(UEqn) (:63-119): rAU = V / diag(A) and HbyA = (b - sum_nb
+// A U_nb) rAU / V, boundary values extrapolated; HbyA takes U's conditions
+inline std::pair<fvcc::VolumeField, fvcc::VolumeVectorField> computeRAUandHByA(
+    PDESolver<Vec3>& UEqn) {
+    fvcc::VolumeVectorField& U = UEqn.psi();
+    const UnstructuredMesh& m = U.mesh();
+    fvcc::VolumeField rAU(m, "rAU", nullptr);
+    fvcc::VolumeVectorField HbyA(m, "HbyA", U.bcsPtr());
+    la::LinearSystem& ls = UEqn.linearSystem();
+    check(fv_compute_rau(ls.get(), rAU.internalVector().data(), rAU.boundaryValues().data()),
+          "fv_compute_rau");
+    check(fv_compute_hbya(ls.get(), UEqn.rhs3().data(), U.internalVector().data(),
+                          rAU.internalVector().data(), HbyA.internalVector().data(),
+                          HbyA.boundaryValues().data()),
+          "fv_compute_hbya");
+    return {std::move(rAU), std::move(HbyA)};
+}
+
+// constrainHbyA (:14-36): HbyA_b = U_b on U's fixedValue patches
+inline void constrainHbyA(const fvcc::VolumeVectorField& U, const fvcc::VolumeField& /*p*/,
+                          fvcc::VolumeVectorField& HbyA) {
+    check(fv_constrain_hbya(U.bcs().get(), U.boundaryValues().data(), HbyA.boundaryValues().data()),
+          "fv_constrain_hbya");
+}
+
+// flux(U) (:213-267): phi = Sf . interpolate(U), boundary Sf_b . U_b
+inline fvcc::SurfaceField flux(const fvcc::VolumeVectorField& U) {
+    const UnstructuredMesh& m = U.mesh();
+    fvcc::SurfaceField phi(m, "phi");
+    check(fv_flux(m.exec().get(), m.get(), FV_F64, U.internalVector().data(),
+                  U.boundaryValues().data(), phi.internalVector().data()),
+          "fv_flux");
+    return phi;
+}
+
+// updateFaceVelocity(phiHbyA, pEqn, phi) (:122-195): phi = phiHbyA - pEqn.flux()
+inline void updateFaceVelocity(const fvcc::SurfaceField& phiHbyA, PDESolver<scalar>& pEqn,
+                               fvcc::SurfaceField& phi) {
+    check(fv_update_face_velocity(pEqn.linearSystem().get(), phiHbyA.internalVector().data(),
+                                  pEqn.psi().internalVector().data(), phi.internalVector().data()),
+          "fv_update_face_velocity");
+}
+
+// updateVelocity(HbyA, rAU, p, U) (:197-211): U = HbyA - rAU grad(p)
+inline void updateVelocity(const fvcc::VolumeVectorField& HbyA, const fvcc::VolumeField& rAU,
+                           const fvcc::VolumeField& p, fvcc::VolumeVectorField& U) {
+    const UnstructuredMesh& m = U.mesh();
+    Vector<double> gradP(m.exec(), 3 * (size_t)m.nCells());
+    fvcc::GaussGreenGrad(m.exec(), m).grad(p, dsl::Coeff(), gradP);
+    check(fv_update_velocity(m.exec().get(), m.get(), HbyA.internalVector().data(),
+                             rAU.internalVector().data(), gradP.data(), U.internalVector().data()),
+          "fv_update_velocity");
+}
+
+// computeCoNum(phi, dt) (scalarAdvection.cpp:69) and setDeltaT (setup.cpp:13-22)
+inline double computeCoNum(const fvcc::SurfaceField& phi, double dt) {
+    double co = 0.0;
+    check(fv_courant_number(phi.mesh().exec().get(), phi.mesh().get(),
+                            phi.internalVector().data(), dt, &co),
+          "fv_courant_number");
+    return co;
+}
+inline double setDeltaT(double dt, double coNum, double maxCo, double maxDeltaT) {
+    double out = dt;
+    check(fv_set_delta_t(dt, coNum, maxCo, maxDeltaT, &out), "fv_set_delta_t");
+    return out;
+}
 
 }  // namespace fvhip
 

@@ -2,8 +2,10 @@
 // header-only facade (include/fvhip.hpp), the way FoamAdapter's C++ calls NeoN:
 // the explicit operators of test/test_operators.cpp:52-131, the heat-transfer
 // time loop of examples/heatTransfer/heatTransfer.cpp:60-86 through
-// dsl::solve, and a pressure equation through PDESolver with setReference
-// (include/FoamAdapter/datastructures/expression.hpp:23-179).  Test
+// dsl::solve, a pressure equation through PDESolver with setReference
+// (include/FoamAdapter/datastructures/expression.hpp:23-179), and the
+// neoIcoFoam time loop (examples/neoIcoFoam/neoIcoFoam.cpp:80-180) on a
+// lid-driven cavity through PDESolver<Vec3> and the coupling helpers.  Test
 // infrastructure: tests/test_gpu_cpp_facade.py runs it on the GPU and compares
 // what it writes (raw FP64 arrays in the output directory) with the oracle.
 #include <cstdio>
@@ -115,6 +117,61 @@ int main(int argc, char** argv) {
         std::printf("pressure: initial %.3e final %.3e iterations %d\n", ps.initResNorm,
                     ps.finalResNorm, ps.numIter);
         write("poisson_p", p.internalVector());
+
+        // ---- neoIcoFoam (examples/neoIcoFoam/neoIcoFoam.cpp:80-180) on a lid-driven
+        // cavity: 8^3 jittered hex block, lid (patch y1) U = (1, 0, 0), no-slip walls,
+        // p zeroGradient with reference cell 0, nu = 0.01, dt = 0.05, upwind
+        // convection, momentum predictor and two pressure correctors, four steps ----
+        {
+            fvhip::HexBlock cb(8, 8, 8, 0.15, 91, 92);
+            auto cm = fvhip::UnstructuredMesh::fromPolyMesh(exec, cb.desc());
+            const size_t Bc = cm.nBoundaryFaces();
+            auto ubc = std::make_shared<fvcc::BoundaryConditions>(
+                cm, std::vector<int32_t>(cm.nPatches(), FV_BC_FIXED_VALUE));
+            auto pbc = std::make_shared<fvcc::BoundaryConditions>(
+                cm, std::vector<int32_t>(cm.nPatches(), FV_BC_FIXED_GRADIENT));
+            fvcc::VolumeVectorField U(cm, "U", ubc);
+            fvcc::VolumeField pc(cm, "p", pbc);
+            {
+                const std::vector<int32_t> po = cm.get<int32_t>(FV_G_PATCH_OFFSETS, cm.nPatches() + 1);
+                std::vector<double> ub(3 * Bc, 0.0);
+                for (int b = po[3]; b < po[4]; ++b) ub[3 * (size_t)b] = 1.0;
+                U.boundaryValues() = Vector<double>(exec, ub);
+            }
+            fvcc::SurfaceField phiC = fvhip::flux(U);
+            fvcc::SurfaceField nu(cm, "nu", 0.01);
+            const double dtc = 0.05;
+            const fvhip::Dictionary schemes{{"div", "Gauss upwind"}};
+            const fvhip::Dictionary solU{{"solver", "PBiCGStab"}, {"relTol", "1e-13"}, {"maxIter", "1000"}};
+            const fvhip::Dictionary solP{{"solver", "PCG"}, {"relTol", "1e-13"}, {"maxIter", "2000"}};
+            for (int step = 0; step < 4; ++step) {
+                U.oldTime().copyFrom(U.internalVector());
+                const double co = fvhip::computeCoNum(phiC, dtc);
+                fvhip::PDESolver<fvhip::Vec3> UEqn(
+                    dsl::imp::ddt(U) + dsl::imp::div(phiC, U) - dsl::imp::laplacian(nu, U), U,
+                    (step + 1) * dtc, dtc, schemes, solU);
+                UEqn.solve();
+                int pIters = 0;
+                for (int corr = 0; corr < 2; ++corr) {
+                    auto [rAU, HbyA] = fvhip::computeRAUandHByA(UEqn);
+                    fvhip::constrainHbyA(U, pc, HbyA);
+                    fvcc::SurfaceField rAUf(cm, "rAUf");
+                    fvcc::SurfaceInterpolation(exec, cm, {"linear"}).interpolate(rAU, rAUf);
+                    fvcc::SurfaceField phiHbyA = fvhip::flux(HbyA);
+                    fvhip::PDESolver pEqn(dsl::imp::laplacian(rAUf, pc) - dsl::exp::div(phiHbyA), pc,
+                                          (step + 1) * dtc, dtc, schemes, solP);
+                    pEqn.setReference(0, 0.0);
+                    pIters += pEqn.solve().numIter;   // corrects p's boundary values
+                    fvhip::updateFaceVelocity(phiHbyA, pEqn, phiC);
+                    fvhip::updateVelocity(HbyA, rAU, pc, U);
+                    U.correctBoundaryConditions();
+                }
+                std::printf("icoFoam step %d: Co %.3f, pressure iterations %d\n", step, co, pIters);
+            }
+            write("ico_U", U.internalVector());
+            write("ico_p", pc.internalVector());
+            write("ico_phi", phiC.internalVector());
+        }
         exec.fence();
     } catch (const fvhip::Error& e) {
         std::fprintf(stderr, "fvhip error %d: %s\n", e.code, e.what());

@@ -5,7 +5,9 @@ examples/heatTransfer/heatTransfer.cpp through dsl::solve, and a pressure
 equation through PDESolver with setReference.  What the program writes is
 compared with the oracle: operators bit for bit, solves within the solver
 tolerance.  The CPU test checks that the facade compiles with the host
-compiler alone and that the demo is built."""
+compiler alone and that the demo is built.  The demo's neoIcoFoam loop
+(PDESolver<Vec3>, the coupling helpers) is compared with the oracle-composed
+loop of test_gpu_icofoam.py after four steps, within 1e-8."""
 import subprocess
 from pathlib import Path
 
@@ -80,3 +82,11 @@ def test_cpp_facade_demo_matches_oracle(tmp_path):
     assert relmax(got["poisson_p"], p) < 1e-6
     r2 = ls.rhs - ls.scipy() @ got["poisson_p"]
     assert np.linalg.norm(r2) / np.linalg.norm(ls.rhs) < 1.01e-10
+    # neoIcoFoam on the lid-driven cavity: the state after four steps against the
+    # same loop composed from the oracle (the cavity of test_gpu_icofoam.py)
+    import test_gpu_icofoam as ico
+    gc = orc.geometry(fa.PolyMesh.hex(8, jitter=0.15, jitter_seed=91, perm_seed=92))
+    Ur, pr, phir = ico._oracle_loop(gc, ico._cavity(gc))[-1]
+    assert relmax(got["ico_U"], Ur.ravel()) < 1e-8
+    assert relmax(got["ico_p"], pr) < 1e-8
+    assert relmax(got["ico_phi"], phir) < 1e-8
