@@ -34,33 +34,40 @@ def _cavity(g):
     return Ub
 
 
-def _oracle_loop(g, Ub):
-    """The neoIcoFoam step restated with the oracle: Euler ddt, upwind
-    convection, laplacian (all walls fixedValue), per-component direct solves
-    on the shared momentum matrix, then NCORR times rAU/HbyA, constrainHbyA,
-    rAUf, phiHbyA, the pressure equation (zeroGradient walls, reference cell
-    0 at 0) solved directly, updateFaceVelocity and updateVelocity."""
+def _oracle_loop(g, Ub, steps=STEPS, ncorr=NCORR, dt=DT, nu_=NU, scheme=1, predictor=True,
+                 uk=None, pk=None, U0=None):
+    """The neoIcoFoam step restated with the oracle: Euler ddt, convection
+    (scheme 1 upwind, 0 linear), laplacian (walls fixedValue unless uk says
+    otherwise), per-component direct solves on the shared momentum matrix (or,
+    without the momentum predictor, the matrix only), then ncorr times
+    rAU/HbyA, constrainHbyA, rAUf, phiHbyA, the pressure equation (pk, by
+    default zeroGradient walls; reference cell 0 at 0) solved directly,
+    updateFaceVelocity and updateVelocity."""
     N, F, B = len(g["V"]), len(g["owner"]), len(g["faceCells"])
+    nP = len(g["patchOffsets"]) - 1
     sp = orc.sparsity(g)
-    uk, pk = [0] * 6, [1] * 6
-    nu = np.full(F + B, NU)
-    U, p, pb = np.zeros((N, 3)), np.zeros(N), np.zeros(B)
+    uk = [0] * nP if uk is None else uk
+    pk = [1] * nP if pk is None else pk
+    nu = np.full(F + B, nu_)
+    U = np.zeros((N, 3)) if U0 is None else U0.copy()
+    p, pb = np.zeros(N), np.zeros(B)
     phi = orc.flux(g, U, Ub)
     out = []
-    for _ in range(STEPS):
+    for _ in range(steps):
         Uold = U.copy()
         rhs = []
         for k in range(3):
             ls = orc.LinSys(g, sp)
-            ls.ddt(Uold[:, k].copy(), DT)
-            ls.div(phi, Ub[:, k].copy(), uk, None, 1)
+            ls.ddt(Uold[:, k].copy(), dt)
+            ls.div(phi, Ub[:, k].copy(), uk, None, scheme)
             ls.laplacian(nu, Ub[:, k].copy(), uk, None, -1.0)
             rhs.append(ls.rhs)
             vals = ls.values
-        A = ls.scipy().tocsc()
-        U = np.stack([spla.spsolve(A, r) for r in rhs], axis=1)
+        if predictor:
+            A = ls.scipy().tocsc()
+            U = np.stack([spla.spsolve(A, r) for r in rhs], axis=1)
         b3 = np.stack(rhs, axis=1)
-        for _ in range(NCORR):
+        for _ in range(ncorr):
             rau, raub = orc.rau(g, sp, vals)
             h, hb = orc.hbya(g, sp, vals, b3, U, rau)
             hb = orc.constrain_hbya(g, uk, Ub, hb)
