@@ -153,95 +153,3 @@ class IcoFoam:
     def run(self, nSteps: int) -> list[StepReport]:
         return [self.step() for _ in range(nSteps)]
 
-
-# ---- a case directory, as the reference's tutorials lay one out ------------------
-
-def _number(v) -> float:
-    """The value of `nu 0.01;` or of the dimensioned `nu [0 2 -1 0 0 0 0] 0.01;`."""
-    if isinstance(v, list):
-        v = v[-1]
-    return float(v)
-
-
-def _volume_field(ctx, mesh, poly, path, ncomp, name):
-    """VolumeField of a case field file: internal values, boundary conditions
-    mapped as readers.hpp maps them, fixedGradient gradients, and the boundary
-    values those conditions give."""
-    from .foamio import read_boundary_conditions, read_field
-    from .mesh import BoundaryConditions
-    internal, _ = read_field(path, poly.patches, ncomp)
-    N = mesh.n_cells
-    internal = np.asarray(internal, np.float64)
-    if internal.size == ncomp:                       # uniform
-        internal = np.tile(internal.reshape(-1), (N, 1)).reshape((N,) if ncomp == 1 else (N, ncomp))
-    bcs = read_boundary_conditions(path, poly.patches, ncomp)
-    # one kind per patch of the mesh; empty patches carry no boundary faces
-    kinds = ["calculated" if p.empty else bcs[p.name][0] for p in poly.patches]
-    shape = (mesh.n_boundary_faces,) if ncomp == 1 else (mesh.n_boundary_faces, ncomp)
-    vals, grads = np.zeros(shape), np.zeros(shape)
-    off = 0
-    for p in poly.fv_patches():
-        k, v = bcs[p.name]
-        if k == "fixedGradient":
-            grads[off:off + p.size] = v
-        else:
-            vals[off:off + p.size] = v
-        off += p.size
-    bc = BoundaryConditions(mesh, kinds)
-    g = ctx.upload(grads) if np.any(grads) else None
-    f = fvcc.VolumeField(mesh, ctx.upload(internal), ctx.upload(vals), bc, name, g)
-    f.correct_boundary_conditions()                  # fixedGradient faces from their cells
-    return f
-
-
-def load_case(ctx, case_dir, overrides: dict | None = None,
-              solver_entries: dict | None = None) -> IcoFoam:
-    """IcoFoam on an OpenFOAM case directory as neoIcoFoam reads one
-    (examples/neoIcoFoam/neoIcoFoam.cpp:30-78): the mesh from constant/polyMesh,
-    or from a single-block system/blockMeshDict; U and p from the start time
-    directory (0, else 0.orig); nu from constant/transportProperties; deltaT,
-    adjustTimeStep, maxCo and maxDeltaT from system/controlDict; the convection
-    scheme of div(phi,U) from system/fvSchemes; the U and p solvers and the PISO
-    controls from system/fvSolution.  solver_entries names the fvSolution
-    entries used for U and p (default {"U": "U", "p": "p"}; the reference's
-    tutorial also holds Ginkgo-form entries nfU / nfP); overrides replaces
-    entries of the result ({"p": {...}, "U": {...}, "PISO": {...}})."""
-    from pathlib import Path
-    from .blockmesh import read_block_mesh_dict
-    from .foamdict import read_foam_dict
-    from .mesh import Mesh, read_polymesh
-    case = Path(case_dir)
-    poly = (read_polymesh(case) if (case / "constant" / "polyMesh" / "owner").exists()
-            else read_block_mesh_dict(case / "system" / "blockMeshDict"))
-    mesh = Mesh.from_poly(ctx, poly)
-    control = read_foam_dict(case / "system" / "controlDict")
-    schemes = read_foam_dict(case / "system" / "fvSchemes")
-    solution = read_foam_dict(case / "system" / "fvSolution")
-    nu = _number(read_foam_dict(case / "constant" / "transportProperties")["nu"])
-    start = str(control.get("startTime", 0))
-    tdir = next((case / d for d in (start, "0", "0.orig") if (case / d / "U").exists()), None)
-    if tdir is None:
-        raise ValueError(f"{case}: no U field in the start time directory")
-    U = _volume_field(ctx, mesh, poly, tdir / "U", 3, "U")
-    p = _volume_field(ctx, mesh, poly, tdir / "p", 1, "p")
-    phi = fvcc.flux(ctx, mesh, U)
-    nFB = mesh.n_internal_faces + mesh.n_boundary_faces
-    nuf = fvcc.SurfaceField(mesh, ctx.upload(np.full(nFB, nu)), "nu")
-    div = schemes.get("divSchemes", {})
-    conv = div.get("div(phi,U)", div.get("default"))
-    fvSchemes = {"div": " ".join(conv) if isinstance(conv, list) else str(conv)} if conv else None
-    sols = solution.get("solvers", {})
-    names = {"U": "U", "p": "p", **(solver_entries or {})}
-    fvSolution = {"U": dict(sols.get(names["U"], {})), "p": dict(sols.get(names["p"], {})),
-                  "PISO": dict(solution.get("PISO", {}))}
-    for k, v in (overrides or {}).items():
-        fvSolution[k] = {**fvSolution.get(k, {}), **v}
-    yes = ("yes", "on", "true", 1)
-    adjust = control.get("adjustTimeStep", "no") in yes
-    solver = IcoFoam(U, p, phi, nuf, float(control.get("deltaT")), t0=float(start),
-                     fvSchemes=fvSchemes, fvSolution=fvSolution, adjustTimeStep=adjust,
-                     maxCo=float(control.get("maxCo", 1.0)),
-                     maxDeltaT=float(control.get("maxDeltaT", np.inf)))
-    solver.poly, solver.endTime = poly, float(control.get("endTime", start))
-    solver.nu = nu
-    return solver

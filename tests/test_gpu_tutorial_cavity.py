@@ -57,7 +57,7 @@ def test_block_mesh_rejects_what_it_does_not_mesh(tmp_path):
 
 @pytest.mark.gpu
 def test_tutorial_cavity_matches_oracle_loop(ctx):
-    from foamadapter_amd.piso import load_case
+    from foamadapter_amd.cases import load_icofoam as load_case
     from test_gpu_icofoam import _oracle_loop, relmax
     print("[case] tutorials/cavity, 100x100x1, pressure to 1e-13, 4 steps", flush=True)
     solver = load_case(ctx, CASE, overrides={"p": {"tolerance": 0.0, "relTol": 1e-13,
@@ -100,7 +100,7 @@ def test_tutorial_cavity_runs_with_its_solver_entries(ctx):
     residual).  (Its OpenFOAM-form entry p asks for an absolute 1e-6 on the raw
     residual norm, which the pressure equation of this case meets before the
     first iteration: the reference's mapping makes the same decision.)"""
-    from foamadapter_amd.piso import load_case
+    from foamadapter_amd.cases import load_icofoam as load_case
     print("[case] tutorials/cavity with nfP, 50 steps", flush=True)
     solver = load_case(ctx, CASE, solver_entries={"p": "nfP"})
     reps = solver.run(50)
