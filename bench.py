@@ -196,15 +196,25 @@ def alg_bytes(op, prec, N, F, B):
     return f * F + c * N + b * B
 
 
-def cg_bytes(N, nnz, iters, reps, mixed, starts=1):
+def cg_bytes(N, nnz, iters, reps, mixed, starts=1, finals=0):
     """Algorithmic bytes of Jacobi-PCG on an N-row CSR system with nnz entries
-    (int32 columns and row offsets): each FP64 residual r = b - A x (the start
-    and every replacement of the mixed solver) reads the matrix, x, b and writes r
-    (12 nnz + 4(N+1) + 24 N); each iteration reads the matrix and the vectors
-    it needs once (FP64 12 nnz + 4(N+1) + 104 N, FP32 inner 8 nnz + 4(N+1) + 52 N)."""
+    (int32 columns and row offsets), run as CG on the symmetrically scaled
+    system (unit diagonal, off-diagonal entries only):
+    * each solve start scales the matrix: reads the diagonal (row offset,
+      diagonal slot, value) and writes g and d = 0 (FP64 32 N, FP32 24 N), reads
+      each off-diagonal entry's source index, column and value and writes its
+      scaled value (FP64 24 (nnz - N), FP32 20 (nnz - N));
+    * each FP64 residual r = b - A x (the start, every replacement of the mixed
+      solver, and the `finals` explicit residuals a plain FP64 solve computes
+      at exit) reads the matrix, x, b and writes r (12 nnz + 4(N+1) + 24 N);
+    * each iteration reads the off-diagonal entries and the vectors it needs
+      once: SpMV p, q; update d, p, r, q, g read, d, r written; direction r, p
+      read, p written (FP64 12 (nnz - N) + 96 N, FP32 inner 8 (nnz - N) + 48 N)."""
+    off = nnz - N
+    scale = (20 * off + 24 * N) if mixed else (24 * off + 32 * N)
     res64 = 12 * nnz + 4 * (N + 1) + 24 * N
-    it = (8 * nnz + 4 * (N + 1) + 52 * N) if mixed else (12 * nnz + 4 * (N + 1) + 104 * N)
-    return (starts + reps) * res64 + iters * it
+    it = (8 * off + 48 * N) if mixed else (12 * off + 96 * N)
+    return starts * scale + (starts + reps + finals) * res64 + iters * it
 
 
 def heat_assembly_bytes(N, F, B):
@@ -464,7 +474,8 @@ def bench_heat(ctx, n=32, steps=10, cpu=True):
         assert np.array_equal(Tf, runs[0][2]), "heat run is not reproducible"
         mixed = mode == "mixed"
         nbytes = sum(heat_assembly_bytes(N, F, B) +
-                     cg_bytes(N, nnz, st.numIter, st.outerIter if mixed else 0, mixed)
+                     cg_bytes(N, nnz, st.numIter, st.outerIter if mixed else 0, mixed,
+                              finals=0 if mixed else 1)   # FP64: the explicit residual at exit
                      for st in stats)
         res[mode] = {"wall_ms": round(wall * 1e3, 2), "ms_per_step": round(wall * 1e3 / steps, 3),
                      "first_run_ms_per_step": round(runs[0][0] * 1e3 / steps, 3),
@@ -549,7 +560,7 @@ def bench_poisson(ctx, n=128, seed=1003, repeats=3, cpu=True):
         ms, st = best
         mixed = mode == "mixed"
         nbytes = cg_bytes(N, nnz, st.numIter, st.outerIter if mixed else 0, mixed,
-                          starts=1 if mixed else 2)   # FP64: start and the explicit residual at exit
+                          finals=0 if mixed else 1)   # FP64: the explicit residual at exit
         out[mode] = {"ms": round(ms, 3), "solves_per_s": round(1e3 / ms, 2), "iters": st.numIter,
                      "outer": st.outerIter, "final_rel_res": st.finalResNorm,
                      "true_rel_res": st.trueResNorm, "roofline": roof(nbytes, ms)}
@@ -660,7 +671,7 @@ def bench_batched(ctx, P=512, seed=1004, steps=5, use_graph=True):
                           p["kap"].ptr, p["bc"].handle, p["Tb"].ptr, None, -1.0)
                 _lib.call("fv_solve_cg", p["ls"].handle, x.ptr, C.byref(o), C.byref(st))
                 iters += st.iters
-                nbytes += p["asm_b"] + cg_bytes(p["N"], p["nnz"], st.iters, 0, False, starts=2)
+                nbytes += p["asm_b"] + cg_bytes(p["N"], p["nnz"], st.iters, 0, False, finals=1)
         ctx.sync()
         return iters, nbytes
 

@@ -107,9 +107,9 @@ void build_tables(fv_batch* b) {
         p.tile0 = (int)ht.size();
         member_tile_list<S>(b, i, p.tile0, ht);
         p.ntiles = (int)ht.size() - p.tile0;
-        p.sellOff = sp->sellOff.p;
-        p.sellCol = sp->sellCol.p;
-        p.sellSrc = sp->sellSrc.p;
+        p.sellOff = sp->sellOffX.p;
+        p.sellCol = sp->sellColX.p;
+        p.sellSrc = sp->sellSrcX.p;
         p.diag = sp->diagOffset.p;
         p.sval = (const S*)m->sval.p;
         p.rowOffs = sp->rowOffs.p;
@@ -121,12 +121,11 @@ void build_tables(fv_batch* b) {
         S* v = (S*)m->vec.p;
         p.d = v;
         p.r = v + N;
-        p.z = v + 2 * (int64_t)N;
-        p.p = v + 3 * (int64_t)N;
-        p.q = v + 4 * (int64_t)N;
-        p.dinv = v + 5 * (int64_t)N;
-        p.p2 = v + 6 * (int64_t)N;
-        p.r64 = (double*)(m->vec.p + 7 * (int64_t)N * sizeof(S));
+        p.p = v + 2 * (int64_t)N;
+        p.q = v + 3 * (int64_t)N;
+        p.g = v + 4 * (int64_t)N;
+        p.p2 = v + 5 * (int64_t)N;
+        p.r64 = (double*)(m->vec.p + 6 * (int64_t)N * sizeof(S));
         p.st = b->states.p + m->slot;
     }
     // Launch table: logical blocks are numbered XCD-contiguously (block i runs on
@@ -209,7 +208,7 @@ void enqueue_chunk(fv_batch* b) {
         kc_direction<S, 0><<<G, kB, 0, s>>>(g, red, (n - 1) & 1);
     }
     // FP64 residual replacement of the systems that stopped in this chunk
-    kc_accum<S><<<G, kB, 0, s>>>(g);
+    kc_accum<S, 0><<<G, kB, 0, s>>>(g);
     kc_resid<S, 0><<<G, kB, 0, s>>>(g, red);
     kc_replace<S, 0><<<G, kB, 0, s>>>(g, red);
     kc_direction<S, 1><<<G, kB, 0, s>>>(g, red, 0);
@@ -274,7 +273,8 @@ void start_members(fv_batch* b) {
     const CgPartials red = partials_of<S>(b);
     const int G = cg_grid((int)st.size());
     FV_HIP(hipEventRecord(b->ev[2], s));
-    kc_start_tile<S><<<G, kB, 0, s>>>(g, (const CgStartParam*)b->startParams.p);
+    kc_start_tile<S, 1><<<G, kB, 0, s>>>(g, (const CgStartParam*)b->startParams.p);
+    kc_scale_tile<S><<<G, kB, 0, s>>>(g);
     kc_resid<S, 1><<<G, kB, 0, s>>>(g, red);
     kc_replace<S, 1><<<G, kB, 0, s>>>(g, red);
     kc_direction<S, 2><<<G, kB, 0, s>>>(g, red, 0);
@@ -349,8 +349,8 @@ BMember* batch_add(fv_batch* b, fv_linsys* ls, double* x, double* xold, void* us
         m->N = ls->sp->nRows;
         m->user = user;
         const size_t e = elem(b->prec);
-        m->vec.alloc(b->ctx, 7 * (size_t)m->N * e + (size_t)m->N * sizeof(double));
-        m->sval.alloc(b->ctx, (size_t)ls->sp->sellTotal * e);
+        m->vec.alloc(b->ctx, 6 * (size_t)m->N * e + (size_t)m->N * sizeof(double));
+        m->sval.alloc(b->ctx, (size_t)ls->sp->sellTotalX * e);
         m->slot = grow_slots(b);
     } catch (...) {
         delete m;

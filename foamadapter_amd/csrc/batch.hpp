@@ -16,8 +16,8 @@ struct BMember {
     bool needStart = true;   // matrix/right-hand side (re)loaded, CG must (re)start
     double tol = 0.0;        // FP64 relative residual target (0: the batch's)
     int maxit = 0;           // CG iteration limit per solve (0: the batch's)
-    fvhip::DBuf<uint8_t> vec;   // d, r, z, p, q, dinv, p2 (S) + r64 (double)
-    fvhip::DBuf<uint8_t> sval;  // sliced-ELL values in S
+    fvhip::DBuf<uint8_t> vec;   // d, r, p, q, g, p2 (S) + r64 (double)
+    fvhip::DBuf<uint8_t> sval;  // scaled off-diagonal sliced-ELL values in S
     fv_solver_stats stats{};
     void* user = nullptr;    // owner's tag (the server's request)
 };

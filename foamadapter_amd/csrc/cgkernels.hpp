@@ -1,6 +1,19 @@
 // cgkernels.hpp -- the Jacobi-preconditioned CG iteration as row-tile kernels,
 // shared by the single-system solver (fv_solve_cg) and the batched engine.
 //
+// Jacobi preconditioning is applied as a symmetric scaling: with g_i the signed
+// square root of the diagonal a_ii (g_i |g_i| = a_ii) and G = diag(g), the
+// kernels run plain CG on  As y = bs,  As = G^-1 A |G|^-1,  bs = G^-1 b,
+// x = |G|^-1 y.  As is symmetric with a unit diagonal when A's diagonal has one
+// sign (every symmetric FV operator: ddt + laplacian, laplacian), and the
+// iterates are those of Jacobi-PCG on A (r = G rs, z = D^-1 r = |G|^-1 rs,
+// p = |G|^-1 ps, r.z = rs.rs, p.Ap = ps.As ps): the preconditioner costs no
+// D^-1 loads and no z stores, and the SpMV reads no diagonal entries (the
+// sliced ELL holds the off-diagonal entries; the unit diagonal is added).  The
+// stop test uses the unscaled residual, ||r||^2 = sum (g_i rs_i)^2.  A zero,
+// non-finite or opposite-sign diagonal entry sets g_i to NaN, so such a
+// system's solve fails (non-finite residual) rather than running another method.
+//
 // A launch covers a list of tiles; a tile is a contiguous range of one
 // system's rows (a multiple of 64) and owns one partial-sum slot.  The single-
 // system solver uses implicit tiles (logical block b covers rows [b*R, b*R+R));
@@ -9,15 +22,16 @@
 // neighbouring row tiles -- and the SpMV gathers between them -- share an L2.
 //
 // A chunk of n iterations is 2n + 1 kernels:
-//   spmv       q = A p (sliced ELL), partials of p.q          (first iteration)
+//   spmv       q = As p (sliced ELL + unit diagonal), partials of p.q
+//                                                            (first iteration)
 //   update     alpha from the p.q partials; d += alpha p, r -= alpha q,
-//              z = D^-1 r, partials of r.r and r.z
-//   dirspmv    beta and the stop decision from the r.r/r.z partials; the new
-//              direction p' = z + beta p of its own rows (stored) and of the
-//              SpMV's columns (recomputed from z and p, the same expression),
-//              q = A p', partials of p'.q                    (iterations 2..n)
+//              partials of ||G r||^2 and r.r
+//   dirspmv    beta and the stop decision from those partials; the new
+//              direction p' = r + beta p of its own rows (stored) and of the
+//              SpMV's columns (recomputed from r and p, the same expression),
+//              q = As p', partials of p'.q                   (iterations 2..n)
 //   update
-//   direction  decision and p = z + beta p after the last update
+//   direction  decision and p = r + beta p after the last update
 // The search direction alternates between two buffers inside a chunk (dirspmv
 // reads one and writes the other, since its SpMV reads the old direction of
 // neighbouring rows); the chunk starts and ends with p in the first buffer.
@@ -25,10 +39,10 @@
 // which large FP64 systems use instead (spmv, update, direction per iteration;
 // see cg_fuse_direction).
 // The FP64 residual replacement of the mixed solver is
-//   accum      x += d, d = 0
-//   resid      r64 = b - A x (FP64 CSR), partials of r.r and b.b
-//   replace    relative residual from those partials; r = r64, z = D^-1 r,
-//              partials of r.r and r.z (a direction kernel in mode 1 follows)
+//   accum      x += |G|^-1 d, d = 0
+//   resid      r64 = b - A x (FP64 CSR, unscaled), partials of r.r and b.b
+//   replace    relative residual from those partials; r = G^-1 r64,
+//              partials of ||G r||^2 and r.r (a direction kernel in mode 1 follows)
 // The replacement runs only for systems that stopped (stop == 1: the FP32
 // residual fell by the trigger factor, its estimate reached the tolerance, or
 // the iteration limit); the others go on iterating across chunks.
@@ -45,9 +59,9 @@ namespace {
 template <typename S>
 struct CgSys {
     int32_t N, tile0, ntiles, pad;  // rows; the system's slots [tile0, tile0 + ntiles)
-    const int64_t* sellOff;         // sliced ELL, 64-row slices, entries column-major
-    const int32_t* sellCol;
-    const S* sval;
+    const int64_t* sellOff;         // sliced ELL of the off-diagonal entries, 64-row
+    const int32_t* sellCol;         // slices, entries column-major
+    const S* sval;                  // their scaled values a_ij / (g_i |g_j|)
     const int32_t* sellSrc;         // CSR index of each sliced-ELL entry (-1: padding)
     const int32_t* diag;            // CSR slot of the diagonal within each row
     const int32_t* rowOffs;         // FP64 CSR (residual replacement)
@@ -56,7 +70,8 @@ struct CgSys {
     const double* rhs;
     double* x;
     double* xold;                   // optional copy target after the solve (may be null)
-    S *d, *r, *z, *p, *q, *dinv;    // d: correction (the solution itself in plain FP64)
+    S *d, *r, *p, *q, *g;           // d: scaled correction (x += |G|^-1 d); g: signed
+                                    // square roots of the diagonal
     S* p2;                          // second search-direction buffer
     double* r64;
     CgState* st;
@@ -167,7 +182,7 @@ __global__ __launch_bounds__(kCgBlock) void kc_spmv(const CgGrid<S>* __restrict_
     if (!cg_locate(g, Pp, row0, row1, slot)) return;
     const CgSys<S>& P = *Pp;
     if (P.st->stop) return;
-    // One row per thread and pass, the first kSpE entries with their loads
+    // One row per thread and pass, the first kSpE off-diagonal entries with their loads
     // issued together (values and columns, then the gathers of p), and the
     // rows software-pipelined: the next row's slice offsets, values and
     // columns are in flight while this row's gathers complete, so a row costs
@@ -192,7 +207,7 @@ __global__ __launch_bounds__(kCgBlock) void kc_spmv(const CgGrid<S>* __restrict_
         for (int k = 0; k < kSpE; ++k)
             if (k < rp.w) pv[k] = pvec[rp.col[k]];
         if (r + kCgBlock < row1) rp.fetch(soff, scol, sval, r + kCgBlock);
-        S sum = 0;
+        S sum = own;  // the unit diagonal, then the off-diagonal entries in column order
 #pragma unroll
         for (int k = 0; k < kSpE; ++k)
             if (k < rp.w) sum += rp.v[k] * pv[k];
@@ -227,7 +242,7 @@ __global__ __launch_bounds__(kCgBlock) void kc_dirspmv(const CgGrid<S>* __restri
     const int64_t* __restrict__ soff = P.sellOff;
     const int32_t* __restrict__ scol = P.sellCol;
     const S* __restrict__ sval = P.sval;
-    const S* __restrict__ zv = P.z;
+    const S* __restrict__ zv = P.r;  // z = r on the scaled system
     const S* __restrict__ po = cg_pbuf(P, pOld);
     S* __restrict__ pn = cg_pbuf(P, pNew);
     S* __restrict__ qvec = P.q;
@@ -273,7 +288,8 @@ __global__ __launch_bounds__(kCgBlock) void kc_dirspmv(const CgGrid<S>* __restri
             gather(r);
             if (r + kCgBlock < row1) rp.fetch(soff, scol, sval, r + kCgBlock);
         }
-        S sum = 0;
+        const S own = zr + beta * pr;
+        S sum = r < N ? own : S(0);
 #pragma unroll
         for (int k = 0; k < kSpE; ++k)
             if (k < rp.w) sum += rp.v[k] * (zc[k] + beta * pc[k]);
@@ -282,7 +298,6 @@ __global__ __launch_bounds__(kCgBlock) void kc_dirspmv(const CgGrid<S>* __restri
             sum += sval[rp.b + 64 * (int64_t)k] * (zv[c] + beta * po[c]);
         }
         if (r < N) {
-            const S own = zr + beta * pr;
             pn[r] = own;
             qvec[r] = sum;
             pq += (double)own * (double)sum;
@@ -316,7 +331,7 @@ __global__ __launch_bounds__(kCgBlock) void kc_update(const CgGrid<S>* __restric
     // the first group's loads go out before the reduction of p.q
     const int r1 = min(row1, P.N);
     const int i0 = row0 + (int)threadIdx.x;
-    S dv[kCgU], pv[kCgU], rv[kCgU], qv[kCgU], iv[kCgU];
+    S dv[kCgU], pv[kCgU], rv[kCgU], qv[kCgU], gv[kCgU];
     auto load = [&](int base) {
 #pragma unroll
         for (int u = 0; u < kCgU; ++u) {
@@ -326,7 +341,7 @@ __global__ __launch_bounds__(kCgBlock) void kc_update(const CgGrid<S>* __restric
                 pv[u] = pvec[i];
                 rv[u] = P.r[i];
                 qv[u] = P.q[i];
-                iv[u] = P.dinv[i];
+                gv[u] = P.g[i];
             }
         }
     };
@@ -346,10 +361,9 @@ __global__ __launch_bounds__(kCgBlock) void kc_update(const CgGrid<S>* __restric
                 P.d[i] = dv[u] + a * pv[u];
                 const S ri = rv[u] - a * qv[u];
                 P.r[i] = ri;
-                const S zi = iv[u] * ri;
-                P.z[i] = zi;
-                rr += (double)ri * (double)ri;
-                rz += (double)ri * (double)zi;
+                const double ur = (double)ri * (double)gv[u];  // unscaled residual
+                rr += ur * ur;
+                rz += (double)ri * (double)ri;
             }
         }
     }
@@ -376,7 +390,7 @@ __global__ __launch_bounds__(kCgBlock) void kc_direction(const CgGrid<S>* __rest
     // the old direction: the chunk's last buffer (mode 0) or that of the
     // system's last update (mode 1); the new one goes to the first buffer
     const S* __restrict__ po = cg_pbuf(P, MODE == 1 ? st->pbuf : pOld);
-    // the first kCgU rows' z and p are loaded before the reduction
+    // the first kCgU rows' r (= z) and p are loaded before the reduction
     const int r1 = min(row1, P.N);
     const int i0 = row0 + (int)threadIdx.x;
     S zv[kCgU], pv[kCgU];
@@ -385,7 +399,7 @@ __global__ __launch_bounds__(kCgBlock) void kc_direction(const CgGrid<S>* __rest
         for (int u = 0; u < kCgU; ++u) {
             const int i = base + u * kCgBlock;
             if (i < r1) {
-                zv[u] = P.z[i];
+                zv[u] = P.r[i];
                 pv[u] = po[i];
             }
         }
@@ -424,16 +438,18 @@ __global__ __launch_bounds__(kCgBlock) void kc_direction(const CgGrid<S>* __rest
     }
 }
 
-template <typename S>
+// x += |G|^-1 d, d = 0: for the systems that stopped (ALL = 0), or for every
+// system (ALL = 1: the end of a plain FP64 solve)
+template <typename S, int ALL>
 __global__ __launch_bounds__(kCgBlock) void kc_accum(const CgGrid<S>* __restrict__ g) {
     const CgSys<S>* Pp;
     int row0, row1, slot;
     if (!cg_locate(g, Pp, row0, row1, slot)) return;
     const CgSys<S>& P = *Pp;
-    if (P.st->stop != 1) return;
+    if (!ALL && P.st->stop != 1) return;
     const int r1 = min(row1, P.N);
     for (int i = row0 + threadIdx.x; i < r1; i += kCgBlock) {
-        P.x[i] += (double)P.d[i];
+        P.x[i] += (double)P.d[i] / fabs((double)P.g[i]);
         P.d[i] = S(0);
     }
 }
@@ -498,8 +514,8 @@ __global__ __launch_bounds__(kCgBlock) void kc_resid(const CgGrid<S>* __restrict
     write_slot<2>(v, red.rb, slot, sh);
 }
 
-// relative FP64 residual from the resid partials; r = (S) r64, z = D^-1 r,
-// START: p = z and ||b||; partials of r.r and r.z
+// relative FP64 residual from the resid partials; r = (S) (r64 / g) (z = r),
+// START: p = r and ||b||; partials of ||G r||^2 and r.r
 template <typename S, int START>
 __global__ __launch_bounds__(kCgBlock) void kc_replace(const CgGrid<S>* __restrict__ g, CgPartials red) {
     __shared__ double sh[kCgShared];
@@ -533,13 +549,13 @@ __global__ __launch_bounds__(kCgBlock) void kc_replace(const CgGrid<S>* __restri
     double rr = 0.0, rz = 0.0;
     const int r1 = min(row1, P.N);
     for (int i = row0 + threadIdx.x; i < r1; i += kCgBlock) {
-        const S ri = (S)P.r64[i];
-        const S zi = P.dinv[i] * ri;
+        const double gi = (double)P.g[i];
+        const S ri = (S)(P.r64[i] / gi);
         P.r[i] = ri;
-        P.z[i] = zi;
-        if (START) P.p[i] = zi;
-        rr += (double)ri * (double)ri;
-        rz += (double)ri * (double)zi;
+        if (START) P.p[i] = ri;
+        const double ur = (double)ri * gi;
+        rr += ur * ur;
+        rz += (double)ri * (double)ri;
     }
     double v[2] = {rr, rz};
     write_slot<2>(v, red.rz, slot, sh);
@@ -566,29 +582,26 @@ struct CgStartParam {
     int32_t maxit, pad;
 };
 
-// Batched start, one workgroup per tile of the (re)starting systems: the
-// tile's sliced-ELL values and Jacobi diagonal in S from the FP64 matrix, d = 0,
-// and (first tile) the system's CG state.  Tile rows are multiples of 64, so a
-// tile owns whole slices.
-template <typename S>
+// Start, one workgroup per tile of the (re)starting systems: g_i (the signed
+// square root of the diagonal, NaN for a zero, non-finite or opposite-sign
+// entry), d = 0, and (STATE, first tile) the system's CG state from prm.
+// Tile rows are multiples of 64, so a tile owns whole slices.
+template <typename S, int STATE>
 __global__ __launch_bounds__(kCgBlock) void kc_start_tile(const CgGrid<S>* __restrict__ g,
                                                           const CgStartParam* __restrict__ prm) {
     const CgSys<S>* Pp;
     int row0, row1, slot;
     if (!cg_locate(g, Pp, row0, row1, slot)) return;
     const CgSys<S>& P = *Pp;
-    S* sval = const_cast<S*>(P.sval);
-    const int64_t e0 = P.sellOff[row0 >> 6], e1 = P.sellOff[row1 >> 6];
-    for (int64_t e = e0 + threadIdx.x; e < e1; e += kCgBlock) {
-        const int32_t j = P.sellSrc[e];
-        sval[e] = j >= 0 ? (S)P.vals[j] : S(0);
-    }
+    const double d0 = P.vals[P.rowOffs[0] + P.diag[0]];
     const int r1 = min(row1, P.N);
     for (int i = row0 + threadIdx.x; i < r1; i += kCgBlock) {
-        P.dinv[i] = (S)(1.0 / P.vals[P.rowOffs[i] + P.diag[i]]);
+        const double di = P.vals[P.rowOffs[i] + P.diag[i]];
+        const bool ok = di != 0.0 && (di > 0.0) == (d0 > 0.0) && fabs(di) <= 1.7976931348623157e308;
+        P.g[i] = ok ? (S)copysign(sqrt(fabs(di)), di) : (S)NAN;
         P.d[i] = S(0);
     }
-    if (row0 == 0 && threadIdx.x == 0) {
+    if (STATE && row0 == 0 && threadIdx.x == 0) {
         const CgStartParam q = prm[Pp - g->probs];
         CgState s{};
         s.tol = q.tol;
@@ -596,6 +609,29 @@ __global__ __launch_bounds__(kCgBlock) void kc_start_tile(const CgGrid<S>* __res
         s.maxit = q.maxit;
         s.delta = q.delta;
         *P.st = s;
+    }
+}
+
+// Scaled sliced-ELL values a_ij / (g_i |g_j|) in S of the tile's rows (after
+// kc_start_tile has written g); one thread per row, so the stores of an entry
+// index are coalesced over a slice.
+template <typename S>
+__global__ __launch_bounds__(kCgBlock) void kc_scale_tile(const CgGrid<S>* __restrict__ g) {
+    const CgSys<S>* Pp;
+    int row0, row1, slot;
+    if (!cg_locate(g, Pp, row0, row1, slot)) return;
+    const CgSys<S>& P = *Pp;
+    S* sval = const_cast<S*>(P.sval);
+    for (int i = row0 + threadIdx.x; i < row1; i += kCgBlock) {
+        const int sl = i >> 6;
+        const int64_t b = P.sellOff[sl] + (i & 63);
+        const int w = (int)((P.sellOff[sl + 1] - P.sellOff[sl]) >> 6);
+        const double gi = i < P.N ? (double)P.g[i] : 1.0;
+        for (int k = 0; k < w; ++k) {
+            const int64_t pos = b + 64 * (int64_t)k;
+            const int32_t j = P.sellSrc[pos];
+            sval[pos] = j >= 0 ? (S)(P.vals[j] / (gi * fabs((double)P.g[P.sellCol[pos]]))) : S(0);
+        }
     }
 }
 
@@ -623,22 +659,17 @@ __global__ void kc_sell_values(int64_t n, const int32_t* __restrict__ src,
     }
 }
 
-template <typename S>
-__global__ void kc_dinv(int N, const int32_t* __restrict__ rowOffs, const int32_t* __restrict__ diag,
-                        const double* __restrict__ values, S* __restrict__ dinv) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < N) dinv[i] = (S)(1.0 / values[rowOffs[i] + diag[i]]);
-}
-
-// Whether a chunk fuses each direction into the next SpMV.  Measured on
-// MI355X: in FP32 the fused kernel is faster at every size (128^3 Poisson
-// 57 -> 55 us per iteration; 32^3 heat step 0.78 -> 0.60 ms); in FP64 its
-// second gather per entry (z and the old p) costs more than the launch it
-// saves once the system is large (128^3: 72.6 -> 74.6 ms per solve), while
-// small FP64 solves still gain (512-problem per-problem heat: +7 %).
+// Whether a chunk fuses each direction into the next SpMV (two kernels per
+// iteration instead of three).  The fused kernel gathers r and the old p for
+// every column instead of p alone; measured on MI355X with the scaled
+// iteration, that second gather costs more than the launch it saves once a
+// launch covers 2^20 rows or more, in both precisions (128^3 Poisson: FP32
+// 39.2 -> 37.1 ms and FP64 unfused as before; the served stream +7 %, the
+// 512-problem batch +2 %), while small systems, whose iterations are launch-
+// and reduction-bound, keep the fused form (32^3 heat step unchanged).
 template <typename S>
 inline bool cg_fuse_direction(int64_t rows) {
-    return sizeof(S) == 4 || rows < (int64_t(1) << 20);
+    return rows < (int64_t(1) << 20);
 }
 
 // Grid for `tiles` logical blocks: a multiple of 8 (one share per XCD)

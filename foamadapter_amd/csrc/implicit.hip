@@ -370,37 +370,47 @@ int fv_sparsity_create(fv_mesh* m, fv_sparsity** out) {
         sp->ownerOffset.upload(ctx, oo.data(), oo.size());
         sp->neighbourOffset.upload(ctx, no.data(), no.size());
         sp->entrySlot.upload(ctx, slot.data(), slot.size());
-        // sliced ELL (64-row slices, column-major inside a slice)
+        // sliced ELL (64-row slices, column-major inside a slice) of the whole
+        // rows, and of the rows without their diagonal entry (the CG's scaled
+        // matrix has a unit diagonal, which its SpMV adds itself)
         const int64_t C = fv_sparsity::kSlice, nS = (N + C - 1) / C;
-        std::vector<int64_t> soff(nS + 1, 0);
-        for (int64_t s = 0; s < nS; ++s) {
-            int32_t wmax = 0;
-            for (int64_t r = s * C; r < std::min<int64_t>(N, (s + 1) * C); ++r)
-                wmax = std::max(wmax, rowOffs[r + 1] - rowOffs[r]);
-            soff[s + 1] = soff[s] + C * wmax;
-        }
-        std::vector<int32_t> scol(soff[nS]), ssrc(soff[nS]);
-        for (int64_t s = 0; s < nS; ++s) {
-            const int64_t wdt = (soff[s + 1] - soff[s]) / C;
-            for (int64_t l = 0; l < C; ++l) {
-                const int64_t r = s * C + l;
-                const int32_t len = r < N ? rowOffs[r + 1] - rowOffs[r] : 0;
-                for (int64_t k = 0; k < wdt; ++k) {
-                    const int64_t pos = soff[s] + k * C + l;
-                    if (k < len) {
-                        scol[pos] = col[rowOffs[r] + k];
-                        ssrc[pos] = rowOffs[r] + (int32_t)k;
-                    } else {
+        auto sliced = [&](bool offDiag, fvhip::DBuf<int64_t>& offOut, fvhip::DBuf<int32_t>& colOut,
+                          fvhip::DBuf<int32_t>& srcOut) -> int64_t {
+            std::vector<int64_t> soff(nS + 1, 0);
+            for (int64_t s = 0; s < nS; ++s) {
+                int32_t wmax = 0;
+                for (int64_t r = s * C; r < std::min<int64_t>(N, (s + 1) * C); ++r)
+                    wmax = std::max(wmax, rowOffs[r + 1] - rowOffs[r] - (offDiag ? 1 : 0));
+                soff[s + 1] = soff[s] + C * wmax;
+            }
+            std::vector<int32_t> scol(soff[nS]), ssrc(soff[nS]);
+            for (int64_t s = 0; s < nS; ++s) {
+                const int64_t wdt = (soff[s + 1] - soff[s]) / C;
+                for (int64_t l = 0; l < C; ++l) {
+                    const int64_t r = s * C + l;
+                    int64_t k = 0;
+                    if (r < N)
+                        for (int32_t j = 0; j < rowOffs[r + 1] - rowOffs[r]; ++j) {
+                            if (offDiag && j == diag[r]) continue;
+                            const int64_t pos = soff[s] + k++ * C + l;
+                            scol[pos] = col[rowOffs[r] + j];
+                            ssrc[pos] = rowOffs[r] + j;
+                        }
+                    for (; k < wdt; ++k) {
+                        const int64_t pos = soff[s] + k * C + l;
                         scol[pos] = r < N ? (int32_t)r : 0;
                         ssrc[pos] = -1;
                     }
                 }
             }
-        }
-        sp->sellTotal = soff[nS];
-        sp->sellOff.upload(ctx, soff.data(), soff.size());
-        sp->sellCol.upload(ctx, scol.data(), scol.size());
-        sp->sellSrc.upload(ctx, ssrc.data(), ssrc.size());
+            offOut.upload(ctx, soff.data(), soff.size());
+            colOut.upload(ctx, scol.data(), scol.size());
+            srcOut.upload(ctx, ssrc.data(), ssrc.size());
+            FV_HIP(hipStreamSynchronize(ctx->stream));  // the host vectors end here
+            return soff[nS];
+        };
+        sp->sellTotal = sliced(false, sp->sellOff, sp->sellCol, sp->sellSrc);
+        sp->sellTotalX = sliced(true, sp->sellOffX, sp->sellColX, sp->sellSrcX);
         FV_HIP(hipStreamSynchronize(ctx->stream));
         *out = sp;
     });

@@ -23,6 +23,11 @@ struct fv_sparsity {
     int64_t sellTotal = 0;
     fvhip::DBuf<int64_t> sellOff;
     fvhip::DBuf<int32_t> sellCol, sellSrc;
+    // the same without the diagonal entries (the CG's SpMV on the symmetrically
+    // scaled matrix, whose diagonal is 1)
+    int64_t sellTotalX = 0;
+    fvhip::DBuf<int64_t> sellOffX;
+    fvhip::DBuf<int32_t> sellColX, sellSrcX;
 };
 
 // Solver workspace kept with a linear system between solves (one per precision).

@@ -156,7 +156,7 @@ size_t estimate_bytes(const fv_server* s, const MeshEntry& me, int kind) {
                + 8 * N * 2 + 8 * B           // x, x_old, boundary values
                + 8 * (F + B)                 // face coefficient (kappa or 1)
                + e * (size_t)me.sellTotal    // sliced-ELL matrix copy
-               + 7 * e * N + 8 * N;          // Krylov vectors, correction, FP64 residual
+               + 6 * e * N + 8 * N;          // Krylov vectors, correction, FP64 residual
     if (kind == FV_REQ_POISSON) b += 24 * N + 24 * B + 8 * (F + B) + 8 * N;  // U, phi, div
     return b + (1 << 16);
 }
@@ -565,7 +565,7 @@ int fv_server_add_mesh(fv_server* s, const fv_polymesh_desc* d, int renumber, in
         me->F = me->mesh->nInt;
         me->B = me->mesh->nB;
         me->nnz = me->sp->nnz;
-        me->sellTotal = me->sp->sellTotal;
+        me->sellTotal = me->sp->sellTotalX;
         std::lock_guard<std::mutex> lk(s->qmu);
         s->residentBytes += s->ctx->in_use - used0;
         *mesh_id = (int)s->meshes.size();

@@ -3,8 +3,9 @@
 // criteria {iteration, relative_residual_norm}, src/compatibility/fvSolution.cpp:22-28,
 // 50-55,106-142), in FP64, and in FP32 inside FP64 iterative refinement.
 //
-// The iteration runs as the row-tile kernels of cgkernels.hpp over implicit
-// tiles of the one system (at most kCgMaxTiles tiles, rows in multiples of 64).
+// The iteration runs as the row-tile kernels of cgkernels.hpp (CG on the
+// symmetrically Jacobi-scaled system, whose iterates are Jacobi-PCG's) over
+// implicit tiles of the one system (at most kCgMaxTiles tiles, rows in multiples of 64).
 // All scalars live on the device; kernels turn into no-ops once the stop flag is
 // set, and the host polls the flag every `check` iterations, so a solve needs
 // no host round trip per iteration.  The workspace stays with the linear
@@ -23,7 +24,7 @@ constexpr int kB = kCgBlock;
 // Workspace of one CG in value type S on one linear system.
 template <typename S>
 struct CgWork : fv_solver_cache {
-    DBuf<S> dbuf, r, z, p, p2, q, dinv, sval;
+    DBuf<S> dbuf, r, p, p2, q, g, sval;
     DBuf<double> r64, pq, rz, rb;
     DBuf<CgState> st, st2;
     DBuf<CgSys<S>> sys;
@@ -35,19 +36,19 @@ struct CgWork : fv_solver_cache {
     int lastIters = 0;  // iterations of the previous solve (time steps repeat them)
     std::vector<int> lastRounds;  // FP32 iterations per round of the previous mixed solve
 
-    CgWork(fv_ctx* c, const fv_linsys* ls, bool separateCorrection) {
+    CgWork(fv_ctx* c, const fv_linsys* ls) {
         const fv_sparsity* sp = ls->sp;
         const int N = sp->nRows;
         const int R = cg_tile_rows(N, kB);
         const int tiles = (N + R - 1) / R;
         G = cg_grid(tiles);
         r.alloc(c, N);
-        z.alloc(c, N);
         p.alloc(c, N);
         p2.alloc(c, N);
         q.alloc(c, N);
-        dinv.alloc(c, N);
-        sval.alloc(c, sp->sellTotal);
+        g.alloc(c, N);
+        dbuf.alloc(c, N);
+        sval.alloc(c, sp->sellTotalX);
         r64.alloc(c, N);
         pq.alloc(c, tiles);
         rz.alloc(c, 2 * (size_t)tiles);
@@ -55,13 +56,12 @@ struct CgWork : fv_solver_cache {
         st.alloc(c, 1);
         st2.alloc(c, 1);
         sys.alloc(c, 1);
-        if (separateCorrection) dbuf.alloc(c, N);
         h.N = N;
         h.tile0 = 0;
         h.ntiles = tiles;
-        h.sellOff = sp->sellOff.p;
-        h.sellCol = sp->sellCol.p;
-        h.sellSrc = sp->sellSrc.p;
+        h.sellOff = sp->sellOffX.p;
+        h.sellCol = sp->sellColX.p;
+        h.sellSrc = sp->sellSrcX.p;
         h.diag = sp->diagOffset.p;
         h.sval = sval.p;
         h.rowOffs = sp->rowOffs.p;
@@ -69,11 +69,11 @@ struct CgWork : fv_solver_cache {
         h.vals = ls->values.p;
         h.rhs = ls->rhs.p;
         h.r = r.p;
-        h.z = z.p;
         h.p = p.p;
         h.p2 = p2.p;
         h.q = q.p;
-        h.dinv = dinv.p;
+        h.g = g.p;
+        h.d = dbuf.p;
         h.r64 = r64.p;
         h.st = st.p;
         grid.tiles = nullptr;
@@ -86,23 +86,20 @@ struct CgWork : fv_solver_cache {
     }
 
     // Binds the solution vector and loads the matrix (values may have changed
-    // since the last solve): sliced-ELL values and the Jacobi diagonal in S.
+    // since the last solve): the diagonal's signed square roots g, d = 0, and
+    // the scaled off-diagonal values in S.
     void bind(fv_ctx* c, const fv_linsys* ls, double* x) {
-        const fv_sparsity* sp = ls->sp;
+        (void)ls;
         hipStream_t s = c->stream;
         h.x = x;
         h.xold = nullptr;
-        h.d = dbuf.p ? dbuf.p : (S*)x;  // plain FP64 updates x in place
         FV_HIP(hipMemcpyAsync(sys.p, &h, sizeof h, hipMemcpyHostToDevice, s));
-        kc_sell_values<S><<<grid_for(sp->sellTotal, kB, 8192), kB, 0, s>>>(
-            sp->sellTotal, sp->sellSrc.p, ls->values.p, sval.p);
-        kc_dinv<S><<<grid_for(sp->nRows, kB), kB, 0, s>>>(sp->nRows, sp->rowOffs.p,
-                                                          sp->diagOffset.p, ls->values.p, dinv.p);
-        if (dbuf.p) FV_HIP(hipMemsetAsync(dbuf.p, 0, dbuf.n * sizeof(S), s));
+        kc_start_tile<S, 0><<<G, kB, 0, s>>>(gdev.p, nullptr);
+        kc_scale_tile<S><<<G, kB, 0, s>>>(gdev.p);
         FV_LAUNCH_CHECK();
     }
 
-    // FP64 residual of the current x, r = (S) r64, z = D^-1 r, p = z, rz_0
+    // FP64 residual of the current x, r = (S) (r64 / g), p = r, r.r
     void start(hipStream_t s, double tol, int maxit, double delta, double atol) {
         kc_init_state<<<1, 64, 0, s>>>(st.p, 1, tol, maxit, delta, atol);
         kc_resid<S, 1><<<G, kB, 0, s>>>(gdev.p, red);
@@ -130,7 +127,7 @@ struct CgWork : fv_solver_cache {
         kc_direction<S, 0><<<G, kB, 0, s>>>(gdev.p, red, (n - 1) & 1);
     }
     void replacement(hipStream_t s) {
-        kc_accum<S><<<G, kB, 0, s>>>(gdev.p);
+        kc_accum<S, 0><<<G, kB, 0, s>>>(gdev.p);
         kc_resid<S, 0><<<G, kB, 0, s>>>(gdev.p, red);
         kc_replace<S, 0><<<G, kB, 0, s>>>(gdev.p, red);
         kc_direction<S, 1><<<G, kB, 0, s>>>(gdev.p, red, 0);
@@ -140,6 +137,11 @@ struct CgWork : fv_solver_cache {
         FV_HIP(hipMemcpyAsync(&o, st.p, sizeof o, hipMemcpyDeviceToHost, s));
         FV_HIP(hipStreamSynchronize(s));
         return o;
+    }
+    // x += |G|^-1 d, d = 0 (the plain FP64 solve's correction into x)
+    void finish(hipStream_t s) {
+        kc_accum<S, 1><<<G, kB, 0, s>>>(gdev.p);
+        FV_LAUNCH_CHECK();
     }
     // explicit FP64 relative residual ||b - A x|| / ||b|| of the bound x into st2
     void enqueue_true_residual(hipStream_t s) {
@@ -167,7 +169,7 @@ struct CgWork : fv_solver_cache {
 template <typename S>
 CgWork<S>& work_of(fv_linsys* ls, bool mixed) {
     auto& slot = ls->cg[mixed ? 1 : 0];
-    if (!slot) slot.reset(new CgWork<S>(ls->sp->mesh->ctx, ls, mixed));
+    if (!slot) slot.reset(new CgWork<S>(ls->sp->mesh->ctx, ls));
     return *static_cast<CgWork<S>*>(slot.get());
 }
 
@@ -221,8 +223,11 @@ struct PollPipe {
     }
 };
 
-// Plain FP64 CG from x; stops on the recurrence residual estimate.  *true_res
-// receives the explicit FP64 residual of the result.  A solve of a system whose
+// Plain FP64 CG from x; stops on the recurrence residual estimate.  The
+// iteration accumulates its scaled correction d; finish() adds it to x before
+// any explicit residual (adding it early, as the repeated-solve read does, only
+// moves where d restarts from zero).  *true_res receives the explicit FP64
+// residual of the result.  A solve of a system whose
 // previous solve (a time step before) took k iterations first launches k at
 // once and reads the state with the explicit residual: one synchronisation when
 // the iteration count repeats.
@@ -242,6 +247,7 @@ CgState run_cg(fv_ctx* ctx, CgWork<double>& w, double tol, int maxit, int check,
         w.chunk(s, n);
         FV_LAUNCH_CHECK();
         launched = n;
+        w.finish(s);
         w.enqueue_true_residual(s);
         h = w.state2(s, &t);
         haveTrue = true;
@@ -280,6 +286,7 @@ CgState run_cg(fv_ctx* ctx, CgWork<double>& w, double tol, int maxit, int check,
     while (pp.npend) h = pp.pop();  // chunks queued behind the stop ran as no-ops
     // a system that needed no iteration polls (blocking) after the first
     w.lastIters = h.stop ? std::max(h.it, 1) : h.it;
+    if (!haveTrue) w.finish(s);
     *true_res = haveTrue ? t.res : w.true_residual(s);
     return h;
 }
