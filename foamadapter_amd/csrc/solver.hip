@@ -13,6 +13,7 @@
 #include "cgkernels.hpp"
 
 #include <cmath>
+#include <deque>
 #include <vector>
 
 using namespace fvhip;
@@ -305,7 +306,11 @@ CgState run_cg(fv_ctx* ctx, CgWork<double>& w, double tol, int maxit, int check,
 // Rounds are queued and read like run_cg's chunks (two polls in flight), and
 // once a rate is known each round is sized to end just after the predicted
 // stop of the FP32 CG (drop trigger or tolerance), where its replacement
-// follows.  The rounds of a solve are recorded: the next solve of the same
+// follows.  The prediction runs through the rounds already queued: a segment
+// (from one replacement to the next trigger) whose stop they cover is followed
+// by the next segment, about as long (the residual falls by the same factor),
+// so the next round is sized for that one and the queue does not drain at
+// every replacement.  The rounds of a solve are recorded: the next solve of the same
 // system (a time step) queues the same rounds at once and reads the state
 // once, continuing adaptively only if that did not converge.
 struct IrOut {
@@ -325,12 +330,36 @@ IrOut run_mixed(fv_ctx* ctx, CgWork<float>& w, double tol, int maxit, int chunk,
     // count): what the next solve replays
     std::vector<int> sched;
     int done = 0;  // iterations accounted in sched
+    std::deque<int> inflight;  // sizes of the rounds queued after the polled state
+    ConvRate cr;
     auto round = [&](int n) {
         w.chunk(s, n);
         w.replacement(s);  // only once the FP32 CG stopped (trigger, estimate, limit)
         FV_LAUNCH_CHECK();
         launched += n;
         ++rounds;
+    };
+    // Iterations the next round should launch, from the polled state h and the
+    // rounds queued after it; -1 while no convergence rate is known, 0 when the
+    // queued rounds are predicted to reach the tolerance.
+    auto plan = [&](const CgState& h, double fin) -> int {
+        double tgt = fin;
+        if (h.bnorm > 0.0 && h.rrep > 0.0) tgt = std::max(tgt, delta * h.rrep / h.bnorm);
+        int left = cr.remaining(h.res, tgt);  // to the stop of the segment in progress
+        if (left < 0) return -1;
+        for (int n : inflight) {
+            if (left > n) {
+                left -= n;
+                continue;
+            }
+            // the segment stops in this round; the replacement at its end
+            // starts the next one (or the solve ends at the tolerance)
+            if (tgt <= fin) return 0;
+            const double next = std::max(fin, tgt * delta);
+            left = cr.remaining(tgt, next);
+            tgt = next;
+        }
+        return left + kPredictMargin;
     };
     chunk = std::max(chunk, 1);
     if (!w.lastRounds.empty()) {
@@ -349,28 +378,24 @@ IrOut run_mixed(fv_ctx* ctx, CgWork<float>& w, double tol, int maxit, int chunk,
         if (h.it > done) sched.push_back(h.it - done);
         done = h.it;
     }
-    ConvRate cr;
     cr.observe(h.it, h.res);
     int ramp = 8;
     while (h.stop < 2 && h.it < maxit) {
         bool queued = false;
         if (pp.npend < 2 && launched < maxit && rounds < max_outer) {
-            // where the FP32 CG will stop: the drop trigger or the tolerance
-            double target = std::max(tol, atol > 0.0 && h.bnorm > 0.0 ? atol / h.bnorm : 0.0);
-            if (h.bnorm > 0.0 && h.rrep > 0.0) target = std::max(target, delta * h.rrep / h.bnorm);
-            const int R = h.it > 0 ? cr.remaining(h.res, target) : -1;
-            int need;
-            if (R < 0) {
+            // the final target; the FP32 CG also stops at each drop trigger
+            const double fin = std::max(tol, atol > 0.0 && h.bnorm > 0.0 ? atol / h.bnorm : 0.0);
+            int need = h.it > 0 ? plan(h, fin) : -1;
+            if (need < 0) {
                 need = ramp;
                 ramp = std::min(2 * ramp, chunk);
-            } else {
-                need = R + kPredictMargin - (launched - h.it);
             }
             if (pp.npend == 0) need = std::max(need, 1);
             need = std::min(std::min(need, chunk), maxit - launched);
             if (need > 0) {
                 round(need);
                 pp.push(s, w.st.p);
+                inflight.push_back(need);
                 queued = true;
             }
         }
@@ -380,6 +405,7 @@ IrOut run_mixed(fv_ctx* ctx, CgWork<float>& w, double tol, int maxit, int chunk,
         }
         if (queued && pp.npend < 2) continue;
         h = pp.pop();
+        inflight.pop_front();
         cr.observe(h.it, h.res);
         if (h.it > done) sched.push_back(h.it - done);  // one poll per round, in order
         done = std::max(done, h.it);
