@@ -107,9 +107,9 @@ void build_tables(fv_batch* b) {
         p.tile0 = (int)ht.size();
         member_tile_list<S>(b, i, p.tile0, ht);
         p.ntiles = (int)ht.size() - p.tile0;
-        p.sellOff = sp->sellOffX.p;
-        p.sellCol = sp->sellColX.p;
-        p.sellSrc = sp->sellSrcX.p;
+        p.sellOff = sp->sellOff.p;
+        p.sellCol = sp->sellCol.p;
+        p.sellSrc = sp->sellSrc.p;
         p.diag = sp->diagOffset.p;
         p.sval = (const S*)m->sval.p;
         p.rowOffs = sp->rowOffs.p;
@@ -350,7 +350,7 @@ BMember* batch_add(fv_batch* b, fv_linsys* ls, double* x, double* xold, void* us
         m->user = user;
         const size_t e = elem(b->prec);
         m->vec.alloc(b->ctx, 6 * (size_t)m->N * e + (size_t)m->N * sizeof(double));
-        m->sval.alloc(b->ctx, (size_t)ls->sp->sellTotalX * e);
+        m->sval.alloc(b->ctx, (size_t)ls->sp->sellTotal * e);
         m->slot = grow_slots(b);
     } catch (...) {
         delete m;

@@ -4,14 +4,19 @@
 // and the backward-Euler advection solve of test/test_advection.cpp:171-178),
 // in FP64.
 //
-// Preconditioned BiCGStab (van der Vorst), M = diag(A):
-//   r = b - A x, rhat = r, p = r at the first iteration, then per iteration i
+// Right-preconditioned BiCGStab (van der Vorst), M = D = diag(A), run as
+// BiCGStab on the column-scaled matrix As = A D^-1 (unit diagonal) with
+// u = D x: the iterates and the residual r = b - A x are the same, while the
+// preconditioner costs no y = M^-1 p or z = M^-1 s vectors and the SpMVs read
+// no diagonal entries (the off-diagonal sliced ELL, a_ij / a_jj; the unit
+// diagonal is added):
+//   r = b - A x (CSR), rhat = r, p = r at the first iteration, then per iteration i
 //   K1  rho_i = rhat.r, stop test on ||r||; beta = (rho_i/rho_{i-1})(alpha/omega)
-//       p = r + beta (p - omega v), y = M^-1 p
-//   K2  v = A y                                   partials rhat.v
-//   K3  alpha = rho_i / rhat.v, s = r - alpha v, z = M^-1 s
-//   K4  t = A z                                   partials t.s, t.t
-//   K5  omega = t.s / t.t, x += alpha y + omega z, r = s - omega t
+//       p = r + beta (p - omega v)
+//   K2  v = As p                                  partials rhat.v
+//   K3  alpha = rho_i / rhat.v, s = r - alpha v
+//   K4  t = As s                                  partials t.s, t.t
+//   K5  omega = t.s / t.t, x += D^-1 (alpha p + omega s), r = s - omega t
 //                                                 partials r.r, rhat.r
 // on the row tiles of cgkernels.hpp: every consuming block sums the producer's
 // per-tile partials itself in one fixed order (no atomics, no fences), each
@@ -33,12 +38,15 @@ struct BiState {
 
 struct BiSys {
     int32_t N, ntiles, rowsPerBlock, pad;
-    const int64_t* sellOff;
+    const int64_t* sellOff;  // off-diagonal sliced ELL
     const int32_t* sellCol;
-    const double* sval;
+    const double* sval;      // a_ij / a_jj
+    const int32_t* rowOffs;  // the CSR of A (the explicit residuals)
+    const int32_t* colIdxs;
+    const double* vals;
     const double* rhs;
     const double* dinv;
-    double *x, *r, *rhat, *p, *v, *y, *s, *z, *t;
+    double *x, *r, *rhat, *p, *v, *s, *t;
     double *pv, *pt, *pr, *pb;  // partial slots: rhat.v | t.s, t.t | r.r, rhat.r | b.b
     BiState* st;
 };
@@ -53,8 +61,8 @@ __device__ inline bool bi_locate(const BiSys& P, int& row0, int& row1, int& slot
     return true;
 }
 
-// (A vec)_i for this thread's rows of [row0, row1), handed to emit(i, value)
-// for i < N.  The rows are software-pipelined as in the CG's SpMV: a row's
+// (As vec)_i for this thread's rows of [row0, row1), handed to emit(i, value)
+// for i < N: the unit diagonal, then the off-diagonal entries.  The rows are software-pipelined as in the CG's SpMV: a row's
 // first kSpE values and columns are loaded together, its gathers are issued,
 // and the next row's slice offsets, values and columns are fetched before the
 // gathers are waited for.  Entries are summed in column order.
@@ -74,8 +82,9 @@ __device__ inline void sell_rows(const BiSys& P, int row0, int row1,
 #pragma unroll
         for (int k = 0; k < kSpE; ++k)
             if (k < rp.w) g[k] = vec[rp.col[k]];
+        const double own = r < P.N ? vec[r] : 0.0;
         if (r + kB < row1) rp.fetch(soff, scol, sval, r + kB);
-        double sum = 0.0;
+        double sum = own;
 #pragma unroll
         for (int k = 0; k < kSpE; ++k)
             if (k < rp.w) sum += rp.v[k] * g[k];
@@ -87,20 +96,26 @@ __device__ inline void sell_rows(const BiSys& P, int row0, int row1,
     }
 }
 
-// r = b - A x, rhat = r; partials r.r, rhat.r (= r.r) and b.b
+// r = b - A x (CSR rows in column order), rhat = r; partials r.r, rhat.r
+// (= r.r) and b.b
 __global__ __launch_bounds__(kB) void kbi_start(const BiSys* __restrict__ Pp) {
     __shared__ double sh[kCgShared];
     const BiSys& P = *Pp;
     int row0, row1, slot;
     if (!bi_locate(P, row0, row1, slot)) return;
     double rr = 0.0, bb = 0.0;
-    sell_rows(P, row0, row1, P.x, [&](int i, double ax) {
-        const double ri = P.rhs[i] - ax;
+    const int r1 = min(row1, P.N);
+    for (int i = row0 + threadIdx.x; i < r1; i += kB) {
+        const int k0 = P.rowOffs[i], k1 = P.rowOffs[i + 1];
+        double ax = 0.0;
+        for (int k = k0; k < k1; ++k) ax += P.vals[k] * P.x[P.colIdxs[k]];
+        const double bi = P.rhs[i];
+        const double ri = bi - ax;
         P.r[i] = ri;
         P.rhat[i] = ri;
         rr += ri * ri;
-        bb += P.rhs[i] * P.rhs[i];
-    });
+        bb += bi * bi;
+    }
     double v[2] = {rr, rr}, w[1] = {bb};
     write_slot<2>(v, P.pr, slot, sh);
     write_slot<1>(w, P.pb, slot, sh);
@@ -158,23 +173,16 @@ __global__ __launch_bounds__(kB) void kbi_dir(const BiSys* __restrict__ Pp) {
     if (stop) return;
     const int r1 = min(row1, P.N);
     if (it == 0) {
-        for (int i = row0 + threadIdx.x; i < r1; i += kB) {
-            const double pi = P.r[i];
-            P.p[i] = pi;
-            P.y[i] = P.dinv[i] * pi;
-        }
+        for (int i = row0 + threadIdx.x; i < r1; i += kB) P.p[i] = P.r[i];
     } else {
         const double beta = (rho / st->rho[(it - 1) & 1]) * (st->alpha / st->omega);
         const double om = st->omega;
-        for (int i = row0 + threadIdx.x; i < r1; i += kB) {
-            const double pi = P.r[i] + beta * (P.p[i] - om * P.v[i]);
-            P.p[i] = pi;
-            P.y[i] = P.dinv[i] * pi;
-        }
+        for (int i = row0 + threadIdx.x; i < r1; i += kB)
+            P.p[i] = P.r[i] + beta * (P.p[i] - om * P.v[i]);
     }
 }
 
-// v = A y; partials rhat.v
+// v = As p; partials rhat.v
 __global__ __launch_bounds__(kB) void kbi_spmv_v(const BiSys* __restrict__ Pp) {
     __shared__ double sh[kCgShared];
     const BiSys& P = *Pp;
@@ -182,7 +190,7 @@ __global__ __launch_bounds__(kB) void kbi_spmv_v(const BiSys* __restrict__ Pp) {
     if (!bi_locate(P, row0, row1, slot)) return;
     if (P.st->stop) return;
     double rv = 0.0;
-    sell_rows(P, row0, row1, P.y, [&](int i, double vi) {
+    sell_rows(P, row0, row1, P.p, [&](int i, double vi) {
         P.v[i] = vi;
         rv += P.rhat[i] * vi;
     });
@@ -190,7 +198,7 @@ __global__ __launch_bounds__(kB) void kbi_spmv_v(const BiSys* __restrict__ Pp) {
     write_slot<1>(a, P.pv, slot, sh);
 }
 
-// alpha = rho / rhat.v; s = r - alpha v; z = M^-1 s
+// alpha = rho / rhat.v; s = r - alpha v
 __global__ __launch_bounds__(kB) void kbi_s(const BiSys* __restrict__ Pp) {
     __shared__ double sh[kCgShared];
     const BiSys& P = *Pp;
@@ -204,13 +212,11 @@ __global__ __launch_bounds__(kB) void kbi_s(const BiSys* __restrict__ Pp) {
     if (row0 == 0 && threadIdx.x == 0) st->alpha = alpha;
     const int r1 = min(row1, P.N);
     for (int i = row0 + threadIdx.x; i < r1; i += kB) {
-        const double si = P.r[i] - alpha * P.v[i];
-        P.s[i] = si;
-        P.z[i] = P.dinv[i] * si;
+        P.s[i] = P.r[i] - alpha * P.v[i];
     }
 }
 
-// t = A z; partials t.s, t.t
+// t = As s; partials t.s, t.t
 __global__ __launch_bounds__(kB) void kbi_spmv_t(const BiSys* __restrict__ Pp) {
     __shared__ double sh[kCgShared];
     const BiSys& P = *Pp;
@@ -218,7 +224,7 @@ __global__ __launch_bounds__(kB) void kbi_spmv_t(const BiSys* __restrict__ Pp) {
     if (!bi_locate(P, row0, row1, slot)) return;
     if (P.st->stop) return;
     double ts = 0.0, tt = 0.0;
-    sell_rows(P, row0, row1, P.z, [&](int i, double ti) {
+    sell_rows(P, row0, row1, P.s, [&](int i, double ti) {
         P.t[i] = ti;
         ts += ti * P.s[i];
         tt += ti * ti;
@@ -227,7 +233,8 @@ __global__ __launch_bounds__(kB) void kbi_spmv_t(const BiSys* __restrict__ Pp) {
     write_slot<2>(a, P.pt, slot, sh);
 }
 
-// omega = t.s / t.t; x += alpha y + omega z; r = s - omega t; partials r.r, rhat.r
+// omega = t.s / t.t; x += D^-1 (alpha p + omega s); r = s - omega t; partials
+// r.r, rhat.r
 __global__ __launch_bounds__(kB) void kbi_update(const BiSys* __restrict__ Pp) {
     __shared__ double sh[kCgShared];
     const BiSys& P = *Pp;
@@ -246,8 +253,9 @@ __global__ __launch_bounds__(kB) void kbi_update(const BiSys* __restrict__ Pp) {
     double rr = 0.0, hr = 0.0;
     const int r1 = min(row1, P.N);
     for (int i = row0 + threadIdx.x; i < r1; i += kB) {
-        P.x[i] += alpha * P.y[i] + omega * P.z[i];
-        const double ri = P.s[i] - omega * P.t[i];
+        const double si = P.s[i];
+        P.x[i] += P.dinv[i] * (alpha * P.p[i] + omega * si);
+        const double ri = si - omega * P.t[i];
         P.r[i] = ri;
         rr += ri * ri;
         hr += P.rhat[i] * ri;
@@ -262,9 +270,20 @@ __global__ void kbi_dinv(int N, const int32_t* __restrict__ rowOffs, const int32
     if (i < N) dinv[i] = 1.0 / values[rowOffs[i] + diag[i]];
 }
 
+// Column-scaled off-diagonal sliced-ELL values a_ij / a_jj (padding 0)
+__global__ void kbi_scale(int64_t n, const int32_t* __restrict__ src, const int32_t* __restrict__ col,
+                          const double* __restrict__ values, const double* __restrict__ dinv,
+                          double* __restrict__ out) {
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n;
+         e += (int64_t)gridDim.x * blockDim.x) {
+        const int32_t j = src[e];
+        out[e] = j >= 0 ? values[j] * dinv[col[e]] : 0.0;
+    }
+}
+
 // Workspace kept with the linear system between solves
 struct BiWork : fv_solver_cache {
-    DBuf<double> sval, dinv, r, rhat, p, v, y, s, z, t, parts;
+    DBuf<double> sval, dinv, r, rhat, p, v, s, t, parts;
     DBuf<BiState> st;
     DBuf<BiSys> sys;
     BiSys h{};
@@ -276,7 +295,7 @@ struct BiWork : fv_solver_cache {
         const int tiles = (N + R - 1) / R;
         G = cg_grid(tiles);
         sval.alloc(c, sp->sellTotal);
-        for (DBuf<double>* b : {&dinv, &r, &rhat, &p, &v, &y, &s, &z, &t}) b->alloc(c, N);
+        for (DBuf<double>* b : {&dinv, &r, &rhat, &p, &v, &s, &t}) b->alloc(c, N);
         parts.alloc(c, 6 * (size_t)tiles);
         st.alloc(c, 1);
         sys.alloc(c, 1);
@@ -286,15 +305,16 @@ struct BiWork : fv_solver_cache {
         h.sellOff = sp->sellOff.p;
         h.sellCol = sp->sellCol.p;
         h.sval = sval.p;
+        h.rowOffs = sp->rowOffs.p;
+        h.colIdxs = sp->colIdxs.p;
+        h.vals = ls->values.p;
         h.rhs = ls->rhs.p;
         h.dinv = dinv.p;
         h.r = r.p;
         h.rhat = rhat.p;
         h.p = p.p;
         h.v = v.p;
-        h.y = y.p;
         h.s = s.p;
-        h.z = z.p;
         h.t = t.p;
         h.pv = parts.p;
         h.pt = parts.p + tiles;
@@ -323,10 +343,10 @@ int fv_solve_bicgstab(fv_linsys* ls, void* d_x, const fv_solve_opts* o, fv_solve
         BiWork& w = *static_cast<BiWork*>(slot.get());
         w.h.x = (double*)d_x;
         FV_HIP(hipMemcpyAsync(w.sys.p, &w.h, sizeof w.h, hipMemcpyHostToDevice, s));
-        kc_sell_values<double><<<grid_for(sp->sellTotal, kB, 8192), kB, 0, s>>>(
-            sp->sellTotal, sp->sellSrc.p, ls->values.p, w.sval.p);
         kbi_dinv<<<grid_for(sp->nRows, kB), kB, 0, s>>>(sp->nRows, sp->rowOffs.p, sp->diagOffset.p,
                                                         ls->values.p, w.dinv.p);
+        kbi_scale<<<grid_for(sp->sellTotal, kB, 8192), kB, 0, s>>>(
+            sp->sellTotal, sp->sellSrc.p, sp->sellCol.p, ls->values.p, w.dinv.p, w.sval.p);
         kbi_start<<<w.G, kB, 0, s>>>(w.sys.p);
         kbi_start_state<<<1, kB, 0, s>>>(w.sys.p, o->rel_tol, o->abs_tol, o->max_iter);
         FV_LAUNCH_CHECK();

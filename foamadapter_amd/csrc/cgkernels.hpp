@@ -649,16 +649,6 @@ __global__ void kc_init_state(CgState* st, int n, double tol, int maxit, double 
     }
 }
 
-template <typename S>
-__global__ void kc_sell_values(int64_t n, const int32_t* __restrict__ src,
-                               const double* __restrict__ values, S* __restrict__ out) {
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-         i += (int64_t)gridDim.x * blockDim.x) {
-        const int32_t j = src[i];
-        out[i] = j >= 0 ? (S)values[j] : (S)0;
-    }
-}
-
 // Whether a chunk fuses each direction into the next SpMV (two kernels per
 // iteration instead of three).  The fused kernel gathers r and the old p for
 // every column instead of p alone; measured on MI355X with the scaled

@@ -370,17 +370,17 @@ int fv_sparsity_create(fv_mesh* m, fv_sparsity** out) {
         sp->ownerOffset.upload(ctx, oo.data(), oo.size());
         sp->neighbourOffset.upload(ctx, no.data(), no.size());
         sp->entrySlot.upload(ctx, slot.data(), slot.size());
-        // sliced ELL (64-row slices, column-major inside a slice) of the whole
-        // rows, and of the rows without their diagonal entry (the CG's scaled
-        // matrix has a unit diagonal, which its SpMV adds itself)
+        // sliced ELL (64-row slices, column-major inside a slice) of the rows
+        // without their diagonal entry: the solvers' scaled matrices have a unit
+        // diagonal, which their SpMVs add themselves
         const int64_t C = fv_sparsity::kSlice, nS = (N + C - 1) / C;
-        auto sliced = [&](bool offDiag, fvhip::DBuf<int64_t>& offOut, fvhip::DBuf<int32_t>& colOut,
+        auto sliced = [&](fvhip::DBuf<int64_t>& offOut, fvhip::DBuf<int32_t>& colOut,
                           fvhip::DBuf<int32_t>& srcOut) -> int64_t {
             std::vector<int64_t> soff(nS + 1, 0);
             for (int64_t s = 0; s < nS; ++s) {
                 int32_t wmax = 0;
                 for (int64_t r = s * C; r < std::min<int64_t>(N, (s + 1) * C); ++r)
-                    wmax = std::max(wmax, rowOffs[r + 1] - rowOffs[r] - (offDiag ? 1 : 0));
+                    wmax = std::max(wmax, rowOffs[r + 1] - rowOffs[r] - 1);
                 soff[s + 1] = soff[s] + C * wmax;
             }
             std::vector<int32_t> scol(soff[nS]), ssrc(soff[nS]);
@@ -391,7 +391,7 @@ int fv_sparsity_create(fv_mesh* m, fv_sparsity** out) {
                     int64_t k = 0;
                     if (r < N)
                         for (int32_t j = 0; j < rowOffs[r + 1] - rowOffs[r]; ++j) {
-                            if (offDiag && j == diag[r]) continue;
+                            if (j == diag[r]) continue;
                             const int64_t pos = soff[s] + k++ * C + l;
                             scol[pos] = col[rowOffs[r] + j];
                             ssrc[pos] = rowOffs[r] + j;
@@ -409,8 +409,7 @@ int fv_sparsity_create(fv_mesh* m, fv_sparsity** out) {
             FV_HIP(hipStreamSynchronize(ctx->stream));  // the host vectors end here
             return soff[nS];
         };
-        sp->sellTotal = sliced(false, sp->sellOff, sp->sellCol, sp->sellSrc);
-        sp->sellTotalX = sliced(true, sp->sellOffX, sp->sellColX, sp->sellSrcX);
+        sp->sellTotal = sliced(sp->sellOff, sp->sellCol, sp->sellSrc);
         FV_HIP(hipStreamSynchronize(ctx->stream));
         *out = sp;
     });

@@ -16,18 +16,14 @@ struct fv_sparsity {
     // per stencil entry k (cell->face stencil of the mesh): CSR slot of the entry's
     // coefficient in the cell's row (-1 for boundary entries)
     fvhip::DBuf<int32_t> entrySlot;
-    // sliced-ELL copy of the structure for the solver's SpMV (slices of 64 rows,
-    // one wavefront each; entry k of row 64s+l at sellOff[s] + 64k + l):
-    // column and source CSR index (-1 for padding)
+    // sliced-ELL copy of the off-diagonal structure for the solvers' SpMVs,
+    // whose scaled matrices have a unit diagonal (slices of 64 rows, one
+    // wavefront each; entry k of row 64s+l at sellOff[s] + 64k + l): column and
+    // source CSR index (-1 for padding)
     static constexpr int kSlice = 64;
     int64_t sellTotal = 0;
     fvhip::DBuf<int64_t> sellOff;
     fvhip::DBuf<int32_t> sellCol, sellSrc;
-    // the same without the diagonal entries (the CG's SpMV on the symmetrically
-    // scaled matrix, whose diagonal is 1)
-    int64_t sellTotalX = 0;
-    fvhip::DBuf<int64_t> sellOffX;
-    fvhip::DBuf<int32_t> sellColX, sellSrcX;
 };
 
 // Solver workspace kept with a linear system between solves (one per precision).

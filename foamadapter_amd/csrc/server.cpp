@@ -565,7 +565,7 @@ int fv_server_add_mesh(fv_server* s, const fv_polymesh_desc* d, int renumber, in
         me->F = me->mesh->nInt;
         me->B = me->mesh->nB;
         me->nnz = me->sp->nnz;
-        me->sellTotal = me->sp->sellTotalX;
+        me->sellTotal = me->sp->sellTotal;
         std::lock_guard<std::mutex> lk(s->qmu);
         s->residentBytes += s->ctx->in_use - used0;
         *mesh_id = (int)s->meshes.size();

@@ -49,7 +49,7 @@ struct CgWork : fv_solver_cache {
         q.alloc(c, N);
         g.alloc(c, N);
         dbuf.alloc(c, N);
-        sval.alloc(c, sp->sellTotalX);
+        sval.alloc(c, sp->sellTotal);
         r64.alloc(c, N);
         pq.alloc(c, tiles);
         rz.alloc(c, 2 * (size_t)tiles);
@@ -60,9 +60,9 @@ struct CgWork : fv_solver_cache {
         h.N = N;
         h.tile0 = 0;
         h.ntiles = tiles;
-        h.sellOff = sp->sellOffX.p;
-        h.sellCol = sp->sellColX.p;
-        h.sellSrc = sp->sellSrcX.p;
+        h.sellOff = sp->sellOff.p;
+        h.sellCol = sp->sellCol.p;
+        h.sellSrc = sp->sellSrc.p;
         h.diag = sp->diagOffset.p;
         h.sval = sval.p;
         h.rowOffs = sp->rowOffs.p;
