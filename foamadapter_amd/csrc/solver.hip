@@ -464,9 +464,14 @@ int fv_solve_cg(fv_linsys* ls, void* d_x, const fv_solve_opts* o, fv_solver_stat
         // residual b - A x for iterative refinement
         if (!(local.final_res <= o->rel_tol) &&
             !(o->abs_tol > 0.0 && local.final_res * bnorm <= o->abs_tol))
-            throw Error{FV_ENOCONVERGE, "CG stopped at relative residual " +
-                                            std::to_string(local.final_res) + " after " +
-                                            std::to_string(local.iters) + " iterations"};
+            throw Error{FV_ENOCONVERGE,
+                        "CG stopped at relative residual " + std::to_string(local.final_res) +
+                            " after " + std::to_string(local.iters) + " iterations" +
+                            (local.final_res == local.final_res
+                                 ? ""
+                                 : " (not finite: a zero, non-finite or mixed-sign matrix "
+                                   "diagonal, which the Jacobi scaling cannot take, or a "
+                                   "non-finite matrix or right-hand side)")};
     });
     if (stats) *stats = local;
     return rc;

@@ -264,6 +264,31 @@ def test_no_convergence_reported(ctx, meshes):
     assert st.numIter == 2 and st.status == _lib.FV_ENOCONVERGE
 
 
+@pytest.mark.parametrize("precision", ["double", "mixed"])
+def test_mixed_sign_diagonal_fails_loudly(ctx, meshes, precision):
+    """The CG runs Jacobi as a symmetric scaling by the square roots of the
+    diagonal, which needs a diagonal of one sign: one cell whose implicit
+    source flips its diagonal makes the solve fail with FV_ENOCONVERGE and a
+    non-finite residual, never a silent answer of another method."""
+    from foamadapter_amd import FvError, _lib, dsl, la
+    p, m, g = meshes["hex10_jit_perm"]
+    T, kap, T0, Tb = _heat(ctx, m, g, dt=1.0, kappa=1.0)
+    N = m.n_cells
+    c = np.zeros(N)
+    c[N // 2] = 1e6
+    ls = dsl.assemble(dsl.imp.ddt(T) - dsl.imp.laplacian(kap, T) - dsl.imp.source(ctx.upload(c), T),
+                      T, 0.0, 1.0)
+    d = ls.scipy().diagonal()
+    assert (d < 0).sum() == 1 and (d > 0).sum() == N - 1     # the forced condition
+    print(f"[case] mixed-sign diagonal, {precision}", flush=True)
+    with pytest.raises(FvError) as e:
+        la.solve_cg(ls, ctx.upload(T0), {"relTol": 1e-10, "maxIter": 50, "precision": precision})
+    assert e.value.code == _lib.FV_ENOCONVERGE and "not finite" in str(e.value)
+    st = la.solve_cg(ls, ctx.upload(T0), {"relTol": 1e-10, "maxIter": 50, "precision": precision},
+                     raise_on_noconverge=False)
+    assert st.status == _lib.FV_ENOCONVERGE and not np.isfinite(st.finalResNorm)
+
+
 def test_pdesolver_set_reference(ctx, meshes):
     """PDESolver with setReference on an all-zeroGradient Poisson problem
     (neoIcoFoam.cpp:144-160 pattern): solvable, and pinned by the reference cell."""
