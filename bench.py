@@ -436,6 +436,11 @@ def cpu_baseline(case, sweep, budget_s=24.0):
                        "sample": f"serial oracle, runs per op {sruns}, {st:.1f}s"}}, parity
 
 
+# repeated CG solves are replayed as captured hipGraphs unless --cg-eager (the form
+# to trace with rocprofv3 --kernel-trace: every kernel is launched by itself)
+CG_GRAPH = True
+
+
 def bench_heat(ctx, n=32, steps=10, cpu=True):
     """Implicit heat transfer: ddt(T) - laplacian(1e-2, T), Euler dt=1e-3, 10 steps,
     Jacobi-CG to 1e-10; walls T=300 (x0), 400 (x1), 350 elsewhere.  Bytes: per step
@@ -466,7 +471,8 @@ def bench_heat(ctx, n=32, steps=10, cpu=True):
             t0 = time.perf_counter()
             for s in range(steps):
                 st = dsl.solve(eqn, T, s * 1e-3, 1e-3, solver={"relTol": 1e-10, "maxIter": 1000,
-                                                                 "precision": mode})
+                                                                 "precision": mode,
+                                                                 "graph": CG_GRAPH})
                 stats.append(st)
             ctx.sync()
             runs.append((time.perf_counter() - t0, stats, T.numpy()))
@@ -551,7 +557,7 @@ def bench_poisson(ctx, n=128, seed=1003, repeats=3, cpu=True):
             ctx.sync()
             ctx.record(50)
             st = la.solve_cg(ls, x, {"relTol": 1e-8, "maxIter": 20000, "precision": mode,
-                                     "checkEvery": 64})
+                                     "checkEvery": 64, "graph": CG_GRAPH})
             ctx.record(51)
             ms = ctx.elapsed_ms(50, 51)
             if best is None or ms < best[0]:
@@ -968,6 +974,9 @@ def main():
     ap.add_argument("--rate-low", type=float, default=40.0,
                     help="a second, lower arrival rate of the served stream (0: skip)")
     ap.add_argument("--no-icofoam", action="store_true")
+    ap.add_argument("--cg-eager", action="store_true",
+                    help="launch the implicit legs' repeated CG solves kernel by kernel instead "
+                         "of replaying them as hipGraphs (for kernel traces)")
     ap.add_argument("--ico-n", type=int, default=64, help="cells per edge of the icoFoam cavity")
     ap.add_argument("--gather-run", type=int, default=0,
                     help="bricks per XCD run of the staged gathers (0: the library default)")
@@ -975,6 +984,8 @@ def main():
                     help="first copy 2 GiB device-to-device 3 times (a known-size reference "
                          "for hardware traffic counters)")
     args = ap.parse_args()
+    global CG_GRAPH
+    CG_GRAPH = not args.cg_eager
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one MI355X: the finite-volume path runs on a single GPU "
               "(no cross-GPU partitioning is built); run with --gpus 1", file=sys.stderr)

@@ -71,7 +71,7 @@ FV_SOLVE_F64, FV_SOLVE_F32_IR = 0, 1
 class FvSolveOpts(C.Structure):
     _fields_ = [("rel_tol", C.c_double), ("max_iter", C.c_int32), ("mode", C.c_int32),
                 ("inner_rel_tol", C.c_double), ("max_outer", C.c_int32),
-                ("check_every", C.c_int32), ("abs_tol", C.c_double)]
+                ("check_every", C.c_int32), ("abs_tol", C.c_double), ("eager", C.c_int32)]
 
 
 class FvSolverStats(C.Structure):

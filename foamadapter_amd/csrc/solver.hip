@@ -9,7 +9,9 @@
 // All scalars live on the device; kernels turn into no-ops once the stop flag is
 // set, and the host polls the flag every `check` iterations, so a solve needs
 // no host round trip per iteration.  The workspace stays with the linear
-// system, so repeated solves (time steps) allocate nothing.
+// system, so repeated solves (time steps) allocate nothing, and a repeated
+// solve launches the previous solve's schedule at once -- as a captured
+// hipGraph once the same schedule comes twice in a row.
 #include "cgkernels.hpp"
 
 #include <cmath>
@@ -34,6 +36,10 @@ struct CgWork : fv_solver_cache {
     DBuf<CgGrid<S>> gdev;  // device copy of grid (the kernels' launch description)
     CgPartials red{};
     int G = 0;  // launch grid (multiple of 8)
+    // the captured replay of a repeated solve and the schedules it was made for
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    std::vector<double> gkey, seenKey;
     int lastIters = 0;  // iterations of the previous solve (time steps repeat them)
     std::vector<int> lastRounds;  // FP32 iterations per round of the previous mixed solve
 
@@ -86,18 +92,55 @@ struct CgWork : fv_solver_cache {
         red = CgPartials{pq.p, rz.p, rb.p};
     }
 
-    // Binds the solution vector and loads the matrix (values may have changed
-    // since the last solve): the diagonal's signed square roots g, d = 0, and
-    // the scaled off-diagonal values in S.
-    void bind(fv_ctx* c, const fv_linsys* ls, double* x) {
-        (void)ls;
-        hipStream_t s = c->stream;
+    ~CgWork() override { drop_graph(); }
+    void drop_graph() {
+        if (gexec) (void)hipGraphExecDestroy(gexec);
+        if (graph) (void)hipGraphDestroy(graph);
+        gexec = nullptr;
+        graph = nullptr;
+    }
+
+    // Binds the solution vector (the device copy of the system description
+    // the kernels read).
+    void bind(fv_ctx* c, double* x) {
         h.x = x;
         h.xold = nullptr;
-        FV_HIP(hipMemcpyAsync(sys.p, &h, sizeof h, hipMemcpyHostToDevice, s));
+        FV_HIP(hipMemcpyAsync(sys.p, &h, sizeof h, hipMemcpyHostToDevice, c->stream));
+    }
+    // Loads the matrix (values may have changed since the last solve): the
+    // diagonal's signed square roots g, d = 0, and the scaled off-diagonal
+    // values in S.
+    void prep(hipStream_t s) {
         kc_start_tile<S, 0><<<G, kB, 0, s>>>(gdev.p, nullptr);
         kc_scale_tile<S><<<G, kB, 0, s>>>(gdev.p);
-        FV_LAUNCH_CHECK();
+    }
+    // Runs enqueue() -- kernel launches only, whose by-value arguments `key`
+    // determines -- eagerly the first time a key is seen, as a captured
+    // hipGraph when the same key comes twice in a row, and by replaying that
+    // graph while it keeps coming.  A schedule that changes every solve (a
+    // pressure solve inside PISO) thus never pays for a capture.
+    template <typename F>
+    void replay(hipStream_t s, const std::vector<double>& key, F&& enqueue) {
+        if (gexec && key == gkey) {
+            FV_HIP(hipGraphLaunch(gexec, s));
+            return;
+        }
+        if (key != seenKey) {
+            seenKey = key;
+            enqueue();
+            return;
+        }
+        drop_graph();
+        FV_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        enqueue();
+        const hipError_t e = hipStreamEndCapture(s, &graph);
+        if (e != hipSuccess) {
+            graph = nullptr;
+            throw Error{FV_EDEVICE, std::string("hipStreamEndCapture: ") + hipGetErrorString(e)};
+        }
+        FV_HIP(hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0));
+        gkey = key;
+        FV_HIP(hipGraphLaunch(gexec, s));
     }
 
     // FP64 residual of the current x, r = (S) (r64 / g), p = r, r.r
@@ -140,18 +183,15 @@ struct CgWork : fv_solver_cache {
         return o;
     }
     // x += |G|^-1 d, d = 0 (the plain FP64 solve's correction into x)
-    void finish(hipStream_t s) {
-        kc_accum<S, 1><<<G, kB, 0, s>>>(gdev.p);
-        FV_LAUNCH_CHECK();
-    }
+    void finish(hipStream_t s) { kc_accum<S, 1><<<G, kB, 0, s>>>(gdev.p); }
     // explicit FP64 relative residual ||b - A x|| / ||b|| of the bound x into st2
     void enqueue_true_residual(hipStream_t s) {
         kc_resid<S, 1><<<G, kB, 0, s>>>(gdev.p, red);
         kc_resid_norm<S><<<1, kB, 0, s>>>(sys.p, red, st2.p);
-        FV_LAUNCH_CHECK();
     }
     double true_residual(hipStream_t s) {
         enqueue_true_residual(s);
+        FV_LAUNCH_CHECK();
         CgState o{};
         FV_HIP(hipMemcpyAsync(&o, st2.p, sizeof o, hipMemcpyDeviceToHost, s));
         FV_HIP(hipStreamSynchronize(s));
@@ -231,27 +271,38 @@ struct PollPipe {
 // residual of the result.  A solve of a system whose
 // previous solve (a time step before) took k iterations first launches k at
 // once and reads the state with the explicit residual: one synchronisation when
-// the iteration count repeats.
+// the iteration count repeats (the whole sequence, from the matrix scaling to
+// the explicit residual, replayed as a hipGraph unless `eager`).
 CgState run_cg(fv_ctx* ctx, CgWork<double>& w, double tol, int maxit, int check, double atol,
-               double* true_res) {
+               bool eager, double* true_res) {
     hipStream_t s = ctx->stream;
     static_assert(2 * sizeof(CgState) <= fv_ctx::kPollBytes, "poll area too small");
     PollPipe pp{static_cast<CgState*>(ctx_poll_host(ctx)), ctx};
-    w.start(s, tol, maxit, 0.0, atol);
-    FV_LAUNCH_CHECK();
     CgState h{}, t{};
     int launched = 0;
     bool haveTrue = false;
     check = std::max(check, 1);
     if (w.lastIters > 0) {
         const int n = std::min(w.lastIters, maxit);
-        w.chunk(s, n);
+        auto seq = [&] {
+            w.prep(s);
+            w.start(s, tol, maxit, 0.0, atol);
+            w.chunk(s, n);
+            w.finish(s);
+            w.enqueue_true_residual(s);
+        };
+        if (eager)
+            seq();
+        else
+            w.replay(s, {(double)n, tol, (double)maxit, atol}, seq);
         FV_LAUNCH_CHECK();
         launched = n;
-        w.finish(s);
-        w.enqueue_true_residual(s);
         h = w.state2(s, &t);
         haveTrue = true;
+    } else {
+        w.prep(s);
+        w.start(s, tol, maxit, 0.0, atol);
+        FV_LAUNCH_CHECK();
     }
     ConvRate cr;
     cr.observe(h.it, h.res);
@@ -319,11 +370,9 @@ struct IrOut {
 };
 
 IrOut run_mixed(fv_ctx* ctx, CgWork<float>& w, double tol, int maxit, int chunk, double delta,
-                int max_outer, double atol) {
+                int max_outer, double atol, bool eager) {
     hipStream_t s = ctx->stream;
     PollPipe pp{static_cast<CgState*>(ctx_poll_host(ctx)), ctx};
-    w.start(s, tol, maxit, delta, atol);
-    FV_LAUNCH_CHECK();
     CgState h{};
     int launched = 0, rounds = 0;
     // iterations that ran in each round (no-op launches behind a stop do not
@@ -363,20 +412,43 @@ IrOut run_mixed(fv_ctx* ctx, CgWork<float>& w, double tol, int maxit, int chunk,
     };
     chunk = std::max(chunk, 1);
     if (!w.lastRounds.empty()) {
-        int replayed = 0;
+        // the rounds of the previous solve, at once (the whole sequence from the
+        // matrix scaling on replayed as a hipGraph unless `eager`)
+        std::vector<int> rr;
         for (int n : w.lastRounds) {
-            if (launched >= maxit || rounds >= max_outer) break;
-            round(std::min(n, maxit - launched));
-            ++replayed;
+            if (launched >= maxit || (int)rr.size() >= max_outer) break;
+            rr.push_back(std::min(n, maxit - launched));
+            launched += rr.back();
         }
+        rounds = (int)rr.size();
+        auto seq = [&] {
+            w.prep(s);
+            w.start(s, tol, maxit, delta, atol);
+            for (int n : rr) {
+                w.chunk(s, n);
+                w.replacement(s);
+            }
+        };
+        if (eager) {
+            seq();
+        } else {
+            std::vector<double> key{tol, (double)maxit, delta, atol};
+            for (int n : rr) key.push_back(n);
+            w.replay(s, key, seq);
+        }
+        FV_LAUNCH_CHECK();
         h = w.state(s);
         // the replayed rounds keep their sizes; the last one ends where the state is
-        for (int i = 0; i + 1 < replayed; ++i) {
-            sched.push_back(w.lastRounds[i]);
-            done += w.lastRounds[i];
+        for (int i = 0; i + 1 < rounds; ++i) {
+            sched.push_back(rr[i]);
+            done += rr[i];
         }
         if (h.it > done) sched.push_back(h.it - done);
         done = h.it;
+    } else {
+        w.prep(s);
+        w.start(s, tol, maxit, delta, atol);
+        FV_LAUNCH_CHECK();
     }
     cr.observe(h.it, h.res);
     int ramp = 8;
@@ -434,9 +506,10 @@ int fv_solve_cg(fv_linsys* ls, void* d_x, const fv_solve_opts* o, fv_solver_stat
         double bnorm = 0.0;
         if (o->mode == FV_SOLVE_F64) {
             CgWork<double>& w = work_of<double>(ls, false);
-            w.bind(ctx, ls, x);
+            w.bind(ctx, x);
             double tr = 0.0;
-            CgState h = run_cg(ctx, w, o->rel_tol, o->max_iter, check, o->abs_tol, &tr);
+            CgState h =
+                run_cg(ctx, w, o->rel_tol, o->max_iter, check, o->abs_tol, o->eager != 0, &tr);
             local.init_res = h.res0;
             local.final_res = h.res;
             bnorm = h.bnorm;
@@ -445,11 +518,12 @@ int fv_solve_cg(fv_linsys* ls, void* d_x, const fv_solve_opts* o, fv_solver_stat
             local.true_res = tr;
         } else if (o->mode == FV_SOLVE_F32_IR) {
             CgWork<float>& w = work_of<float>(ls, true);
-            w.bind(ctx, ls, x);
+            w.bind(ctx, x);
             IrOut r = run_mixed(ctx, w, o->rel_tol, o->max_iter,
                                 o->check_every > 0 ? o->check_every : 64,
                                 o->inner_rel_tol > 0.0 ? o->inner_rel_tol : kIrDelta,
-                                o->max_outer > 0 ? o->max_outer : (1 << 30), o->abs_tol);
+                                o->max_outer > 0 ? o->max_outer : (1 << 30), o->abs_tol,
+                                o->eager != 0);
             local.init_res = r.init_res;
             local.final_res = r.res;
             bnorm = r.bnorm;

@@ -218,6 +218,8 @@ def solver_opts(solver: dict | None) -> _lib.FvSolveOpts:
     o.inner_rel_tol = float(d.get("innerRelTol", 0.0))  # 0: the library's default
     o.max_outer = int(d.get("maxOuter", 0))
     o.check_every = int(d.get("checkEvery", 32))
+    # a repeated CG solve is replayed as a captured hipGraph unless 'graph' is False
+    o.eager = 0 if d.get("graph", True) else 1
     _method(d)
     pre = d.get("preconditioner", "DIC")
     if isinstance(pre, dict):

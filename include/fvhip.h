@@ -364,6 +364,9 @@ typedef struct {
     int32_t check_every;  /* host polls the stop flag every this many iterations */
     double abs_tol;       /* also stop when ||r|| <= abs_tol (criteria absolute_residual_norm,
                              fvSolution.cpp:139-144); 0: off */
+    int32_t eager;        /* fv_solve_cg: 1 launches a repeated solve's kernels one by one;
+                             0 (default) replays them as a hipGraph once the same
+                             schedule repeats (same bits either way) */
 } fv_solve_opts;
 typedef struct {
     double init_res;      /* SolverStats::initResNorm, relative */
