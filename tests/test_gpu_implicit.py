@@ -176,9 +176,10 @@ def test_heat_transfer_steps_match_oracle(ctx, n, precision):
 
 @pytest.mark.parametrize("precision", ["double", "mixed"])
 def test_repeated_solves_replay_as_graph(ctx, precision):
-    """A repeated solve launches the previous solve's schedule at once, and from
-    the third solve with the same schedule on replays it as a captured hipGraph.
-    A 12-step heat march (32^3, 5 CG iterations per step) gives the same bits
+    """A repeated solve launches the previous solve's schedule at once, and once
+    the same schedule has come five times in a row replays it as a captured
+    hipGraph.  A 12-step heat march (32^3, 5 CG iterations per step: captured at
+    step 6, replayed at steps 7-12) gives the same bits
     and iteration counts at every step with the graph as launched kernel by
     kernel, with the march's own eager work (assembly, old-time copy, boundary
     update, the reads here) between the replays."""
