@@ -808,7 +808,8 @@ def bench_icofoam(ctx, n=64, seed=1006, warmup=2, steps=5):
     nu = fvcc.SurfaceField(m, ctx.upload(np.full(F + B, 0.01)), "nu")
     sol = IcoFoam(U, p, phi, nu, 0.5 / n, fvSchemes={"div": "Gauss upwind"},
                   fvSolution={"U": {"solver": "PBiCGStab", "tolerance": 1e-5, "relTol": 0.0},
-                              "p": {"solver": "PCG", "tolerance": 1e-6, "relTol": 0.05},
+                              "p": {"solver": "PCG", "tolerance": 1e-6, "relTol": 0.05,
+                                    "graph": CG_GRAPH},
                               "PISO": {"nCorrectors": 2, "pRefCell": 0, "pRefValue": 0}})
     with contextlib.redirect_stdout(sys.stderr):      # the solvers' log lines
         for _ in range(warmup):
