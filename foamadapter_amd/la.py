@@ -201,7 +201,9 @@ def solver_opts(solver: dict | None) -> _lib.FvSolveOpts:
     """Map an fvSolution-style solver dictionary (src/compatibility/fvSolution.cpp:106-159:
     relTol -> relative_residual_norm, maxIter -> iteration, default 1000, tolerance ->
     absolute_residual_norm), or its mapped Ginkgo form, to solver options.
-    'precision': 'mixed' selects FP32 CG inside FP64 iterative refinement."""
+    'precision': 'mixed' selects FP32 CG inside FP64 iterative refinement;
+    'graph': False launches a repeated CG solve kernel by kernel instead of
+    replaying it as a captured hipGraph (the same bits)."""
     d = dict(solver or {})
     crit = d.get("criteria", {})
     o = _lib.FvSolveOpts()
