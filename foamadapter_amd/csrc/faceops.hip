@@ -435,15 +435,36 @@ __global__ __launch_bounds__(THREADS) void k_gather(int N, int nBlocks, int run,
         const int c = c0 + (int)threadIdx.x + q * THREADS;
         vols[q] = c < c1 ? V[c] : S(1);
     }
-    for (int l = threadIdx.x; l < CELLS; l += THREADS) {
-        upS[l] = upE[l] = exS[l] = exE[l] = bS[l] = bE[l] = 0;
-        lowN[l] = 0;
-    }
-    if (threadIdx.x == 0) overflow = fits ? 0 : 1;
-    __syncthreads();
-    const int nUp = min(nF, kMaxUp);
     using In = typename Op::In;
     using BIn = typename Op::BIn;
+    const int nUp = min(nF, kMaxUp);
+    // On a fitting workgroup the own faces' owners are not read from memory: in
+    // upper-triangular order the faces of cell c are [ownStart[c],
+    // ownStart[c+1]), so each cell writes its own-face range, and its local
+    // index into the LDS slot fluxU[0][i] of each of its faces, which phase 1
+    // overwrites with the face's contribution in the one thread that reads it.
+    // The cells' offsets are loaded first, then the first pass's neighbour
+    // indices and the list records, so their latencies overlap before the
+    // barrier.
+    int oa[kPer], ob[kPer];
+#pragma unroll
+    for (int q = 0; q < kPer; ++q) {
+        const int c = c0 + (int)threadIdx.x + q * THREADS;
+        oa[q] = ob[q] = fs;
+        if (fits && c < c1) {
+            oa[q] = t.ownStart[c];
+            ob[q] = t.ownStart[c + 1];
+        }
+    }
+    int nn0[kUL] = {}, oo0[kUL] = {};
+    if (nUp > 0) {
+#pragma unroll
+        for (int u = 0; u < kUL; ++u) {
+            const int f = fs + min((int)threadIdx.x + u * THREADS, nUp - 1);
+            nn0[u] = t.nbr[f];
+            if (!fits) oo0[u] = t.own[f];
+        }
+    }
     // One pass over kUL own faces per thread (index i0 + u*THREADS).  The first
     // pass also takes this thread's list records (at most one external lower
     // face and one boundary face: the lists of a fitting workgroup are no longer
@@ -466,18 +487,38 @@ __global__ __launch_bounds__(THREADS) void k_gather(int N, int nBlocks, int run,
         bcp = L.bndCell[max(kb - 1, b0)];
         bcn = L.bndCell[min(kb + 1, b1 - 1)];
     }
+    for (int l = threadIdx.x; l < CELLS; l += THREADS) {
+        exS[l] = exE[l] = bS[l] = bE[l] = 0;
+        lowN[l] = 0;
+    }
+#pragma unroll
+    for (int q = 0; q < kPer; ++q) {
+        const int l = (int)threadIdx.x + q * THREADS;
+        const int a = oa[q] - fs, b = ob[q] - fs;  // empty unless fits and c < c1
+        upS[l] = (uint16_t)a;
+        upE[l] = (uint16_t)b;
+        for (int i = a; i < b; ++i) fluxU[0][i] = S(l);
+    }
+    if (threadIdx.x == 0) overflow = fits ? 0 : 1;
+    __syncthreads();
     for (int i0 = threadIdx.x, pass = 0; i0 < nUp || pass == 0; i0 += THREADS * kUL, ++pass) {
         // (a workgroup may own no face: then pass 0 only takes the list records;
-        // lanes past the range load the last face and store nothing)
-        int oo[kUL] = {}, nn[kUL] = {}, op_[kUL] = {}, on_[kUL] = {};
+        // lanes past the range load the last face and store nothing -- their
+        // owner is c0, not read from a slot another lane may have overwritten)
+        int oo[kUL] = {}, nn[kUL] = {};
         if (nUp > 0) {
 #pragma unroll
             for (int u = 0; u < kUL; ++u) {
-                const int f = fs + min(i0 + u * THREADS, nUp - 1);
-                oo[u] = t.own[f];
-                nn[u] = t.nbr[f];
-                op_[u] = t.own[max(f - 1, fs)];
-                on_[u] = t.own[min(f + 1, fs + nUp - 1)];
+                const int iu = i0 + u * THREADS;
+                const int f = fs + min(iu, nUp - 1);
+                if (pass == 0) {
+                    nn[u] = nn0[u];
+                    oo[u] = oo0[u];
+                } else {
+                    nn[u] = t.nbr[f];
+                    if (!fits) oo[u] = t.own[f];
+                }
+                if (fits) oo[u] = iu < nUp ? c0 + (int)fluxU[0][iu] : c0;
             }
         }
         In ein{};
@@ -511,14 +552,11 @@ __global__ __launch_bounds__(THREADS) void k_gather(int N, int nBlocks, int run,
         for (int u = 0; u < kUL; ++u) {
             const int i = i0 + u * THREADS;
             if (i >= nUp) break;
-            const int o = oo[u], n = nn[u];
+            const int n = nn[u];
             S sv[NC];
             op.make(fin[u], sv);
             for (int j = 0; j < NC; ++j) fluxU[j][i] = sv[j];
             if (fits) {
-                const int ol = o - c0;
-                if (i == 0 || op_[u] != o) upS[ol] = (uint16_t)i;
-                if (i == nF - 1 || on_[u] != o) upE[ol] = (uint16_t)(i + 1);
                 if (n < c1) {
                     const int nl = n - c0;
                     const int slot = atomicAdd(&lowN[nl], 1);
