@@ -906,7 +906,7 @@ def bench_served(n_req=96, seed=1005, rate=200.0, tile_rows=4096, chunk=16, rate
                 "cg_iteration_roofline": roof(it_bytes, wall * 1e3),
                 "warmup_completed": srv_stats0["completed"],
                 "worker_ms": {k: round(s[k] - srv_stats0[k], 1)
-                              for k in ("admit_ms", "gpu_ms", "advance_ms")},
+                              for k in ("admit_ms", "gpu_ms", "advance_ms", "solo_ms")},
                 "chunks": s["chunks"] - srv_stats0["chunks"],
                 "table_builds": s["table_builds"] - srv_stats0["table_builds"],
                 "graph_builds": s["graph_builds"] - srv_stats0["graph_builds"],

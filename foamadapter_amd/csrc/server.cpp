@@ -13,6 +13,11 @@
 //     memory freed), so requests that arrive while others run join at the next
 //     chunk; or
 //   * plain: runs one request at a time through fv_solve_cg (the drop-in path).
+// A batched server runs the requests on meshes of at least solo_rows cells --
+// systems that fill the GPU by themselves -- alone through fv_solve_cg, on a
+// second lane: its own context (stream and memory pool, with those meshes
+// resident a second time) and its own worker thread, so the device interleaves
+// these HBM-bound solves with the batch's latency-bound chunks.
 // Latency is measured from submission to the result being on the host.
 #include "batch.hpp"
 
@@ -45,6 +50,8 @@ struct MeshEntry {
     bool renumbered = false;
     int32_t N = 0, F = 0, B = 0;
     int64_t nnz = 0, sellTotal = 0;
+    // the same mesh on the solo lane's context (meshes of at least solo_rows cells)
+    std::unique_ptr<MeshEntry> solo;
 };
 
 struct Req {
@@ -57,6 +64,10 @@ struct Req {
     int status = FV_OK;
     Clock::time_point tSubmit, tStart, tDone;
     size_t estimate = 0;
+    // the lane that runs it: the server's context and mesh entry, or the solo
+    // lane's (set at admission)
+    fv_ctx* ctx = nullptr;
+    MeshEntry* me = nullptr;
     // device resources while running
     fv_linsys* ls = nullptr;
     DBuf<double> x, xold, Tb, gamma, scratch;
@@ -81,7 +92,13 @@ struct fv_server {
     std::deque<Req*> pending;
     std::unordered_map<int64_t, Req*> reqs;
     std::vector<Req*> active;
-    std::deque<Req*> solo;  // admitted requests that run alone (batched servers)
+    // the solo lane (batched servers): its context, queue, admitted requests,
+    // the users of its stream, and its worker
+    fv_ctx* soloCtx = nullptr;
+    std::deque<Req*> soloQ;
+    std::vector<Req*> soloActive;
+    std::mutex solomu;
+    std::thread soloWorker;
     int64_t nextId = 1;
     bool stop = false;
     std::thread worker;
@@ -94,6 +111,8 @@ struct fv_server {
     double admitMs = 0, gpuMs = 0, advanceMs = 0;  // worker time split (under qmu when read)
     int64_t chunks = 0, tableBuilds = 0, graphBuilds = 0;  // copies of the batch's counters
     int64_t soloRuns = 0;                                   // requests that ran alone
+    double soloMs = 0;                                      // the solo lane's solve time
+    size_t laneBytes[2] = {0, 0};  // device bytes in use per lane (each written by its thread)
     double chunkEventMs = 0, startEventMs = 0;             // and of its event timings
     size_t peakBytes = 0;
     int peakActive = 0;
@@ -179,7 +198,8 @@ void check(int rc) {
 
 // Assemble the request's system for its current step.
 void assemble(fv_server* s, Req* r) {
-    MeshEntry& me = *s->meshes[r->d.mesh_id];
+    (void)s;
+    MeshEntry& me = *r->me;
     if (r->d.kind == FV_REQ_HEAT) {
         // ddt(T) - laplacian(kappa, T), one fused pass
         check(fv_assemble_ddt_laplacian(r->ls, r->xold.p, r->d.dt, r->gamma.p, me.bcHeat, r->Tb.p,
@@ -193,10 +213,10 @@ void assemble(fv_server* s, Req* r) {
     }
 }
 
-// Build a request's fields and system on its mesh (server context stream).
+// Build a request's fields and system on its mesh (its lane's stream).
 void setup(fv_server* s, Req* r) {
-    MeshEntry& me = *s->meshes[r->d.mesh_id];
-    fv_ctx* ctx = s->ctx;
+    MeshEntry& me = *r->me;
+    fv_ctx* ctx = r->ctx;
     const int32_t N = me.N, F = me.F, B = me.B;
     check(fv_linsys_create(me.sp, &r->ls));
     r->x.alloc(ctx, N);
@@ -247,8 +267,8 @@ void setup(fv_server* s, Req* r) {
 }
 
 void finish(fv_server* s, Req* r, int status) {
-    MeshEntry& me = *s->meshes[r->d.mesh_id];
-    fv_ctx* ctx = s->ctx;
+    MeshEntry& me = *r->me;
+    fv_ctx* ctx = r->ctx;
     if (status == FV_OK) {
         r->result = pinned_take(s, me.N);
         r->resultN = me.N;
@@ -294,7 +314,7 @@ std::vector<Req*> admit(fv_server* s, bool one_at_a_time) {
             s->qcv.notify_all();
             continue;
         }
-        const int nActive = (int)s->active.size() + (int)out.size();
+        const int nActive = (int)(s->active.size() + s->soloActive.size() + out.size());
         if (one_at_a_time && nActive >= 1) break;
         if (s->opts.max_active > 0 && nActive >= s->opts.max_active) break;
         if (s->opts.hbm_budget && inUse + r->estimate > s->opts.hbm_budget && nActive > 0) break;
@@ -318,7 +338,7 @@ void run_plain(fv_server* s, Req* r) {
     const int steps = r->d.kind == FV_REQ_HEAT ? std::max(1, r->d.steps) : 1;
     for (int st = 0; st < steps; ++st) {
         if (st > 0) {
-            check(fv_memcpy_d2d(s->ctx, r->xold.p, r->x.p, r->x.n * sizeof(double)));
+            check(fv_memcpy_d2d(r->ctx, r->xold.p, r->x.p, r->x.n * sizeof(double)));
             assemble(s, r);
         }
         fv_solver_stats ss{};
@@ -344,16 +364,23 @@ void worker_loop(fv_server* s) {
         const Clock::time_point t0 = Clock::now();
         std::vector<Req*> adm = admit(s, !batched);
         for (Req* r : adm) {
+            // a system large enough to fill the GPU runs alone through the
+            // single-system solver on the solo lane (its polling and replay beat
+            // the batch tables' per-chunk round trips); the others join the batch
+            MeshEntry* base = s->meshes[r->d.mesh_id].get();
+            const bool alone = batched && base->solo != nullptr;
+            r->ctx = alone ? s->soloCtx : s->ctx;
+            r->me = alone ? base->solo.get() : base;
+            if (alone) {
+                std::lock_guard<std::mutex> lk(s->qmu);
+                s->soloActive.push_back(r);
+                s->soloQ.push_back(r);
+                s->qcv.notify_all();
+                continue;
+            }
             try {
                 setup(s, r);
-                // a system large enough to fill the GPU runs alone through the
-                // single-system solver (its polling and replay beat the batch
-                // tables' per-chunk round trips); the others join the batch
-                const bool alone = batched && s->opts.solo_rows > 0 &&
-                                   s->meshes[r->d.mesh_id]->N >= s->opts.solo_rows;
-                if (alone) {
-                    s->solo.push_back(r);
-                } else if (batched) {
+                if (batched) {
                     r->member = batch_add(s->batch, r->ls, r->x.p, r->xold.p, r);
                     r->member->tol = r->d.rel_tol;
                     r->member->maxit = r->d.max_iter;
@@ -368,18 +395,16 @@ void worker_loop(fv_server* s) {
         Clock::time_point t1 = Clock::now();
         {
             std::lock_guard<std::mutex> lk(s->qmu);
-            s->peakActive = std::max(s->peakActive, (int)s->active.size());
-            s->peakBytes = std::max(s->peakBytes, s->ctx->in_use);
+            s->peakActive =
+                std::max(s->peakActive, (int)(s->active.size() + s->soloActive.size()));
+            s->laneBytes[0] = s->ctx->in_use;
+            s->peakBytes = std::max(s->peakBytes, s->laneBytes[0] + s->laneBytes[1]);
             s->admitMs += ms_between(t0, t1);
         }
-        if (!batched || !s->solo.empty()) {
-            // one request alone: the next of a plain server, or the next large
-            // request of a batched one (then one chunk of the batch follows)
+        if (!batched) {
+            // plain: the next request, alone
             Req* r = nullptr;
-            if (batched) {
-                r = s->solo.front();
-                s->solo.pop_front();
-            } else {
+            {
                 std::lock_guard<std::mutex> lk(s->qmu);
                 if (!s->active.empty()) r = s->active.front();
             }
@@ -395,7 +420,6 @@ void worker_loop(fv_server* s) {
                 std::lock_guard<std::mutex> lk(s->qmu);
                 s->active.erase(std::find(s->active.begin(), s->active.end(), r));
                 s->gpuMs += ms_between(t1, Clock::now());
-                if (batched) ++s->soloRuns;
             }
             try {
                 finish(s, r, status);
@@ -403,8 +427,7 @@ void worker_loop(fv_server* s) {
                 set_error(e.msg);
                 finish(s, r, e.code);
             }
-            if (!batched) continue;
-            t1 = Clock::now();  // the batch's chunk below is timed on its own
+            continue;
         }
         if (s->batch->members.empty()) continue;
         try {
@@ -424,7 +447,6 @@ void worker_loop(fv_server* s) {
                 std::lock_guard<std::mutex> lk(s->qmu);
                 act.swap(s->active);
             }
-            s->solo.clear();  // (its requests are in act)
             for (Req* r : act) {
                 if (r->member) batch_remove(s->batch, r->member);
                 r->member = nullptr;
@@ -484,6 +506,92 @@ void worker_loop(fv_server* s) {
     }
 }
 
+// The solo lane: requests on the largest meshes, one at a time through
+// fv_solve_cg on the lane's own context, while the worker steps the batch.
+void solo_loop(fv_server* s) {
+    while (true) {
+        Req* r = nullptr;
+        {
+            std::unique_lock<std::mutex> lk(s->qmu);
+            s->qcv.wait(lk, [&] { return s->stop || !s->soloQ.empty(); });
+            if (s->stop) return;
+            r = s->soloQ.front();
+            s->soloQ.pop_front();
+        }
+        std::lock_guard<std::mutex> g(s->solomu);
+        const Clock::time_point t0 = Clock::now();
+        int status = FV_OK;
+        try {
+            if (r->cancel) throw Error{FV_ECANCELLED, "cancelled"};
+            setup(s, r);
+            run_plain(s, r);
+        } catch (const Error& e) {
+            set_error(e.msg);
+            status = e.code;
+        }
+        {
+            std::lock_guard<std::mutex> lk(s->qmu);
+            s->soloActive.erase(std::find(s->soloActive.begin(), s->soloActive.end(), r));
+            s->soloMs += ms_between(t0, Clock::now());
+            ++s->soloRuns;
+            s->laneBytes[1] = s->soloCtx->in_use;
+            s->peakBytes = std::max(s->peakBytes, s->laneBytes[0] + s->laneBytes[1]);
+        }
+        try {
+            finish(s, r, status);
+        } catch (const Error& e) {
+            set_error(e.msg);
+            finish(s, r, e.code);
+        }
+    }
+}
+
+// A resident mesh on context ctx from a (possibly renumbered) description:
+// device mesh, cell map, sparsity, the two boundary-condition sets and the
+// staged-gather lists.  Returns the device bytes it took.
+size_t make_entry(fv_ctx* ctx, const fv_polymesh_desc* d, const std::vector<int32_t>* cmap,
+                  MeshEntry& me) {
+    const size_t used0 = ctx->in_use;
+    check(fv_mesh_create_poly(ctx, d, &me.mesh));
+    if (cmap) {
+        me.cellMap.upload(ctx, cmap->data(), cmap->size());
+        check(fv_sync(ctx));
+        me.renumbered = true;
+    }
+    check(fv_sparsity_create(me.mesh, &me.sp));
+    const int np = me.mesh->nPatches;
+    std::vector<int32_t> kh(np, FV_BC_FIXED_VALUE), kp(np, FV_BC_FIXED_GRADIENT);
+    if (np) kp[0] = FV_BC_FIXED_VALUE;
+    check(fv_bc_create(me.mesh, kh.data(), &me.bcHeat));
+    check(fv_bc_create(me.mesh, kp.data(), &me.bcPoisson));
+    {
+        // A Poisson request's explicit surface integral builds the mesh's
+        // staged-gather lists on first use (24 ms at 128^3, 55 ms at 160^3):
+        // build them with the mesh, by the same call, not in a request.
+        DBuf<double> phi, out;
+        phi.alloc(ctx, (size_t)me.mesh->nInt + me.mesh->nB);
+        out.alloc(ctx, (size_t)me.mesh->nCells);
+        check(fv_fill(ctx, FV_F64, phi.p, (int64_t)phi.n, 0.0));
+        check(fv_div_gauss(ctx, me.mesh, FV_LINEAR, FV_F64, 1.0, phi.p, nullptr, nullptr, out.p));
+        check(fv_sync(ctx));
+    }
+    me.N = me.mesh->nCells;
+    me.F = me.mesh->nInt;
+    me.B = me.mesh->nB;
+    me.nnz = me.sp->nnz;
+    me.sellTotal = me.sp->sellTotal;
+    return ctx->in_use - used0;
+}
+
+void destroy_entry(MeshEntry& me) {
+    if (me.solo) destroy_entry(*me.solo);
+    if (me.bcHeat) fv_bc_destroy(me.bcHeat);
+    if (me.bcPoisson) fv_bc_destroy(me.bcPoisson);
+    if (me.sp) fv_sparsity_destroy(me.sp);
+    me.cellMap.release();
+    if (me.mesh) fv_mesh_destroy(me.mesh);
+}
+
 MeshEntry& mesh_of(fv_server* s, int id) {
     FV_REQUIRE(id >= 0 && id < (int)s->meshes.size(), "unknown mesh id");
     return *s->meshes[id];
@@ -508,11 +616,24 @@ int fv_server_create(int device, const fv_server_opts* opts, fv_server** out) {
         const int ch = opts->chunk > 0 ? opts->chunk : 16;
         check(fv_batch_create(s->ctx, opts->mode, tr, ch, &s->batch));
         s->batch->useGraph = opts->use_graph != 0;
+        if (s->opts.batched && s->opts.solo_rows > 0) {
+            check(fv_ctx_create(device, 0, &s->soloCtx));
+            s->soloWorker = std::thread(solo_loop, s);
+        }
         s->worker = std::thread(worker_loop, s);
         *out = s;
     });
     if (rc != FV_OK && s) {
+        if (s->soloWorker.joinable()) {
+            {
+                std::lock_guard<std::mutex> lk(s->qmu);
+                s->stop = true;
+                s->qcv.notify_all();
+            }
+            s->soloWorker.join();
+        }
         if (s->batch) fv_batch_destroy(s->batch);
+        if (s->soloCtx) fv_ctx_destroy(s->soloCtx);
         if (s->ctx) fv_ctx_destroy(s->ctx);
         delete s;
     }
@@ -523,51 +644,44 @@ int fv_server_add_mesh(fv_server* s, const fv_polymesh_desc* d, int renumber, in
     return guarded([&] {
         FV_REQUIRE(s && d && mesh_id, "NULL argument");
         auto me = std::make_unique<MeshEntry>();
-        std::lock_guard<std::mutex> g(s->gpumu);
-        const size_t used0 = s->ctx->in_use;
+        fv_polymesh_desc r = *d;
+        std::vector<int32_t> foff, fpts, own, nbr, cmap, fmap;
         if (renumber) {
             const int64_t nF = d->n_faces, N = d->n_cells;
-            std::vector<int32_t> foff(nF + 1), fpts(d->h_face_offsets[nF]), own(nF),
-                nbr(d->n_internal_faces), cmap(N), fmap(nF);
+            foff.resize(nF + 1);
+            fpts.resize(d->h_face_offsets[nF]);
+            own.resize(nF);
+            nbr.resize(d->n_internal_faces);
+            cmap.resize(N);
+            fmap.resize(nF);
             check(fv_polymesh_renumber(d, 1, foff.data(), fpts.data(), own.data(), nbr.data(),
                                        cmap.data(), fmap.data()));
-            fv_polymesh_desc r = *d;
             r.h_face_offsets = foff.data();
             r.h_face_points = fpts.data();
             r.h_owner = own.data();
             r.h_neighbour = nbr.data();
-            check(fv_mesh_create_poly(s->ctx, &r, &me->mesh));
-            me->cellMap.upload(s->ctx, cmap.data(), N);
-            check(fv_sync(s->ctx));
-            me->renumbered = true;
-        } else {
-            check(fv_mesh_create_poly(s->ctx, d, &me->mesh));
         }
-        check(fv_sparsity_create(me->mesh, &me->sp));
-        const int np = me->mesh->nPatches;
-        std::vector<int32_t> kh(np, FV_BC_FIXED_VALUE), kp(np, FV_BC_FIXED_GRADIENT);
-        if (np) kp[0] = FV_BC_FIXED_VALUE;
-        check(fv_bc_create(me->mesh, kh.data(), &me->bcHeat));
-        check(fv_bc_create(me->mesh, kp.data(), &me->bcPoisson));
+        size_t bytes = 0;
         {
-            // A Poisson request's explicit surface integral builds the mesh's
-            // staged-gather lists on first use (24 ms at 128^3, 55 ms at 160^3):
-            // build them with the mesh, by the same call, not in a request.
-            DBuf<double> phi, out;
-            phi.alloc(s->ctx, (size_t)me->mesh->nInt + me->mesh->nB);
-            out.alloc(s->ctx, (size_t)me->mesh->nCells);
-            check(fv_fill(s->ctx, FV_F64, phi.p, (int64_t)phi.n, 0.0));
-            check(fv_div_gauss(s->ctx, me->mesh, FV_LINEAR, FV_F64, 1.0, phi.p, nullptr, nullptr,
-                               out.p));
-            check(fv_sync(s->ctx));
+            std::lock_guard<std::mutex> g(s->gpumu);
+            bytes += make_entry(s->ctx, &r, renumber ? &cmap : nullptr, *me);
         }
-        me->N = me->mesh->nCells;
-        me->F = me->mesh->nInt;
-        me->B = me->mesh->nB;
-        me->nnz = me->sp->nnz;
-        me->sellTotal = me->sp->sellTotal;
+        if (s->soloCtx && me->N >= s->opts.solo_rows) {
+            // the solo lane runs requests on this mesh: resident there too
+            auto solo = std::make_unique<MeshEntry>();
+            std::lock_guard<std::mutex> g(s->solomu);
+            try {
+                bytes += make_entry(s->soloCtx, &r, renumber ? &cmap : nullptr, *solo);
+            } catch (...) {
+                destroy_entry(*solo);
+                std::lock_guard<std::mutex> g2(s->gpumu);
+                destroy_entry(*me);
+                throw;
+            }
+            me->solo = std::move(solo);
+        }
         std::lock_guard<std::mutex> lk(s->qmu);
-        s->residentBytes += s->ctx->in_use - used0;
+        s->residentBytes += bytes;
         *mesh_id = (int)s->meshes.size();
         s->meshes.push_back(std::move(me));
     });
@@ -697,6 +811,7 @@ int fv_server_get_stats(fv_server* s, fv_server_stats* o) {
         o->chunk_event_ms = s->chunkEventMs;
         o->start_event_ms = s->startEventMs;
         o->solo_runs = s->soloRuns;
+        o->solo_ms = s->soloMs;
         if (s->haveFirst && s->completed) o->wall_s = ms_between(s->tFirst, s->tLast) * 1e-3;
         if (o->wall_s > 0) o->solves_per_s = s->completed / o->wall_s;
         if (!s->latencies.empty()) {
@@ -727,18 +842,15 @@ int fv_server_destroy(fv_server* s) {
             s->qcv.notify_all();
         }
         if (s->worker.joinable()) s->worker.join();
+        if (s->soloWorker.joinable()) s->soloWorker.join();
         for (Req* r : s->active) free_request_device(r);
+        for (Req* r : s->soloActive) free_request_device(r);
         for (auto& kv : s->reqs) drop_request(s, kv.second);
         for (auto& kv : s->pinFree) (void)hipHostFree(kv.second);
         s->pinFree.clear();
         if (s->batch) fv_batch_destroy(s->batch);
-        for (auto& me : s->meshes) {
-            if (me->bcHeat) fv_bc_destroy(me->bcHeat);
-            if (me->bcPoisson) fv_bc_destroy(me->bcPoisson);
-            if (me->sp) fv_sparsity_destroy(me->sp);
-            me->cellMap.release();
-            if (me->mesh) fv_mesh_destroy(me->mesh);
-        }
+        for (auto& me : s->meshes) destroy_entry(*me);
+        if (s->soloCtx) fv_ctx_destroy(s->soloCtx);
         fv_ctx_destroy(s->ctx);
         delete s;
     });

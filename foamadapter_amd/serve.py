@@ -49,7 +49,7 @@ class FvServerStats(C.Structure):
                 ("admit_ms", C.c_double), ("gpu_ms", C.c_double), ("advance_ms", C.c_double),
                 ("chunks", C.c_int64), ("table_builds", C.c_int64), ("graph_builds", C.c_int64),
                 ("chunk_event_ms", C.c_double), ("start_event_ms", C.c_double),
-                ("solo_runs", C.c_int64)]
+                ("solo_runs", C.c_int64), ("solo_ms", C.c_double)]
 
 
 HEAT, POISSON = 0, 1

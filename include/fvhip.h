@@ -503,8 +503,10 @@ typedef struct {
     int32_t chunk;       /* CG iterations per FP64 residual replacement, 0: 16 */
     int32_t use_graph;   /* replay chunks as hipGraphs */
     int32_t solo_rows;   /* batched: a request on a mesh of at least this many cells runs
-                            alone through fv_solve_cg between chunks (a system that size
-                            fills the GPU by itself); 0: 1 << 20, negative: never */
+                            alone through fv_solve_cg on a second lane -- its own context
+                            and worker thread, the mesh resident there too -- concurrently
+                            with the batch (a system that size fills the GPU by itself);
+                            0: 1 << 20, negative: never */
 } fv_server_opts;
 typedef struct {
     int32_t mesh_id;
@@ -546,6 +548,8 @@ typedef struct {
      * around each (the part of gpu_ms the device was working) */
     double chunk_event_ms, start_event_ms;
     int64_t solo_runs;   /* batched: requests that ran alone (solo_rows) */
+    double solo_ms;      /* batched: time of those solves on the solo lane (its own
+                            context and worker thread, concurrent with the batch) */
 } fv_server_stats;
 int fv_server_create(int device, const fv_server_opts* opts, fv_server** out);
 /* Adds a resident mesh (renumber != 0: Z-order renumbered internally). */

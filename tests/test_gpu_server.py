@@ -77,6 +77,45 @@ def test_heat_requests_match_oracle(polys, batched, precision, solo):
     srv.close()
 
 
+def test_solo_lane_matches_plain_and_closes_cleanly(polys):
+    """Requests on meshes of at least solo_rows cells run on the solo lane (a
+    second context and worker thread, concurrent with the batch): their results
+    are bit-identical to the same requests through a plain server (the same
+    fv_solve_cg), while the batch serves the small mesh; a queued solo request
+    can be cancelled, and one never collected is released when the server
+    closes with both workers running."""
+    work = [("poisson", 2, 11), ("heat", 1, 12), ("heat", 0, 13), ("poisson", 1, 14),
+            ("heat", 2, 15), ("poisson", 0, 16)]
+
+    def submit(srv, ids, kind, k, seed):
+        if kind == "poisson":
+            return srv.submit_poisson(ids[k], seed=seed, rel_tol=1e-8)
+        return srv.submit_heat(ids[k], kappa=2e-2, steps=2, seed=seed, rel_tol=1e-8)
+
+    print("[case] solo lane vs plain", flush=True)
+    plain, pids = _server(polys, batched=False, precision="mixed", chunk=8)
+    ref = [plain.wait(submit(plain, pids, *w), pids[w[1]]) for w in work]
+    plain.close()
+    srv, ids = _server(polys, batched=True, precision="mixed", tile_rows=256, chunk=8,
+                       solo_rows=500)
+    rids = [submit(srv, ids, *w) for w in work]
+    extra = [srv.submit_poisson(ids[2], seed=s) for s in (31, 32, 33)]
+    srv.cancel(extra[2])
+    for (kind, k, seed), rid, (x0, st0) in zip(work, rids, ref):
+        x, st = srv.wait(rid, ids[k])
+        assert st.status == 0 and st0.status == 0, (kind, k)
+        if k > 0:                                    # the 9^3 and 12^3 meshes: solo lane
+            assert np.array_equal(x, x0), (kind, k)
+        else:
+            assert np.linalg.norm(x - x0) <= 1e-6 * np.linalg.norm(x0), (kind, k)
+    assert srv.wait(extra[0], ids[2])[1].status == 0
+    st2 = srv.wait(extra[2])[1].status
+    assert st2 in (0, 6)                             # cancelled unless it already ran
+    s = srv.stats()
+    assert s["solo_runs"] >= 5 and s["solo_ms"] > 0
+    srv.close()                                      # extra[1] never collected
+
+
 def test_poisson_requests_solve_their_system(ctx, polys):
     """Poisson request result satisfies the assembled system (rebuilt here on the
     caller's numbering with the same seeded velocity) to the tolerance."""
