@@ -142,6 +142,59 @@ __global__ void k_permute(int64_t n, int nc, int mode, int oriented,
     }
 }
 
+// k_permute for NC = 1 or 3 components: each thread takes kPermU elements
+// (i = base + u * blockDim), their map entries and values loaded together
+// before any store, so a thread has that many random accesses in flight.
+constexpr int kPermU = 4;
+template <typename S, int NC, int MODE>
+__global__ __launch_bounds__(256) void k_permute_nc(int64_t n, int oriented,
+                                                    const int32_t* __restrict__ map,
+                                                    const S* __restrict__ src, S* __restrict__ dst) {
+    const int64_t base = (int64_t)blockIdx.x * blockDim.x * kPermU + threadIdx.x;
+    int32_t e[kPermU];
+#pragma unroll
+    for (int u = 0; u < kPermU; ++u) {
+        const int64_t i = base + (int64_t)u * blockDim.x;
+        e[u] = i < n ? map[i] : 0;
+    }
+    S v[kPermU][NC];
+#pragma unroll
+    for (int u = 0; u < kPermU; ++u) {
+        const int64_t i = base + (int64_t)u * blockDim.x;
+        const int64_t j = e[u] >= 0 ? e[u] : ~e[u];
+        const int64_t from = MODE == 0 ? j : i;
+#pragma unroll
+        for (int k = 0; k < NC; ++k) v[u][k] = i < n ? src[from * NC + k] : S(0);
+    }
+#pragma unroll
+    for (int u = 0; u < kPermU; ++u) {
+        const int64_t i = base + (int64_t)u * blockDim.x;
+        if (i >= n) break;
+        const int64_t j = e[u] >= 0 ? e[u] : ~e[u];
+        const int64_t to = MODE == 0 ? i : j;
+        const bool neg = oriented && e[u] < 0;
+#pragma unroll
+        for (int k = 0; k < NC; ++k) dst[to * NC + k] = neg ? -v[u][k] : v[u][k];
+    }
+}
+
+template <typename S>
+void launch_permute(hipStream_t s, int64_t n, int nc, int mode, int oriented, const int32_t* map,
+                    const S* src, S* dst) {
+    const int64_t per = 256 * (int64_t)kPermU;
+    const unsigned g = (unsigned)((n + per - 1) / per);
+    if (nc == 1 && mode == 0)
+        k_permute_nc<S, 1, 0><<<g, 256, 0, s>>>(n, oriented, map, src, dst);
+    else if (nc == 1)
+        k_permute_nc<S, 1, 1><<<g, 256, 0, s>>>(n, oriented, map, src, dst);
+    else if (nc == 3 && mode == 0)
+        k_permute_nc<S, 3, 0><<<g, 256, 0, s>>>(n, oriented, map, src, dst);
+    else if (nc == 3)
+        k_permute_nc<S, 3, 1><<<g, 256, 0, s>>>(n, oriented, map, src, dst);
+    else
+        k_permute<S><<<grid_for(n, 256, 8192), 256, 0, s>>>(n, nc, mode, oriented, map, src, dst);
+}
+
 // one component of an interleaved field: mode 0 dst[i] = src[i*nc + k] (out),
 // mode 1 dst[i*nc + k] = src[i] (in)
 template <typename S>
@@ -376,13 +429,13 @@ int fv_permute(fv_ctx* ctx, fv_prec prec, int64_t n, int ncomp, int mode, int or
         FV_REQUIRE(ctx && n >= 0 && ncomp >= 1 && (mode == 0 || mode == 1), "bad argument");
         if (!n) return;
         FV_REQUIRE(map && src && dst && src != dst, "NULL or aliased pointer");
-        int g = grid_for(n, 256, 8192);
+        FV_REQUIRE(n <= (int64_t)0xFFFFFFFF * 256 * kPermU, "too many elements");
         if (prec == FV_F64)
-            k_permute<double><<<g, 256, 0, ctx->stream>>>(n, ncomp, mode, oriented, map,
-                                                          (const double*)src, (double*)dst);
+            launch_permute<double>(ctx->stream, n, ncomp, mode, oriented, map,
+                                   (const double*)src, (double*)dst);
         else
-            k_permute<float><<<g, 256, 0, ctx->stream>>>(n, ncomp, mode, oriented, map,
-                                                         (const float*)src, (float*)dst);
+            launch_permute<float>(ctx->stream, n, ncomp, mode, oriented, map, (const float*)src,
+                                  (float*)dst);
         FV_LAUNCH_CHECK();
     });
 }
