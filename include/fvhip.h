@@ -538,8 +538,8 @@ typedef struct {
     size_t resident_bytes; /* device bytes of the resident meshes */
     /* where the worker's time went (ms): admitting requests (allocation, inputs,
      * assembly), GPU chunks (batched: one chunk of every member, with its
-     * synchronisation, and the whole solves of requests that run alone; plain:
-     * whole solves), and advancing requests between
+     * synchronisation; plain: whole solves; the batched server's solo lane
+     * reports its solves in solo_ms), and advancing requests between
      * chunks (next Euler step, results, release); chunks run, launch tables
      * built and chunk graphs captured (batched) */
     double admit_ms, gpu_ms, advance_ms;
