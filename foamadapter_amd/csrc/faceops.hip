@@ -11,6 +11,8 @@
 // float atomics are used and the result does not depend on scheduling.
 // Face-result operators (interpolate, snGrad, flux) are one coalesced pass
 // over the faces.
+#include <type_traits>
+
 #include "mesh.hpp"
 
 using namespace fvhip;
@@ -202,7 +204,8 @@ struct Split {
 
 // An operator gives each face a contribution of NC components.  For the
 // staged gather it splits a face into its loads (load/eload/bload: global reads
-// only, into an In/BIn record) and the arithmetic on them (make/bmake), so the
+// only, into an In/BIn record; load = fload, the face streams, then tload, the
+// cell values by index) and the arithmetic on them (make/bmake), so the
 // kernel can issue the loads of every face it holds -- own, external lower and
 // boundary -- before it waits for any.  The arithmetic is the serial kernels'
 // own expression, so the bits are the same.  face/bface (make of load) serve
@@ -231,6 +234,14 @@ struct GradOp : FaceOp<S, GradOp<S>, 3> {
     __device__ In load(int f, int o, int n) const {
         const int64_t f3 = 3 * (int64_t)f;
         return {w[f], T[o], T[n], Sf[f3], Sf[f3 + 1], Sf[f3 + 2]};
+    }
+    __device__ In fload(int f) const {
+        const int64_t f3 = 3 * (int64_t)f;
+        return {w[f], S(0), S(0), Sf[f3], Sf[f3 + 1], Sf[f3 + 2]};
+    }
+    __device__ void tload(In& a, int o, int n) const {
+        a.TP = T[o];
+        a.TN = T[n];
     }
     __device__ In eload(int k, int, int o, int n, const ExGeo<S>& x) const {
         const int64_t k3 = 3 * (int64_t)k;
@@ -268,6 +279,11 @@ struct DivOp : FaceOp<S, DivOp<S, SCHEME>, 1> {
     __device__ In load(int f, int o, int n) const {
         return {phi[f], SCHEME == 0 ? w[f] : S(0), T[o], T[n]};
     }
+    __device__ In fload(int f) const { return {phi[f], SCHEME == 0 ? w[f] : S(0), S(0), S(0)}; }
+    __device__ void tload(In& a, int o, int n) const {
+        a.TP = T[o];
+        a.TN = T[n];
+    }
     __device__ In eload(int k, int f, int o, int n, const ExGeo<S>& x) const {
         return {phi[f], SCHEME == 0 ? x.w[k] : S(0), T[o], T[n]};
     }
@@ -288,6 +304,8 @@ struct IntegrateOp : FaceOp<S, IntegrateOp<S>, 1> {  // surfaceIntegrate(phi): d
     };
     using BIn = In;
     __device__ In load(int f, int, int) const { return {phi[f]}; }
+    __device__ In fload(int f) const { return {phi[f]}; }
+    __device__ void tload(In&, int, int) const {}
     __device__ In eload(int, int f, int, int, const ExGeo<S>&) const { return {phi[f]}; }
     __device__ void make(const In& a, S* v) const { v[0] = a.ph; }
     __device__ BIn bload(int b, int) const { return {phi[F + b]}; }
@@ -312,6 +330,11 @@ struct LapOp : FaceOp<S, LapOp<S>, 1> {
     };
     __device__ In load(int f, int o, int n) const {
         return {gamma[f], magSf[f], nodc[f], T[n], T[o]};
+    }
+    __device__ In fload(int f) const { return {gamma[f], magSf[f], nodc[f], S(0), S(0)}; }
+    __device__ void tload(In& a, int o, int n) const {
+        a.TN = T[n];
+        a.TP = T[o];
     }
     __device__ In eload(int k, int f, int o, int n, const ExGeo<S>& x) const {
         return {gamma[f], x.magSf[k], x.nodc[k], T[n], T[o]};
@@ -396,7 +419,7 @@ static_assert(kMaxLow == 8, "sort8 sorts the per-cell lower-face lists");
 // face contribution the workgroup needs into LDS -- its own faces (streamed),
 // its external lower faces and its boundary faces (from the per-workgroup
 // lists) -- and builds each cell's face lists in LDS: the own-face range from
-// the owner sequence, the lower faces from inside the workgroup by LDS atomics
+// the cells' face offsets, the lower faces from inside the workgroup by LDS atomics
 // (then sorted, so the order is fixed), the external and boundary ranges from
 // the lists (sorted by cell).  Phase 2 accumulates each cell in ascending face
 // order (external lower, inner lower, own, boundary) and divides by V.  A
@@ -437,15 +460,18 @@ __global__ __launch_bounds__(THREADS) void k_gather(int N, int nBlocks, int run,
     }
     using In = typename Op::In;
     using BIn = typename Op::BIn;
-    const int nUp = min(nF, kMaxUp);
-    // On a fitting workgroup the own faces' owners are not read from memory: in
-    // upper-triangular order the faces of cell c are [ownStart[c],
-    // ownStart[c+1]), so each cell writes its own-face range, and its local
-    // index into the LDS slot fluxU[0][i] of each of its faces, which phase 1
-    // overwrites with the face's contribution in the one thread that reads it.
-    // The cells' offsets are loaded first, then the first pass's neighbour
-    // indices and the list records, so their latencies overlap before the
-    // barrier.
+    // A workgroup whose faces do not fit stages nothing: the per-cell path
+    // recomputes every face it needs.
+    const int nUp = fits ? nF : 0;
+    // The own faces' owners are not read from memory: in upper-triangular
+    // order the faces of cell c are [ownStart[c], ownStart[c+1]), so each cell
+    // writes its own-face range, and its local index into the LDS slot
+    // fluxU[0][i] of each of its faces, which phase 1 overwrites with the
+    // face's contribution in the one thread that reads it.  Everything that
+    // needs no index -- the cells' offsets, the first pass's neighbour indices
+    // and face streams, the list records -- is loaded before the barrier, in
+    // one round trip; after it only the cell values (by owner and neighbour
+    // index) and the list faces' inputs are.
     int oa[kPer], ob[kPer];
 #pragma unroll
     for (int q = 0; q < kPer; ++q) {
@@ -456,37 +482,29 @@ __global__ __launch_bounds__(THREADS) void k_gather(int N, int nBlocks, int run,
             ob[q] = t.ownStart[c + 1];
         }
     }
-    int nn0[kUL] = {}, oo0[kUL] = {};
+    int nn0[kUL] = {};
+    In fin0[kUL] = {};  // the first pass's face streams (the cell values come after the barrier)
     if (nUp > 0) {
 #pragma unroll
-        for (int u = 0; u < kUL; ++u) {
-            const int f = fs + min((int)threadIdx.x + u * THREADS, nUp - 1);
-            nn0[u] = t.nbr[f];
-            if (!fits) oo0[u] = t.own[f];
-        }
+        for (int u = 0; u < kUL; ++u) nn0[u] = t.nbr[fs + min((int)threadIdx.x + u * THREADS, nUp - 1)];
+#pragma unroll
+        for (int u = 0; u < kUL; ++u) fin0[u] = op.fload(fs + min((int)threadIdx.x + u * THREADS, nUp - 1));
     }
-    // One pass over kUL own faces per thread (index i0 + u*THREADS).  The first
-    // pass also takes this thread's list records (at most one external lower
-    // face and one boundary face: the lists of a fitting workgroup are no longer
-    // than the workgroup).  Loads go out in two rounds -- indices, then the
-    // operator's inputs for all faces held -- before any arithmetic waits.
+    // Each thread takes kUL own faces per pass (index i0 + u*THREADS).  The
+    // first pass also takes this thread's list records (at most one external
+    // lower face and one boundary face: the lists of a fitting workgroup are no
+    // longer than the workgroup).
     static_assert(kMaxExt <= THREADS && kMaxBnd <= THREADS, "one list record per thread");
     const int ke = e0 + (int)threadIdx.x, kb = b0 + (int)threadIdx.x;
     const bool hasE = fits && ke < e1, hasB = fits && kb < b1;
-    int ef = 0, eo = 0, ecl = 0, ecp = 0, ecn = 0, bf = 0, bcl = 0, bcp = 0, bcn = 0;
-    if (hasE) {
-        ef = L.extFace[ke];
-        eo = L.extOwn[ke];
-        ecl = L.extCell[ke];
-        ecp = L.extCell[max(ke - 1, e0)];
-        ecn = L.extCell[min(ke + 1, e1 - 1)];
-    }
-    if (hasB) {
-        bf = L.bndFace[kb];
-        bcl = L.bndCell[kb];
-        bcp = L.bndCell[max(kb - 1, b0)];
-        bcn = L.bndCell[min(kb + 1, b1 - 1)];
-    }
+    // Every lane reads records (past its workgroup's range: the first record
+    // after it, at most the zero record past the last workgroup), so no lane
+    // branches around these loads; lanes without a record ignore them.
+    const int kec = min(ke, e1), kbc = min(kb, b1);
+    const int ef = L.extFace[kec], eo = L.extOwn[kec], ecl = L.extCell[kec];
+    const int ecp = L.extCell[max(kec - 1, e0)], ecn = L.extCell[max(min(kec + 1, e1 - 1), e0)];
+    const int bf = L.bndFace[kbc], bcl = L.bndCell[kbc];
+    const int bcp = L.bndCell[max(kbc - 1, b0)], bcn = L.bndCell[max(min(kbc + 1, b1 - 1), b0)];
     for (int l = threadIdx.x; l < CELLS; l += THREADS) {
         exS[l] = exE[l] = bS[l] = bE[l] = 0;
         lowN[l] = 0;
@@ -501,53 +519,43 @@ __global__ __launch_bounds__(THREADS) void k_gather(int N, int nBlocks, int run,
     }
     if (threadIdx.x == 0) overflow = fits ? 0 : 1;
     __syncthreads();
-    for (int i0 = threadIdx.x, pass = 0; i0 < nUp || pass == 0; i0 += THREADS * kUL, ++pass) {
-        // (a workgroup may own no face: then pass 0 only takes the list records;
-        // lanes past the range load the last face and store nothing -- their
-        // owner is c0, not read from a slot another lane may have overwritten)
-        int oo[kUL] = {}, nn[kUL] = {};
-        if (nUp > 0) {
-#pragma unroll
-            for (int u = 0; u < kUL; ++u) {
-                const int iu = i0 + u * THREADS;
-                const int f = fs + min(iu, nUp - 1);
-                if (pass == 0) {
-                    nn[u] = nn0[u];
-                    oo[u] = oo0[u];
-                } else {
-                    nn[u] = t.nbr[f];
-                    if (!fits) oo[u] = t.own[f];
-                }
-                if (fits) oo[u] = iu < nUp ? c0 + (int)fluxU[0][iu] : c0;
-            }
-        }
+    // This thread's list faces: loads, contribution into LDS, the cell ranges.
+    auto list_faces = [&] {
         In ein{};
         BIn bin{};
-        if (pass == 0) {
-            if (hasE) ein = op.eload(ke, ef, eo, c0 + ecl, xg);
-            if (hasB) bin = op.bload(bf, c0 + bcl);
+        if (hasE) ein = op.eload(ke, ef, eo, c0 + ecl, xg);
+        if (hasB) bin = op.bload(bf, c0 + bcl);
+        if (hasE) {
+            S sv[NC];
+            op.make(ein, sv);
+            for (int j = 0; j < NC; ++j) fluxE[j][ke - e0] = sv[j];
+            if (ke == e0 || ecp != ecl) exS[ecl] = (uint16_t)(ke - e0);
+            if (ke == e1 - 1 || ecn != ecl) exE[ecl] = (uint16_t)(ke - e0 + 1);
         }
-        In fin[kUL] = {};
-        if (nUp > 0) {
+        if (hasB) {
+            S sv[NC];
+            op.bmake(bin, sv);
+            for (int j = 0; j < NC; ++j) fluxB[j][kb - b0] = sv[j];
+            if (kb == b0 || bcp != bcl) bS[bcl] = (uint16_t)(kb - b0);
+            if (kb == b1 - 1 || bcn != bcl) bE[bcl] = (uint16_t)(kb - b0 + 1);
+        }
+    };
+    // One pass over this thread's kUL own faces from i0 (nUp > 0).  The loads
+    // of every face the pass holds, then in the first pass the list faces',
+    // go out before any arithmetic waits, and no lane branches around a load
+    // (the compiler waits for every load in flight where such a branch joins):
+    // lanes past the range read the map slot and load the inputs of the last
+    // face, take owner c0 and store nothing.
+    auto own_pass = [&](int i0, const int (&nn)[kUL], In (&fin)[kUL], auto first) {
+        constexpr bool kFirst = decltype(first)::value;
 #pragma unroll
-            for (int u = 0; u < kUL; ++u) fin[u] = op.load(fs + min(i0 + u * THREADS, nUp - 1), oo[u], nn[u]);
+        for (int u = 0; u < kUL; ++u) {
+            const int iu = min(i0 + u * THREADS, nUp - 1);
+            const S sl = fluxU[0][iu];  // the owner map
+            const int ol = i0 + u * THREADS < nUp ? (int)sl : 0;
+            op.tload(fin[u], c0 + ol, nn[u]);
         }
-        if (pass == 0) {
-            if (hasE) {
-                S sv[NC];
-                op.make(ein, sv);
-                for (int j = 0; j < NC; ++j) fluxE[j][ke - e0] = sv[j];
-                if (ke == e0 || ecp != ecl) exS[ecl] = (uint16_t)(ke - e0);
-                if (ke == e1 - 1 || ecn != ecl) exE[ecl] = (uint16_t)(ke - e0 + 1);
-            }
-            if (hasB) {
-                S sv[NC];
-                op.bmake(bin, sv);
-                for (int j = 0; j < NC; ++j) fluxB[j][kb - b0] = sv[j];
-                if (kb == b0 || bcp != bcl) bS[bcl] = (uint16_t)(kb - b0);
-                if (kb == b1 - 1 || bcn != bcl) bE[bcl] = (uint16_t)(kb - b0 + 1);
-            }
-        }
+        if constexpr (kFirst) list_faces();
 #pragma unroll
         for (int u = 0; u < kUL; ++u) {
             const int i = i0 + u * THREADS;
@@ -556,17 +564,31 @@ __global__ __launch_bounds__(THREADS) void k_gather(int N, int nBlocks, int run,
             S sv[NC];
             op.make(fin[u], sv);
             for (int j = 0; j < NC; ++j) fluxU[j][i] = sv[j];
-            if (fits) {
-                if (n < c1) {
-                    const int nl = n - c0;
-                    const int slot = atomicAdd(&lowN[nl], 1);
-                    if (slot < kMaxLow)
-                        lowL[slot][nl] = (uint16_t)i;
-                    else
-                        overflow = 1;
-                }
+            if (n < c1) {
+                const int nl = n - c0;
+                const int slot = atomicAdd(&lowN[nl], 1);
+                if (slot < kMaxLow)
+                    lowL[slot][nl] = (uint16_t)i;
+                else
+                    overflow = 1;
             }
         }
+    };
+    if (nUp > 0) {
+        own_pass((int)threadIdx.x, nn0, fin0, std::true_type{});
+        for (int i0 = (int)threadIdx.x + THREADS * kUL; i0 < nUp; i0 += THREADS * kUL) {
+            int nn[kUL];
+            In fin[kUL];
+#pragma unroll
+            for (int u = 0; u < kUL; ++u) {
+                const int f = fs + min(i0 + u * THREADS, nUp - 1);
+                nn[u] = t.nbr[f];
+                fin[u] = op.fload(f);
+            }
+            own_pass(i0, nn, fin, std::false_type{});
+        }
+    } else {
+        list_faces();  // a workgroup that owns no face (none when it does not fit)
     }
     __syncthreads();
     if (overflow) {

@@ -285,7 +285,7 @@ const GatherLists& gather_lists(fv_mesh* m, int cells, bool f32) {
         if (f32 && !L.ready32) {
             ensure_f32(m);
             const MeshF32& g = m->f32;
-            pack_ext_geo<float>(m->ctx, L.extFace, (int64_t)L.extFace.n, g.w.p, g.magSf.p,
+            pack_ext_geo<float>(m->ctx, L.extFace, L.nExt, g.w.p, g.magSf.p,
                                 g.nodc.p, g.Sf.p, L.exW32, L.exMagSf32, L.exNodc32, L.exSf32);
             L.ready32 = true;
         }
@@ -332,6 +332,13 @@ const GatherLists& gather_lists(fv_mesh* m, int cells, bool f32) {
     }
     L.nBlocks = nBlocks;
     L.nOverflow = nOverflow;
+    L.nExt = (int64_t)ef.size();
+    // the zero records past the last block
+    ef.push_back(0);
+    ew.push_back(0);
+    ec.push_back(0);
+    bf.push_back(0);
+    bc.push_back(0);
     fv_ctx* ctx = m->ctx;
     L.extOff.upload(ctx, eo.data(), eo.size());
     L.extFace.upload(ctx, ef.data(), ef.size());
@@ -341,7 +348,7 @@ const GatherLists& gather_lists(fv_mesh* m, int cells, bool f32) {
     L.bndFace.upload(ctx, bf.data(), bf.size());
     L.bndCell.upload(ctx, bc.data(), bc.size());
     FV_HIP(hipStreamSynchronize(ctx->stream));
-    pack_ext_geo<double>(ctx, L.extFace, (int64_t)ef.size(), m->w.p, m->magSf.p, m->nodc.p,
+    pack_ext_geo<double>(ctx, L.extFace, L.nExt, m->w.p, m->magSf.p, m->nodc.p,
                          m->Sf.p, L.exW, L.exMagSf, L.exNodc, L.exSf);
     L.ready32 = false;
     L.cells = cells;

@@ -29,8 +29,12 @@ struct MeshF32 {
 //     faces whose owner lies in an earlier block, sorted by (cell, face)
 //   bndOff[nBlocks+1]; bndFace (boundary face b), bndCell: the block's boundary
 //     faces sorted by (cell, b)
+// Each record array holds one zero record past its last block, so a kernel can
+// read a record at any index up to extOff[nBlocks] / bndOff[nBlocks] without a
+// branch.
 struct GatherLists {
     int cells = 0;  // 0: not built
+    int64_t nExt = 0;  // external lower-face records (without the zero record)
     fvhip::DBuf<int32_t> extOff, extFace, extOwn, bndOff, bndFace;
     fvhip::DBuf<uint16_t> extCell, bndCell;
     // geometry of the external lower faces packed in record order (w, magSf,
