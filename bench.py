@@ -336,6 +336,30 @@ def time_sweep(ctx, case, steps, warmup):
             "per_op": per_op, "dominant": (dom, dom_ms, dom_bytes), "ops": ops}
 
 
+def bench_box(ctx, steps, warmup, dims=(400, 300, 140), seed=1007):
+    """The sweep on a box whose sides are not powers of two (400 x 300 x 140 =
+    16.8M cells, the 256^3 count; jittered, permuted, renumbered): its Z-order
+    runs are ragged, which the staged gathers' lists must absorb.  Reports the
+    sweep and per-op GB/s and, per workgroup size, the workgroups that take the
+    per-cell path."""
+    import foamadapter_amd as fa
+    n = max(dims)
+    poly = fa.PolyMesh.hex(dims, lengths=tuple(d / n for d in dims), jitter=0.2,
+                           jitter_seed=seed, perm_seed=seed)
+    case = Case(ctx, poly, seed, renumber=True)
+    del poly
+    sw = time_sweep(ctx, case, steps, warmup)
+    m = case.mesh
+    out = {"mesh": f"{dims[0]}x{dims[1]}x{dims[2]} jittered, permuted, renumbered",
+           "cells": m.n_cells, "GB/s": round(sw["GB/s"], 1), "ms_per_step": round(sw["ms_per_step"], 3),
+           "per_op_GB/s": {k: v["GB/s"] for k, v in sw["per_op"].items()},
+           "gather_workgroups_per_cell_path": {str(c): list(m.gather_stats(c)) for c in (256, 512, 1024)}}
+    for k in list(case.fields):
+        del case.fields[k]
+    m.close()
+    return out
+
+
 def cpu_baseline(case, sweep, budget_s=24.0):
     """The CPU restatement on the same mesh and fields, FP64 sweep of the 6
     operators: the multi-core form (OpenMP cell walks in ascending face order,
@@ -975,6 +999,8 @@ def main():
     ap.add_argument("--rate-low", type=float, default=40.0,
                     help="a second, lower arrival rate of the served stream (0: skip)")
     ap.add_argument("--no-icofoam", action="store_true")
+    ap.add_argument("--no-box", action="store_true",
+                    help="skip the sweep on the 400 x 300 x 140 box")
     ap.add_argument("--cg-eager", action="store_true",
                     help="launch the implicit legs' repeated CG solves kernel by kernel instead "
                          "of replaying them as hipGraphs (for kernel traces)")
@@ -1055,6 +1081,9 @@ def main():
             del g.fields[k]
         g.mesh.close()
     del poly
+    if not args.no_box:
+        log("sweep on a 400 x 300 x 140 box")
+        rec["ragged_box"] = bench_box(ctx, max(3, args.steps // 4), 1)
     if not args.no_cpu:
         log("timing the CPU baseline (oracle, 1 thread)")
         rec["cpu_baseline"], rec["parity_full_size"] = cpu_baseline(case, sweep)

@@ -663,7 +663,18 @@ void launch_gather_t(fv_ctx* ctx, const fv_mesh* m, const Op& op, const S* V, S 
 template <typename S, typename Op>
 void launch_gather(fv_ctx* ctx, const fv_mesh* m, const Op& op, const S* V, S coeff, S* out) {
     // default workgroup size per operator (measured at 256^3)
-    const int cells = m->gatherMode == 0 ? Op::kDefaultCells : gather_cells(m);
+    int cells = m->gatherMode == 0 ? Op::kDefaultCells : gather_cells(m);
+    if (m->gatherMode == 0) {
+        // A mesh whose Z-order runs are ragged (sides that are not powers of
+        // two, irregular cells) can overflow the larger workgroups' lists: more
+        // than 1 % of the workgroups taking the per-cell path moves the
+        // operator to the next smaller size (same bits; the lists are cached).
+        while (cells > 256) {
+            const GatherLists& gl = gather_lists(const_cast<fv_mesh*>(m), cells, sizeof(S) == 4);
+            if (gl.nOverflow * 100 <= gl.nBlocks) break;
+            cells /= 2;
+        }
+    }
     switch (cells) {
         case 1024: launch_gather_t<S, Op, 1024, 512>(ctx, m, op, V, coeff, out); break;
         case 512: launch_gather_t<S, Op, 512, 512>(ctx, m, op, V, coeff, out); break;

@@ -53,11 +53,13 @@ struct GatherLists {
 // LDS capacities of the staged gather for a CELLS-cell workgroup (faces owned,
 // external lower faces, boundary faces, and inner lower faces per cell).  A
 // Z-order brick of 8x8x4 / 8x8x8 / 16x8x8 cells has 128 / 192 / 320 faces on
-// its three lower sides, and as many boundary faces at a corner of the domain:
-// both lists hold CELLS/2 records, so corner and edge bricks stay on the fast
-// path.
+// its three lower sides, and as many boundary faces at a corner of the domain.
+// The external list holds one record per thread of the workgroup (256 / 512 /
+// 512), room for the ragged bricks of a domain whose sides are not powers of
+// two (a 17 x 23 x 55 box: a median of 140 such faces per 256-cell brick); the
+// boundary list holds CELLS/2, so corner and edge bricks stay on the fast path.
 constexpr int gather_max_up(int cells) { return cells * 3 + cells / 4; }
-constexpr int gather_max_ext(int cells) { return cells / 2; }
+constexpr int gather_max_ext(int cells) { return cells < 512 ? cells : 512; }
 constexpr int gather_max_bnd(int cells) { return cells / 2; }
 constexpr int kGatherMaxLow = 8;
 

@@ -84,7 +84,10 @@ inline uint64_t spread3(uint64_t v) {
 }
 
 // Z-order (Morton) of approximate cell centres (average of the cell's face-point
-// averages), quantised to 2^21 levels per axis over the points' bounding box.
+// averages), quantised to 2^21 levels over the longest side of the points'
+// bounding box, the same step on every axis: a run of consecutive cells then
+// fills a near-cubic region of space whatever the domain's aspect ratio (a
+// step per axis would stretch the runs along the long axes of a long box).
 void morton_order(const fv_polymesh_desc* d, std::vector<int32_t>& order) {
     const int64_t N = d->n_cells, nF = d->n_faces, F = d->n_internal_faces, nP = d->n_points;
     double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
@@ -112,10 +115,12 @@ void morton_order(const fv_polymesh_desc* d, std::vector<int32_t>& order) {
     }
     std::vector<std::pair<uint64_t, int32_t>> key(N);
     const double scale = 2097152.0;  // 2^21
+    double ext = 0.0;
+    for (int k = 0; k < 3; ++k) ext = std::max(ext, hi[k] - lo[k]);
+    if (!(ext > 0.0)) ext = 1.0;
     for (int64_t c = 0; c < N; ++c) {
         uint64_t code = 0;
         for (int k = 0; k < 3; ++k) {
-            const double ext = hi[k] > lo[k] ? hi[k] - lo[k] : 1.0;
             double u = (cs[3 * c + k] / cnt[c] - lo[k]) / ext * scale;
             uint64_t q = u <= 0.0 ? 0 : (u >= scale - 1.0 ? (uint64_t)scale - 1 : (uint64_t)u);
             code |= spread3(q) << (2 - k);

@@ -309,6 +309,37 @@ def test_staged_and_walk_kernels_agree(ctx, cases, dt):
         m.set_gather(0)
 
 
+@pytest.mark.parametrize("dt", [np.float64, np.float32])
+def test_staged_gather_box_with_one_cell_tail(ctx, dt):
+    """A 17 x 23 x 55 box (21505 cells), jittered, permuted, Z-order renumbered.
+    Its sides are not powers of two, so the bricks are ragged: nine in ten still
+    fit the staged gather's lists at every workgroup size (the Z-order uses one
+    quantisation step on all axes, and the external list holds a record per
+    thread).  With 256, 512 and 1024 cells per workgroup the last workgroup holds
+    one cell, the highest-numbered, which owns no face, so that workgroup takes
+    only its list faces.  Every size is bit-identical to the oracle on the
+    renumbered mesh, the few workgroups that do not fit included."""
+    import foamadapter_amd as fa
+    dims = (17, 23, 55)
+    poly = fa.PolyMesh.hex(dims, lengths=tuple(0.1 * d for d in dims), jitter=0.2,
+                           jitter_seed=71, perm_seed=72)
+    m = fa.Mesh.from_poly(ctx, poly, renumber=True)
+    g = orc.geometry(m.poly)
+    N = len(g["V"])
+    assert N == 21505 and all(N % c == 1 for c in (256, 512, 1024))
+    assert not np.any(g["owner"] == N - 1)   # the tail cell owns no internal face
+    for mode, cells in ((2, 256), (3, 512), (4, 1024)):
+        nb, over = m.gather_stats(cells)
+        print(f"[case] {cells}-cell workgroups: {over} of {nb} take the per-cell path", flush=True)
+        assert nb == (N + cells - 1) // cells and over <= nb // 10, (cells, nb, over)
+        m.set_gather(mode)
+        res, ref = _run_all(ctx, m, g, dt, seed=73)
+        for op in ("grad", "div_linear", "div_upwind", "surface_integrate", "laplacian"):
+            assert np.array_equal(res[op], ref[op]), (cells, op)
+    m.set_gather(0)
+    m.close()
+
+
 @pytest.mark.parametrize("method", ["morton", "rcm"])
 def test_renumbered_methods_bitwise(ctx, method):
     """Both renumberings: operators bit-identical to the oracle on the renumbered
