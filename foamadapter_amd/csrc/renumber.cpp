@@ -84,10 +84,14 @@ inline uint64_t spread3(uint64_t v) {
 }
 
 // Z-order (Morton) of approximate cell centres (average of the cell's face-point
-// averages), quantised to 2^21 levels over the longest side of the points'
-// bounding box, the same step on every axis: a run of consecutive cells then
-// fills a near-cubic region of space whatever the domain's aspect ratio (a
-// step per axis would stretch the runs along the long axes of a long box).
+// averages).  Each axis is quantised in units of the mesh's typical cell
+// spacing along it -- the median distance between face-neighbour centres whose
+// offset points mainly along that axis -- from the points' bounding-box corner,
+// with the same number of sub-cell levels on every axis.  On a structured mesh
+// (a box of any size, its vertices jittered or not) the top bits of a code are
+// then a cell's integer coordinates, so a run of 2^k consecutive cells is a
+// brick of cells whatever the box's sides; on an unstructured mesh the spacing
+// is an average and the runs stay compact.
 void morton_order(const fv_polymesh_desc* d, std::vector<int32_t>& order) {
     const int64_t N = d->n_cells, nF = d->n_faces, F = d->n_internal_faces, nP = d->n_points;
     double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
@@ -113,16 +117,53 @@ void morton_order(const fv_polymesh_desc* d, std::vector<int32_t>& order) {
             cnt[nb]++;
         }
     }
+    for (int64_t c = 0; c < N; ++c)
+        for (int k = 0; k < 3; ++k) cs[3 * c + k] /= cnt[c];
+    // typical spacing per axis from (at most about 2^20 sampled) internal faces
+    double h[3];
+    {
+        std::vector<double> dk[3];
+        const int64_t stride = std::max<int64_t>(1, F >> 20);
+        for (int64_t f = 0; f < F; f += stride) {
+            const int32_t o = d->h_owner[f], nb = d->h_neighbour[f];
+            double v[3];
+            int ax = 0;
+            for (int k = 0; k < 3; ++k) {
+                v[k] = std::abs(cs[3 * nb + k] - cs[3 * o + k]);
+                if (v[k] > v[ax]) ax = k;
+            }
+            if (v[ax] > 0.0) dk[ax].push_back(v[ax]);
+        }
+        double hmax = 0.0;
+        for (int k = 0; k < 3; ++k) {
+            h[k] = 0.0;
+            if (!dk[k].empty()) {
+                auto mid = dk[k].begin() + dk[k].size() / 2;
+                std::nth_element(dk[k].begin(), mid, dk[k].end());
+                h[k] = *mid;
+            }
+            hmax = std::max(hmax, h[k]);
+        }
+        // an axis with no neighbours along it (one cell thick) or no faces at
+        // all: quantise it like the widest spacing, or the box
+        double ext = 0.0;
+        for (int k = 0; k < 3; ++k) ext = std::max(ext, hi[k] - lo[k]);
+        if (!(ext > 0.0)) ext = 1.0;
+        for (int k = 0; k < 3; ++k)
+            if (!(h[k] > 0.0)) h[k] = hmax > 0.0 ? hmax : ext;
+    }
+    // sub-cell levels: the most cells along an axis times 2^b stays below 2^21
+    double cells = 1.0;
+    for (int k = 0; k < 3; ++k) cells = std::max(cells, (hi[k] - lo[k]) / h[k] + 1.0);
+    int b = 0;
+    while (b < 20 && cells * double(1 << (b + 1)) < 2097152.0) ++b;
+    const double top = 2097151.0;
     std::vector<std::pair<uint64_t, int32_t>> key(N);
-    const double scale = 2097152.0;  // 2^21
-    double ext = 0.0;
-    for (int k = 0; k < 3; ++k) ext = std::max(ext, hi[k] - lo[k]);
-    if (!(ext > 0.0)) ext = 1.0;
     for (int64_t c = 0; c < N; ++c) {
         uint64_t code = 0;
         for (int k = 0; k < 3; ++k) {
-            double u = (cs[3 * c + k] / cnt[c] - lo[k]) / ext * scale;
-            uint64_t q = u <= 0.0 ? 0 : (u >= scale - 1.0 ? (uint64_t)scale - 1 : (uint64_t)u);
+            const double u = (cs[3 * c + k] - lo[k]) / h[k] * double(1 << b);
+            const uint64_t q = u <= 0.0 ? 0 : (u >= top ? (uint64_t)top : (uint64_t)u);
             code |= spread3(q) << (2 - k);
         }
         key[c] = {code, (int32_t)c};
