@@ -175,8 +175,10 @@ int fv_hexmesh_generate(int32_t nx, int32_t ny, int32_t nz, double lx, double ly
 /* Locality renumbering of a polyMesh (host), the counterpart of OpenFOAM's
  * renumberMesh that a case runs before the solver: method 0 reverse
  * Cuthill-McKee (a breadth-first sweep of the face graph), method 1 Z-order
- * (Morton code of the cell centres: aligned runs of cells are compact bricks,
- * which the face kernels exploit); internal
+ * (Morton code of the cell centres, each axis quantised in units of the
+ * mesh's typical cell spacing along it: aligned runs of cells are compact
+ * bricks, whole bricks of cells on a structured mesh of any size, which the
+ * face kernels exploit); internal
  * faces rewritten in upper-triangular order of the new numbering (a face whose
  * owner/neighbour swap has its point list reversed), boundary faces unchanged.
  * Outputs are sized like the input (points and patches are unchanged):
