@@ -395,25 +395,31 @@ struct Lists {
     const uint16_t *extCell, *bndCell;
 };
 
-constexpr int kMaxLow = kGatherMaxLow;  // lower faces per cell from inside the workgroup, in LDS
-
-// Sorts 8 keys ascending in registers (the 19-comparator network for 8 inputs:
-// layers (0,2)(1,3)(4,6)(5,7) / (0,4)(1,5)(2,6)(3,7) / (0,1)(2,3)(4,5)(6,7) /
-// (2,4)(3,5) / (1,4)(3,6) / (1,2)(3,4)(5,6)).
-__device__ inline void sort8(uint16_t (&k)[8]) {
+// Sorts 8 (or 4) keys ascending in registers: the 19-comparator network for 8
+// inputs, layers (0,2)(1,3)(4,6)(5,7) / (0,4)(1,5)(2,6)(3,7) /
+// (0,1)(2,3)(4,5)(6,7) / (2,4)(3,5) / (1,4)(3,6) / (1,2)(3,4)(5,6); the
+// 5-comparator one for 4, (0,1)(2,3) / (0,2)(1,3) / (1,2).
+template <int K>
+__device__ inline void sort_keys(uint16_t (&k)[K]) {
+    static_assert(K == 8 || K == 4, "per-cell lower-face lists hold 8 or 4 entries");
     auto cx = [&](int a, int b) {
         const uint16_t x = k[a], y = k[b];
         k[a] = x < y ? x : y;
         k[b] = x < y ? y : x;
     };
-    cx(0, 2); cx(1, 3); cx(4, 6); cx(5, 7);
-    cx(0, 4); cx(1, 5); cx(2, 6); cx(3, 7);
-    cx(0, 1); cx(2, 3); cx(4, 5); cx(6, 7);
-    cx(2, 4); cx(3, 5);
-    cx(1, 4); cx(3, 6);
-    cx(1, 2); cx(3, 4); cx(5, 6);
+    if constexpr (K == 8) {
+        cx(0, 2); cx(1, 3); cx(4, 6); cx(5, 7);
+        cx(0, 4); cx(1, 5); cx(2, 6); cx(3, 7);
+        cx(0, 1); cx(2, 3); cx(4, 5); cx(6, 7);
+        cx(2, 4); cx(3, 5);
+        cx(1, 4); cx(3, 6);
+        cx(1, 2); cx(3, 4); cx(5, 6);
+    } else {
+        cx(0, 1); cx(2, 3);
+        cx(0, 2); cx(1, 3);
+        cx(1, 2);
+    }
 }
-static_assert(kMaxLow == 8, "sort8 sorts the per-cell lower-face lists");
 
 // A workgroup of CELLS cells.  Phase 1 computes, with coalesced loads, every
 // face contribution the workgroup needs into LDS -- its own faces (streamed),
@@ -435,10 +441,17 @@ __global__ __launch_bounds__(THREADS) void k_gather(int N, int nBlocks, int run,
     constexpr int kMaxUp = max_up<CELLS>();
     constexpr int kMaxExt = gather_max_ext(CELLS);
     constexpr int kMaxBnd = gather_max_bnd(CELLS);
+    constexpr int kMaxLow = gather_max_low(CELLS);  // inner lower faces per cell, in LDS
     __shared__ S fluxU[NC][kMaxUp];
     __shared__ S fluxE[NC][kMaxExt];
     __shared__ S fluxB[NC][kMaxBnd];
-    __shared__ uint16_t upS[CELLS], upE[CELLS], exS[CELLS], exE[CELLS], bS[CELLS], bE[CELLS];
+    // upS[l], upE[l]: the own-face range of cell l (local indices).  A
+    // 1024-cell workgroup keeps no upE (its end is upS[l + 1]): the 2 KB let
+    // four FP32 workgroups share a CU's LDS.  (Smaller workgroups keep it:
+    // reading the neighbouring cell's start made them 2-3 % slower.)
+    constexpr bool kUpE = CELLS != 1024;
+    __shared__ uint16_t upS[CELLS + 1], upE[kUpE ? CELLS : 1], exS[CELLS], exE[CELLS], bS[CELLS],
+        bE[CELLS];
     __shared__ uint16_t lowL[kMaxLow][CELLS];  // slot-major: lanes read consecutive cells
     __shared__ int lowN[CELLS];
     __shared__ int overflow;
@@ -476,7 +489,7 @@ __global__ __launch_bounds__(THREADS) void k_gather(int N, int nBlocks, int run,
 #pragma unroll
     for (int q = 0; q < kPer; ++q) {
         const int c = c0 + (int)threadIdx.x + q * THREADS;
-        oa[q] = ob[q] = fs;
+        oa[q] = ob[q] = fe;  // cells past the last one: empty ranges at the end
         if (fits && c < c1) {
             oa[q] = t.ownStart[c];
             ob[q] = t.ownStart[c + 1];
@@ -514,10 +527,13 @@ __global__ __launch_bounds__(THREADS) void k_gather(int N, int nBlocks, int run,
         const int l = (int)threadIdx.x + q * THREADS;
         const int a = oa[q] - fs, b = ob[q] - fs;  // empty unless fits and c < c1
         upS[l] = (uint16_t)a;
-        upE[l] = (uint16_t)b;
+        if constexpr (kUpE) upE[l] = (uint16_t)b;
         for (int i = a; i < b; ++i) fluxU[0][i] = S(l);
     }
-    if (threadIdx.x == 0) overflow = fits ? 0 : 1;
+    if (threadIdx.x == 0) {
+        overflow = fits ? 0 : 1;
+        upS[CELLS] = (uint16_t)nF;
+    }
     __syncthreads();
     // This thread's list faces: loads, contribution into LDS, the cell ranges.
     auto list_faces = [&] {
@@ -617,7 +633,7 @@ __global__ __launch_bounds__(THREADS) void k_gather(int N, int nBlocks, int run,
         uint16_t key[kMaxLow];
 #pragma unroll
         for (int a = 0; a < kMaxLow; ++a) key[a] = a < n ? lowL[a][l] : uint16_t(0xFFFF);
-        sort8(key);
+        sort_keys<kMaxLow>(key);
 #pragma unroll
         for (int a = 0; a < kMaxLow; ++a)
             if (a < n) {
@@ -625,7 +641,8 @@ __global__ __launch_bounds__(THREADS) void k_gather(int N, int nBlocks, int run,
                 for (int j = 0; j < NC; ++j) v[j] = fluxU[j][i];
                 for (int j = 0; j < NC; ++j) acc[j] -= v[j];
             }
-        for (int i = upS[l]; i < upE[l]; ++i) {
+        const int ue = kUpE ? upE[l] : upS[l + 1];
+        for (int i = upS[l]; i < ue; ++i) {
             for (int j = 0; j < NC; ++j) v[j] = fluxU[j][i];
             for (int j = 0; j < NC; ++j) acc[j] += v[j];
         }

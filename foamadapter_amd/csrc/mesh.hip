@@ -327,7 +327,7 @@ const GatherLists& gather_lists(fv_mesh* m, int cells, bool f32) {
         const bool fits = ownStart[c1] - ownStart[c0] <= gather_max_up(cells) &&
                           eo[blk + 1] - eo[blk] <= gather_max_ext(cells) &&
                           bo[blk + 1] - bo[blk] <= gather_max_bnd(cells) &&
-                          innerMax <= kGatherMaxLow;
+                          innerMax <= gather_max_low(cells);
         nOverflow += fits ? 0 : 1;
     }
     L.nBlocks = nBlocks;

@@ -45,7 +45,7 @@ struct GatherLists {
     bool ready32 = false;
     // workgroups whose faces do not fit the kernel's LDS lists (more owned,
     // external lower or boundary faces than it holds, or a cell with more
-    // than kGatherMaxLow lower faces from inside the workgroup): they take the
+    // than gather_max_low lower faces from inside the workgroup): they take the
     // per-cell path, same bits, slower
     int64_t nBlocks = 0, nOverflow = 0;
 };
@@ -61,7 +61,10 @@ struct GatherLists {
 constexpr int gather_max_up(int cells) { return cells * 3 + cells / 4; }
 constexpr int gather_max_ext(int cells) { return cells < 512 ? cells : 512; }
 constexpr int gather_max_bnd(int cells) { return cells / 2; }
-constexpr int kGatherMaxLow = 8;
+// Inner lower faces per cell: 8 (a hex cell has 3; polyhedra up to 8), 4 for
+// 1024-cell workgroups, whose LDS then fits four workgroups per CU (a mesh with
+// more such cells moves to 512-cell workgroups, see launch_gather).
+constexpr int gather_max_low(int cells) { return cells == 1024 ? 4 : 8; }
 
 struct fv_mesh {
     fv_ctx* ctx = nullptr;
