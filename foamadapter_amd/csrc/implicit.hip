@@ -53,6 +53,12 @@ struct Stencil {
     const int32_t *off, *idx, *slot;
 };
 
+// The first kAsmE stencil entries of a cell have their loads issued in two
+// rounds (entries, then the face data of all of them) before any arithmetic;
+// the arithmetic and its order are those of the loop over the stencil, which
+// the entries past kAsmE (cells with more faces) take.
+constexpr int kAsmE = 8;
+
 __global__ __launch_bounds__(kBlock) void k_laplacian(
     int N, int F, Stencil st, const int32_t* __restrict__ rowOffs,
     const int32_t* __restrict__ diag, const double* __restrict__ magSf,
@@ -63,36 +69,80 @@ __global__ __launch_bounds__(kBlock) void k_laplacian(
     double* __restrict__ rhs, double* __restrict__ bMat, double* __restrict__ bRhs) {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= N) return;
-    const int row = rowOffs[c];
+    const int row = rowOffs[c], dg = diag[c];
+    const int k0 = st.off[c], k1 = st.off[c + 1];
+    int ev[kAsmE], sl[kAsmE];
+#pragma unroll
+    for (int j = 0; j < kAsmE; ++j)
+        if (k0 + j < k1) {
+            ev[j] = st.idx[k0 + j];
+            sl[j] = st.slot[k0 + j];
+        }
+    double ga[kAsmE], ma[kAsmE], da[kAsmE], tb[kAsmE], tg[kAsmE];
+    int kd[kAsmE];
+#pragma unroll
+    for (int j = 0; j < kAsmE; ++j)
+        if (k0 + j < k1) {
+            const int e = ev[j];
+            if (e >= F) {
+                const int b = e - F;
+                ga[j] = gamma[e];
+                ma[j] = magSfb[b];
+                kd[j] = kind[b];
+                da[j] = dcb[b];
+                tb[j] = Tb[b];
+                tg[j] = Tg ? Tg[b] : 0.0;
+            } else {
+                const int f = e >= 0 ? e : ~e;
+                ga[j] = gamma[f];
+                ma[j] = magSf[f];
+                da[j] = nodc[f];
+            }
+        }
     double dsum = 0.0, bsum = 0.0;
-    const int e1 = st.off[c + 1];
-    for (int k = st.off[c]; k < e1; ++k) {
-        const int e = st.idx[k];
+    auto entry = [&](int e, int slot, double g0, double m0, double d0, int k, double tb0,
+                     double tg0) {
         if (e >= F) {
             // boundary coefficients: the cell is the only writer of its faces
             const int b = e - F;
-            const double gm = gamma[e] * magSfb[b];
-            if (kind[b] == FV_BC_FIXED_GRADIENT) {
-                const double g = gm * (Tg ? Tg[b] : 0.0);
+            const double gm = g0 * m0;
+            if (k == FV_BC_FIXED_GRADIENT) {
+                const double g = gm * tg0;
                 bsum -= g;
                 bMat[b] += 0.0;
                 bRhs[b] += coeff * g;
             } else {
-                const double ic = gm * dcb[b];
-                const double it = ic * Tb[b];
+                const double ic = gm * d0;
+                const double it = ic * tb0;
                 dsum -= ic;
                 bsum -= it;
                 bMat[b] += coeff * ic;
                 bRhs[b] += coeff * it;
             }
         } else {
-            const int f = e >= 0 ? e : ~e;
-            const double up = nodc[f] * (gamma[f] * magSf[f]);
-            values[row + st.slot[k]] += coeff * up;
+            const double up = d0 * (g0 * m0);
+            values[row + slot] += coeff * up;
             dsum -= up;
         }
+    };
+#pragma unroll
+    for (int j = 0; j < kAsmE; ++j)
+        if (k0 + j < k1) {
+            const bool bnd = ev[j] >= F;
+            entry(ev[j], sl[j], ga[j], ma[j], da[j], bnd ? kd[j] : 0, bnd ? tb[j] : 0.0,
+                  bnd ? tg[j] : 0.0);
+        }
+    for (int k = k0 + kAsmE; k < k1; ++k) {
+        const int e = st.idx[k];
+        if (e >= F) {
+            const int b = e - F;
+            entry(e, 0, gamma[e], magSfb[b], dcb[b], kind[b], Tb[b], Tg ? Tg[b] : 0.0);
+        } else {
+            const int f = e >= 0 ? e : ~e;
+            entry(e, st.slot[k], gamma[f], magSf[f], nodc[f], 0, 0.0, 0.0);
+        }
     }
-    values[row + diag[c]] += coeff * dsum;
+    values[row + dg] += coeff * dsum;
     rhs[c] += coeff * bsum;
 }
 
@@ -223,38 +273,78 @@ __global__ __launch_bounds__(kBlock) void k_div(
     double* __restrict__ rhs) {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= N) return;
-    const int row = rowOffs[c];
-    double dsum = 0.0, bsum = 0.0;
-    const int e1 = st.off[c + 1];
-    for (int k = st.off[c]; k < e1; ++k) {
-        const int e = st.idx[k];
-        if (e >= F) {
-            const int b = e - F;
-            const double pf = phi[e];
-            if (kind[b] == FV_BC_FIXED_GRADIENT) {
-                dsum += pf;
-                bsum -= pf * ((Tg ? Tg[b] : 0.0) / dcb[b]);
+    const int row = rowOffs[c], dg = diag[c];
+    const int k0 = st.off[c], k1 = st.off[c + 1];
+    // loads in two rounds for the first kAsmE entries, as in k_laplacian
+    int ev[kAsmE], sl[kAsmE];
+#pragma unroll
+    for (int j = 0; j < kAsmE; ++j)
+        if (k0 + j < k1) {
+            ev[j] = st.idx[k0 + j];
+            sl[j] = st.slot[k0 + j];
+        }
+    double pa[kAsmE], wa[kAsmE], tb[kAsmE], tg[kAsmE], da[kAsmE];
+    int kd[kAsmE];
+#pragma unroll
+    for (int j = 0; j < kAsmE; ++j)
+        if (k0 + j < k1) {
+            const int e = ev[j];
+            if (e >= F) {
+                const int b = e - F;
+                pa[j] = phi[e];
+                kd[j] = kind[b];
+                tb[j] = Tb[b];
+                tg[j] = Tg ? Tg[b] : 0.0;
+                da[j] = dcb[b];
             } else {
-                bsum -= pf * Tb[b];
+                const int f = e >= 0 ? e : ~e;
+                pa[j] = phi[f];
+                wa[j] = scheme == 0 ? w[f] : 0.0;
+            }
+        }
+    double dsum = 0.0, bsum = 0.0;
+    auto entry = [&](int e, int slot, double pf, double wf, int k, double tb0, double tg0,
+                     double d0) {
+        if (e >= F) {
+            if (k == FV_BC_FIXED_GRADIENT) {
+                dsum += pf;
+                bsum -= pf * (tg0 / d0);
+            } else {
+                bsum -= pf * tb0;
             }
         } else {
-            const int f = e >= 0 ? e : ~e;
-            const double ph = phi[f];
-            const double lam = scheme == 0 ? w[f] : (ph >= 0.0 ? 1.0 : 0.0);
-            const double lower = -lam * ph;
-            const double upper = lower + ph;
+            const double lam = scheme == 0 ? wf : (pf >= 0.0 ? 1.0 : 0.0);
+            const double lower = -lam * pf;
+            const double upper = lower + pf;
             // gaussConvectionScheme::fvmDiv: owner row gets upper in column N,
             // neighbour row gets lower in column P; negSumDiag
             if (e >= 0) {
-                values[row + st.slot[k]] += coeff * upper;
+                values[row + slot] += coeff * upper;
                 dsum -= lower;
             } else {
-                values[row + st.slot[k]] += coeff * lower;
+                values[row + slot] += coeff * lower;
                 dsum -= upper;
             }
         }
+    };
+#pragma unroll
+    for (int j = 0; j < kAsmE; ++j)
+        if (k0 + j < k1) {
+            const bool bnd = ev[j] >= F;
+            entry(ev[j], sl[j], pa[j], bnd ? 0.0 : wa[j], bnd ? kd[j] : 0, bnd ? tb[j] : 0.0,
+                  bnd ? tg[j] : 0.0, bnd ? da[j] : 1.0);
+        }
+    for (int k = k0 + kAsmE; k < k1; ++k) {
+        const int e = st.idx[k];
+        if (e >= F) {
+            const int b = e - F;
+            entry(e, 0, phi[e], 0.0, kind[b], Tb[b], Tg ? Tg[b] : 0.0, dcb[b]);
+        } else {
+            const int f = e >= 0 ? e : ~e;
+            entry(e, st.slot[k], phi[f], scheme == 0 ? w[f] : 0.0, 0, 0.0, 0.0, 1.0);
+        }
     }
-    values[row + diag[c]] += coeff * dsum;
+    values[row + dg] += coeff * dsum;
     rhs[c] += coeff * bsum;
 }
 
