@@ -65,9 +65,38 @@ __global__ __launch_bounds__(kBlock) void k_hbya(int N, int F, const int32_t* __
     if (c >= N) return;
     const int row = rowOffs[c];
     double h0 = 0.0, h1 = 0.0, h2 = 0.0;
-    const int e1 = off[c + 1];
-    for (int k = off[c]; k < e1; ++k) {
-        if (idx[k] >= F) continue;  // boundary faces carry no off-diagonal
+    const int k0 = off[c], e1 = off[c + 1];
+    // the first kE stencil entries: indices, then their coefficients and the
+    // neighbours' velocities, loaded in two rounds before the sums, which run
+    // in stencil order (boundary faces carry no off-diagonal)
+    constexpr int kE = 8;
+    int ie[kE], oe[kE], se[kE];
+#pragma unroll
+    for (int j = 0; j < kE; ++j)
+        if (k0 + j < e1) {
+            ie[j] = idx[k0 + j];
+            oe[j] = oth[k0 + j];
+            se[j] = slot[k0 + j];
+        }
+    double av[kE], u0[kE], u1[kE], u2[kE];
+#pragma unroll
+    for (int j = 0; j < kE; ++j)
+        if (k0 + j < e1 && ie[j] < F) {
+            const int64_t o = 3 * (int64_t)oe[j];
+            av[j] = values[row + se[j]];
+            u0[j] = U[o];
+            u1[j] = U[o + 1];
+            u2[j] = U[o + 2];
+        }
+#pragma unroll
+    for (int j = 0; j < kE; ++j)
+        if (k0 + j < e1 && ie[j] < F) {
+            h0 -= av[j] * u0[j];
+            h1 -= av[j] * u1[j];
+            h2 -= av[j] * u2[j];
+        }
+    for (int k = k0 + kE; k < e1; ++k) {  // cells with more than kE faces
+        if (idx[k] >= F) continue;
         const double a = values[row + slot[k]];
         const int64_t o = 3 * (int64_t)oth[k];
         h0 -= a * U[o];
