@@ -505,8 +505,8 @@ __global__ __launch_bounds__(THREADS) void k_gather(int N, int nBlocks, int run,
     }
     // Each thread takes kUL own faces per pass (index i0 + u*THREADS).  The
     // first pass also takes this thread's list records (at most one external
-    // lower face and one boundary face: the lists of a fitting workgroup are no
-    // longer than the workgroup).
+    // lower face and one boundary face: the lists of a fitting workgroup hold
+    // no more records than it has threads).
     static_assert(kMaxExt <= THREADS && kMaxBnd <= THREADS, "one list record per thread");
     const int ke = e0 + (int)threadIdx.x, kb = b0 + (int)threadIdx.x;
     const bool hasE = fits && ke < e1, hasB = fits && kb < b1;
