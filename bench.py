@@ -1085,7 +1085,7 @@ def main():
         log("sweep on a 400 x 300 x 140 box")
         rec["ragged_box"] = bench_box(ctx, max(3, args.steps // 4), 1)
     if not args.no_cpu:
-        log("timing the CPU baseline (oracle, 1 thread)")
+        log("timing the CPU baseline (oracle: OpenMP and serial)")
         rec["cpu_baseline"], rec["parity_full_size"] = cpu_baseline(case, sweep)
     case.mesh.close()
     if not args.no_implicit:
