@@ -327,6 +327,9 @@ __global__ __launch_bounds__(kCgBlock) void kc_update(const CgGrid<S>* __restric
     }
     if (stop) return;
     const S* __restrict__ pvec = cg_pbuf(P, pb);
+    // (the other vectors are addressed through P: reading their addresses
+    // once, as the other kernels do, costs this kernel 6 VGPRs and a wave per
+    // SIMD in FP64, 2.5 % of a 128^3 solve)
     // rows are taken kCgU per thread at a time, their loads issued together;
     // the first group's loads go out before the reduction of p.q
     const int r1 = min(row1, P.N);
@@ -389,7 +392,13 @@ __global__ __launch_bounds__(kCgBlock) void kc_direction(const CgGrid<S>* __rest
     if ((MODE == 0 && done) || (MODE == 1 && done != 1)) return;
     // the old direction: the chunk's last buffer (mode 0) or that of the
     // system's last update (mode 1); the new one goes to the first buffer
-    const S* __restrict__ po = cg_pbuf(P, MODE == 1 ? st->pbuf : pOld);
+    const S* po = cg_pbuf(P, MODE == 1 ? st->pbuf : pOld);  // may be the new buffer
+    // the vector addresses are read from the system description once: read
+    // through P inside the row loop they would be re-read after every store
+    // (a store might change the description), each re-read waiting for the
+    // loads before it
+    const S* __restrict__ rvec = P.r;
+    S* pn = P.p;
     // the first kCgU rows' r (= z) and p are loaded before the reduction
     const int r1 = min(row1, P.N);
     const int i0 = row0 + (int)threadIdx.x;
@@ -399,7 +408,7 @@ __global__ __launch_bounds__(kCgBlock) void kc_direction(const CgGrid<S>* __rest
         for (int u = 0; u < kCgU; ++u) {
             const int i = base + u * kCgBlock;
             if (i < r1) {
-                zv[u] = P.r[i];
+                zv[u] = rvec[i];
                 pv[u] = po[i];
             }
         }
@@ -427,7 +436,6 @@ __global__ __launch_bounds__(kCgBlock) void kc_direction(const CgGrid<S>* __rest
     }
     if (MODE == 2 || stop != 0 || it == 0) return;
     const S beta = (S)(tot[1] / st->rhoPrev);
-    S* pn = P.p;
     for (int base = i0; base < r1; base += kCgU * kCgBlock) {
         if (base != i0) load(base);
 #pragma unroll
@@ -448,9 +456,12 @@ __global__ __launch_bounds__(kCgBlock) void kc_accum(const CgGrid<S>* __restrict
     const CgSys<S>& P = *Pp;
     if (!ALL && P.st->stop != 1) return;
     const int r1 = min(row1, P.N);
+    double* __restrict__ x = P.x;  // addresses read once (see kc_direction)
+    S* __restrict__ d = P.d;
+    const S* __restrict__ gv = P.g;
     for (int i = row0 + threadIdx.x; i < r1; i += kCgBlock) {
-        P.x[i] += (double)P.d[i] / fabs((double)P.g[i]);
-        P.d[i] = S(0);
+        x[i] += (double)d[i] / fabs((double)gv[i]);
+        d[i] = S(0);
     }
 }
 
@@ -484,11 +495,13 @@ __global__ __launch_bounds__(kCgBlock) void kc_resid(const CgGrid<S>* __restrict
     const int32_t* __restrict__ ci = P.colIdxs;
     const double* __restrict__ va = P.vals;
     const double* __restrict__ xv = P.x;
+    const double* __restrict__ rhs = P.rhs;
+    double* __restrict__ r64 = P.r64;
     double rr = 0.0, bb = 0.0;
     const int r1 = min(row1, P.N);
     for (int i = row0 + threadIdx.x; i < r1; i += kCgBlock) {
         const int k0 = ro[i], k1 = ro[i + 1];
-        const double bi = P.rhs[i];
+        const double bi = rhs[i];
         double v[kRE], xg[kRE];
         int c[kRE];
 #pragma unroll
@@ -506,7 +519,7 @@ __global__ __launch_bounds__(kCgBlock) void kc_resid(const CgGrid<S>* __restrict
             if (k0 + k < k1) s += v[k] * xg[k];
         for (int k = k0 + kRE; k < k1; ++k) s += va[k] * xv[ci[k]];
         const double ri = bi - s;
-        P.r64[i] = ri;
+        r64[i] = ri;
         rr += ri * ri;
         bb += bi * bi;
     }
@@ -548,11 +561,15 @@ __global__ __launch_bounds__(kCgBlock) void kc_replace(const CgGrid<S>* __restri
     }
     double rr = 0.0, rz = 0.0;
     const int r1 = min(row1, P.N);
+    const S* __restrict__ gv = P.g;  // addresses read once (see kc_direction)
+    const double* __restrict__ r64 = P.r64;
+    S* __restrict__ rv = P.r;
+    S* __restrict__ pv = P.p;
     for (int i = row0 + threadIdx.x; i < r1; i += kCgBlock) {
-        const double gi = (double)P.g[i];
-        const S ri = (S)(P.r64[i] / gi);
-        P.r[i] = ri;
-        if (START) P.p[i] = ri;
+        const double gi = (double)gv[i];
+        const S ri = (S)(r64[i] / gi);
+        rv[i] = ri;
+        if (START) pv[i] = ri;
         const double ur = (double)ri * gi;
         rr += ur * ur;
         rz += (double)ri * (double)ri;
